@@ -1,0 +1,199 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+The oracle (cpu_ref.hpp) restates the reference's CPU algorithm for the
+patch-AMR advection step.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module; the product path under
+gpu-amr_amd/ never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "liboracle.so"
+
+# Reference device metadata record (include/cuda/halo_exchange.hpp:11-26).
+HALO_META_DTYPE = np.dtype(
+    [
+        ("neighbor", "<i4"),
+        ("finer_ids", "<i4", (4,)),
+        ("quadrant", "<i4", (3,)),
+        ("relation", "i1"),
+        ("pad", "i1", (3,)),
+    ]
+)
+assert HALO_META_DTYPE.itemsize == 36
+
+_lib = None
+
+
+def load():
+    """Load (and if needed build) liboracle.so."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise RuntimeError(f"{_LIB_PATH} is missing: run `make -C oracle` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(str(_LIB_PATH))
+    vp, sz, dp, i8p, u64p = (
+        ctypes.c_void_p,
+        ctypes.c_size_t,
+        ctypes.POINTER(ctypes.c_double),
+        ctypes.POINTER(ctypes.c_int8),
+        ctypes.POINTER(ctypes.c_uint64),
+    )
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    lib.oracle_create.restype = vp
+    lib.oracle_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, dp]
+    lib.oracle_destroy.argtypes = [vp]
+    lib.oracle_size.restype = sz
+    lib.oracle_size.argtypes = [vp]
+    lib.oracle_keys.argtypes = [vp, u64p]
+    lib.oracle_metadata.argtypes = [vp, vp]
+    lib.oracle_reconstruct.argtypes = [vp, i8p, ctypes.POINTER(ctypes.c_int)]
+    lib.oracle_last_sets.argtypes = [vp, u64p, ctypes.POINTER(sz), u64p, ctypes.POINTER(sz)]
+    for name in ("oracle_set_interior", "oracle_get_interior", "oracle_set_padded", "oracle_get_padded", "oracle_cell_centers"):
+        getattr(lib, name).argtypes = [vp, dp]
+    lib.oracle_halo_exchange.argtypes = [vp]
+    lib.oracle_set_params.argtypes = [vp, dp, ctypes.c_double, ctypes.c_int]
+    lib.oracle_compute_dt.restype = ctypes.c_double
+    lib.oracle_compute_dt.argtypes = [vp]
+    lib.oracle_step.argtypes = [vp, ctypes.c_double]
+    lib.oracle_step_kernel_only.argtypes = [vp, ctypes.c_double]
+    lib.oracle_advance_batch.argtypes = [vp, sz, ctypes.c_double, dp, ctypes.POINTER(sz)]
+    lib.oracle_criterion.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, i8p]
+    _lib = lib
+    return lib
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class OracleTree:
+    """The reference's ndtree + amr_solver (advection) restated on the CPU."""
+
+    CRIT_MAX_VALUE = 0
+    CRIT_GRADIENT = 1
+
+    def __init__(self, rank, n, halo, depth, capacity, lengths=(1.0, 1.0, 1.0)):
+        self.lib = load()
+        self.rank, self.n, self.halo, self.depth = rank, n, halo, depth
+        self.ni = n**rank
+        self.pad = n + 2 * halo
+        self.flat = self.pad**rank
+        L = np.ascontiguousarray(np.array(list(lengths) + [1.0] * (3 - len(lengths)), dtype=np.float64)[:3])
+        self.h = self.lib.oracle_create(rank, n, halo, depth, capacity, _ptr(L, ctypes.c_double))
+        if not self.h:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.oracle_destroy(h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+
+    def size(self):
+        return int(self.lib.oracle_size(self.h))
+
+    def keys(self):
+        out = np.zeros(self.size(), dtype=np.uint64)
+        self._check(self.lib.oracle_keys(self.h, _ptr(out, ctypes.c_uint64)))
+        return out
+
+    def metadata(self):
+        out = np.zeros(self.size() * 2 * self.rank, dtype=HALO_META_DTYPE)
+        self._check(self.lib.oracle_metadata(self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def reconstruct(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int8)
+        assert f.shape == (self.size(),)
+        changed = ctypes.c_int(0)
+        self._check(self.lib.oracle_reconstruct(self.h, _ptr(f, ctypes.c_int8), ctypes.byref(changed)))
+        return bool(changed.value)
+
+    def refine_all(self):
+        return self.reconstruct(np.ones(self.size(), dtype=np.int8))
+
+    def last_sets(self):
+        nr, nc = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        self._check(self.lib.oracle_last_sets(self.h, None, ctypes.byref(nr), None, ctypes.byref(nc)))
+        r = np.zeros(nr.value, dtype=np.uint64)
+        c = np.zeros(nc.value, dtype=np.uint64)
+        self._check(self.lib.oracle_last_sets(self.h, _ptr(r, ctypes.c_uint64), ctypes.byref(nr), _ptr(c, ctypes.c_uint64), ctypes.byref(nc)))
+        return r, c
+
+    def set_interior(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        assert v.size == self.size() * self.ni
+        self._check(self.lib.oracle_set_interior(self.h, _ptr(v, ctypes.c_double)))
+
+    def get_interior(self):
+        out = np.zeros(self.size() * self.ni, dtype=np.float64)
+        self._check(self.lib.oracle_get_interior(self.h, _ptr(out, ctypes.c_double)))
+        return out.reshape(self.size(), *([self.n] * self.rank))
+
+    def set_padded(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        assert v.size == self.size() * self.flat
+        self._check(self.lib.oracle_set_padded(self.h, _ptr(v, ctypes.c_double)))
+
+    def get_padded(self):
+        out = np.zeros(self.size() * self.flat, dtype=np.float64)
+        self._check(self.lib.oracle_get_padded(self.h, _ptr(out, ctypes.c_double)))
+        return out.reshape(self.size(), *([self.pad] * self.rank))
+
+    def cell_centers(self):
+        out = np.zeros(self.size() * self.ni * self.rank, dtype=np.float64)
+        self._check(self.lib.oracle_cell_centers(self.h, _ptr(out, ctypes.c_double)))
+        return out.reshape(self.size(), self.ni, self.rank)
+
+    def halo_exchange(self):
+        self._check(self.lib.oracle_halo_exchange(self.h))
+
+    def set_params(self, velocity, cfl, threads=1):
+        v = np.zeros(3, dtype=np.float64)
+        v[: len(velocity)] = velocity
+        self._check(self.lib.oracle_set_params(self.h, _ptr(v, ctypes.c_double), float(cfl), int(threads)))
+
+    def compute_dt(self):
+        return float(self.lib.oracle_compute_dt(self.h))
+
+    def step(self, dt):
+        self._check(self.lib.oracle_step(self.h, float(dt)))
+
+    def step_kernel_only(self, dt):
+        self._check(self.lib.oracle_step_kernel_only(self.h, float(dt)))
+
+    def advance_batch(self, steps, remaining=np.finfo(np.float64).max):
+        s = ctypes.c_double(0.0)
+        ex = ctypes.c_size_t(0)
+        self._check(self.lib.oracle_advance_batch(self.h, steps, float(remaining), ctypes.byref(s), ctypes.byref(ex)))
+        return s.value, ex.value
+
+    def advance(self):
+        return self.advance_batch(1)[0]
+
+    def criterion(self, kind, refine_thr, coarsen_thr, min_level, max_level):
+        out = np.zeros(self.size(), dtype=np.int8)
+        self._check(self.lib.oracle_criterion(self.h, int(kind), float(refine_thr), float(coarsen_thr), int(min_level), int(max_level), _ptr(out, ctypes.c_int8)))
+        return out
+
+
+def build(verbose=False):
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+
+    r = subprocess.run(["make", "-s", "-C", str(_HERE)], capture_output=not verbose, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return str(_LIB_PATH)

@@ -1,0 +1,234 @@
+// cpu_ref_capi.cpp -- extern "C" face of the CPU oracle (test infrastructure
+// only; see cpu_ref.hpp).  Loaded with ctypes by tests/, smoke() and the
+// cpu_baseline leg of bench.py.
+#include "cpu_ref.hpp"
+
+#include <memory>
+#include <string>
+
+namespace
+{
+
+thread_local std::string g_err;
+
+struct iface
+{
+    virtual ~iface()                                                     = default;
+    virtual int         rank() const                                      = 0;
+    virtual int         interior_cells() const                            = 0;
+    virtual int         padded_cells() const                              = 0;
+    virtual std::size_t size() const                                      = 0;
+    virtual void        keys(std::uint64_t* out) const                    = 0;
+    virtual void        metadata(oracle::halo_meta* out) const            = 0;
+    virtual bool        reconstruct(std::int8_t const* flags)             = 0;
+    virtual void        last_sets(std::uint64_t* r, std::size_t* nr, std::uint64_t* c, std::size_t* nc) const = 0;
+    virtual void        set_interior(double const* v)                     = 0;
+    virtual void        get_interior(double* v) const                     = 0;
+    virtual void        set_padded(double const* v)                       = 0;
+    virtual void        get_padded(double* v) const                       = 0;
+    virtual void        cell_centers(double* v) const                     = 0;
+    virtual void        halo()                                            = 0;
+    virtual void        params(double const* vel, double cfl, int threads) = 0;
+    virtual double      dt() const                                        = 0;
+    virtual void        step(double dt)                                   = 0;
+    virtual void        step_kernel_only(double dt)                       = 0;
+    virtual double      advance(std::size_t steps, double rem, std::size_t* ex) = 0;
+    virtual void        criterion(int kind, double r, double c, int mn, int mx, std::int8_t* f) const = 0;
+};
+
+template <int R, int N, int H>
+struct impl final : iface
+{
+    using tree_t = oracle::tree<R, N, H>;
+    tree_t t;
+
+    impl(int depth, std::size_t cap, std::array<double, 3> L)
+        : t(depth, cap, L)
+    {
+    }
+
+    int         rank() const override { return R; }
+    int         interior_cells() const override { return tree_t::NI; }
+    int         padded_cells() const override { return tree_t::FLAT; }
+    std::size_t size() const override { return t.size(); }
+    void        keys(std::uint64_t* out) const override
+    {
+        for (std::size_t i = 0; i < t.size(); ++i) out[i] = t.key_at(i);
+    }
+    void metadata(oracle::halo_meta* out) const override { t.export_metadata(out); }
+    bool reconstruct(std::int8_t const* flags) override { return t.reconstruct_tree(flags); }
+    void last_sets(std::uint64_t* r, std::size_t* nr, std::uint64_t* c, std::size_t* nc) const override
+    {
+        auto const& a = t.last_refine_set();
+        auto const& b = t.last_coarsen_set();
+        if (r) std::copy(a.begin(), a.end(), r);
+        if (c) std::copy(b.begin(), b.end(), c);
+        if (nr) *nr = a.size();
+        if (nc) *nc = b.size();
+    }
+    void set_interior(double const* v) override
+    {
+        for (std::size_t i = 0; i < t.size(); ++i) t.set_interior(i, v + i * tree_t::NI);
+    }
+    void get_interior(double* v) const override
+    {
+        for (std::size_t i = 0; i < t.size(); ++i) t.get_interior(i, v + i * tree_t::NI);
+    }
+    void set_padded(double const* v) override
+    {
+        for (std::size_t i = 0; i < t.size(); ++i) std::copy_n(v + i * tree_t::FLAT, tree_t::FLAT, t.patch(i));
+    }
+    void get_padded(double* v) const override
+    {
+        for (std::size_t i = 0; i < t.size(); ++i) std::copy_n(t.patch(i), tree_t::FLAT, v + i * tree_t::FLAT);
+    }
+    void cell_centers(double* v) const override
+    {
+        for (std::size_t i = 0; i < t.size(); ++i) t.cell_centers(i, v + i * tree_t::NI * R);
+    }
+    void   halo() override { t.halo_exchange_update(); }
+    void   params(double const* vel, double cfl, int threads) override
+    {
+        t.set_velocity({ vel[0], vel[1], vel[2] });
+        t.set_cfl(cfl);
+        t.set_threads(threads);
+    }
+    double dt() const override { return t.compute_time_step(); }
+    void   step(double dt) override { t.time_step(dt); }
+    void   step_kernel_only(double dt) override
+    {
+        t.time_step_kernel(dt);
+        t.swap_buffers();
+    }
+    double advance(std::size_t steps, double rem, std::size_t* ex) override { return t.advance_batch(steps, rem, ex); }
+    void   criterion(int kind, double r, double c, int mn, int mx, std::int8_t* f) const override
+    {
+        if (kind == 0)
+            t.criterion_max_value(r, c, mn, mx, f);
+        else
+            t.criterion_gradient(r, c, mn, mx, f);
+    }
+};
+
+iface* make(int rank, int n, int h, int depth, std::size_t cap, std::array<double, 3> L)
+{
+#define ORACLE_CASE(R_, N_, H_) \
+    if (rank == R_ && n == N_ && h == H_) return new impl<R_, N_, H_>(depth, cap, L);
+    ORACLE_CASE(2, 4, 1)
+    ORACLE_CASE(2, 4, 2)
+    ORACLE_CASE(2, 8, 1)
+    ORACLE_CASE(2, 16, 1)
+    ORACLE_CASE(2, 10, 2)
+    ORACLE_CASE(2, 50, 2)
+    ORACLE_CASE(3, 4, 1)
+    ORACLE_CASE(3, 8, 1)
+#undef ORACLE_CASE
+    throw std::invalid_argument("no oracle instance for this (rank, patch size, halo)");
+}
+
+template <typename Fn>
+int guarded(Fn&& fn)
+{
+    try
+    {
+        fn();
+        return 0;
+    }
+    catch (std::exception const& e)
+    {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+} // namespace
+
+extern "C"
+{
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+void* oracle_create(int rank, int n, int h, int depth, std::size_t capacity, double const* lengths)
+{
+    iface* p = nullptr;
+    guarded([&] { p = make(rank, n, h, depth, capacity, { lengths[0], lengths[1], lengths[2] }); });
+    return p;
+}
+
+void oracle_destroy(void* o) { delete static_cast<iface*>(o); }
+
+std::size_t oracle_size(void* o) { return static_cast<iface*>(o)->size(); }
+
+int oracle_keys(void* o, std::uint64_t* out)
+{
+    return guarded([&] { static_cast<iface*>(o)->keys(out); });
+}
+
+int oracle_metadata(void* o, void* out)
+{
+    return guarded([&] { static_cast<iface*>(o)->metadata(static_cast<oracle::halo_meta*>(out)); });
+}
+
+int oracle_reconstruct(void* o, std::int8_t const* flags, int* changed)
+{
+    return guarded(
+        [&]
+        {
+            const bool c = static_cast<iface*>(o)->reconstruct(flags);
+            if (changed) *changed = c ? 1 : 0;
+        }
+    );
+}
+
+int oracle_last_sets(void* o, std::uint64_t* r, std::size_t* nr, std::uint64_t* c, std::size_t* nc)
+{
+    return guarded([&] { static_cast<iface*>(o)->last_sets(r, nr, c, nc); });
+}
+
+int oracle_set_interior(void* o, double const* v)
+{
+    return guarded([&] { static_cast<iface*>(o)->set_interior(v); });
+}
+int oracle_get_interior(void* o, double* v)
+{
+    return guarded([&] { static_cast<iface*>(o)->get_interior(v); });
+}
+int oracle_set_padded(void* o, double const* v)
+{
+    return guarded([&] { static_cast<iface*>(o)->set_padded(v); });
+}
+int oracle_get_padded(void* o, double* v)
+{
+    return guarded([&] { static_cast<iface*>(o)->get_padded(v); });
+}
+int oracle_cell_centers(void* o, double* v)
+{
+    return guarded([&] { static_cast<iface*>(o)->cell_centers(v); });
+}
+int oracle_halo_exchange(void* o)
+{
+    return guarded([&] { static_cast<iface*>(o)->halo(); });
+}
+int oracle_set_params(void* o, double const* vel, double cfl, int threads)
+{
+    return guarded([&] { static_cast<iface*>(o)->params(vel, cfl, threads); });
+}
+double oracle_compute_dt(void* o) { return static_cast<iface*>(o)->dt(); }
+int    oracle_step(void* o, double dt)
+{
+    return guarded([&] { static_cast<iface*>(o)->step(dt); });
+}
+int oracle_step_kernel_only(void* o, double dt)
+{
+    return guarded([&] { static_cast<iface*>(o)->step_kernel_only(dt); });
+}
+int oracle_advance_batch(void* o, std::size_t steps, double remaining, double* dt_sum, std::size_t* executed)
+{
+    return guarded([&] { *dt_sum = static_cast<iface*>(o)->advance(steps, remaining, executed); });
+}
+int oracle_criterion(void* o, int kind, double r, double c, int mn, int mx, std::int8_t* flags)
+{
+    return guarded([&] { static_cast<iface*>(o)->criterion(kind, r, c, mn, mx, flags); });
+}
+
+} // extern "C"
