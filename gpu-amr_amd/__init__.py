@@ -1,0 +1,313 @@
+"""gpu-amr_amd: MI355X-native patch-AMR advection step (Python face).
+
+A thin ctypes binding of libamr_hip.so (C ABI in include/amr_hip.h) that
+mirrors the reference's ndtree / amr_solver interface
+(include/ndtree/ndtree.hpp, include/solver/amr_solver.hpp).  There is no CPU
+fallback: every call runs the HIP kernels, and importing the package fails
+loudly when the shared library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libamr_hip.so"
+
+AMR_OK, AMR_ERR_HIP, AMR_ERR_CAPACITY, AMR_ERR_ARGUMENT, AMR_ERR_UNSUPPORTED, AMR_ERR_TOPOLOGY = range(6)
+CRITERION_MAX_VALUE, CRITERION_GRADIENT = 0, 1
+STABLE, REFINE, COARSEN = 0, 1, 2
+
+# amr_halo_direction_metadata (include/cuda/halo_exchange.hpp:11-26 layout)
+HALO_META_DTYPE = np.dtype(
+    [
+        ("neighbor", "<i4"),
+        ("finer_ids", "<i4", (4,)),
+        ("quadrant", "<i4", (3,)),
+        ("relation", "i1"),
+        ("pad", "i1", (3,)),
+    ]
+)
+
+
+class AmrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class TreeConfig(ctypes.Structure):
+    _fields_ = [
+        ("rank", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("patch_size", ctypes.c_int32),
+        ("halo_width", ctypes.c_int32),
+        ("capacity", ctypes.c_uint64),
+        ("domain_length", ctypes.c_double * 3),
+    ]
+
+
+class Criterion(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("min_level", ctypes.c_int32),
+        ("max_level", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("refine_threshold", ctypes.c_double),
+        ("coarsen_threshold", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+_SIGS = {
+    "amr_tree_create": [ctypes.POINTER(TreeConfig), ctypes.POINTER(ctypes.c_void_p)],
+    "amr_tree_destroy": [ctypes.c_void_p],
+    "amr_tree_size": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)],
+    "amr_tree_get_keys": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tree_get_neighbor_metadata": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tree_reconstruct_flags": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)],
+    "amr_tree_reconstruct_criterion": [ctypes.c_void_p, ctypes.POINTER(Criterion), ctypes.POINTER(ctypes.c_int)],
+    "amr_tree_compute_flags": [ctypes.c_void_p, ctypes.POINTER(Criterion), ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tree_set_interior": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tree_get_interior": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tree_get_patches": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tree_halo_exchange_update": [ctypes.c_void_p],
+    "amr_tree_swap_buffers": [ctypes.c_void_p],
+    "amr_tree_device_interior": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
+    "amr_tree_stream": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
+    "amr_solver_configure": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_double],
+    "amr_solver_compute_dt": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)],
+    "amr_solver_advance_batch_async": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double],
+    "amr_solver_finish_advance_batch": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)],
+    "amr_solver_advance": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)],
+    "amr_ctx_synchronize": [ctypes.c_void_p],
+}
+
+
+def exported_symbols():
+    """The entry points include/amr_hip.h declares."""
+    return ["amr_last_error", *_SIGS]
+
+
+def load():
+    """Load libamr_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or gpu-amr_amd/build.py")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.amr_last_error.restype = ctypes.c_char_p
+    lib.amr_last_error.argtypes = []
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != AMR_OK:
+        raise AmrError(rc, load().amr_last_error().decode())
+
+
+def _vp(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def decode_keys(keys, rank):
+    """Morton keys -> (coords[n, rank] at 2**depth resolution, level[n])."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    m = keys >> np.uint64(6)
+    lvl = (keys & np.uint64(63)).astype(np.int64)
+    c = np.zeros((keys.size, rank), dtype=np.int64)
+    for b in range(21):
+        for i in range(rank):
+            c[:, i] |= ((m >> np.uint64(rank * b + i)) & np.uint64(1)).astype(np.int64) << b
+    return c, lvl
+
+
+class NdTree:
+    """Periodic quad-/octree of leaf patches on one GPU (ndtree.hpp:94-2157)."""
+
+    def __init__(self, rank, max_depth, patch_size, halo_width, capacity, domain_lengths=(1.0, 1.0, 1.0)):
+        self.lib = load()
+        cfg = TreeConfig()
+        cfg.rank, cfg.max_depth, cfg.patch_size, cfg.halo_width = rank, max_depth, patch_size, halo_width
+        cfg.capacity = capacity
+        L = list(domain_lengths) + [1.0] * (3 - len(domain_lengths))
+        for i in range(3):
+            cfg.domain_length[i] = L[i]
+        h = ctypes.c_void_p()
+        _check(self.lib.amr_tree_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.rank, self.depth, self.n, self.halo = rank, max_depth, patch_size, halo_width
+        self.lengths = tuple(L[:rank])
+        self.ni = patch_size**rank
+        self.pad = patch_size + 2 * halo_width
+        self.flat = self.pad**rank
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.amr_tree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- topology ---------------------------------------------------------
+    def size(self):
+        n = ctypes.c_uint64()
+        _check(self.lib.amr_tree_size(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def keys(self):
+        out = np.zeros(self.size(), dtype=np.uint64)
+        _check(self.lib.amr_tree_get_keys(self.h, _vp(out), out.size))
+        return out
+
+    def get_node_index_at(self, i):
+        return int(self.keys()[i])
+
+    def neighbor_metadata(self):
+        out = np.zeros(self.size() * 2 * self.rank, dtype=HALO_META_DTYPE)
+        _check(self.lib.amr_tree_get_neighbor_metadata(self.h, _vp(out), out.size))
+        return out
+
+    def reconstruct_tree(self, fn):
+        """reconstruct_tree(fn) (ndtree.hpp:1249-1271).
+
+        fn is either a Criterion (evaluated on the device), an array of
+        per-leaf flags, or a callable key -> flag evaluated per leaf.
+        Returns True when the topology changed."""
+        changed = ctypes.c_int(0)
+        if isinstance(fn, Criterion):
+            _check(self.lib.amr_tree_reconstruct_criterion(self.h, ctypes.byref(fn), ctypes.byref(changed)))
+            return bool(changed.value)
+        if callable(fn):
+            flags = np.array([int(fn(int(k))) for k in self.keys()], dtype=np.int8)
+        else:
+            flags = np.ascontiguousarray(fn, dtype=np.int8)
+        _check(self.lib.amr_tree_reconstruct_flags(self.h, _vp(flags), flags.size, ctypes.byref(changed)))
+        return bool(changed.value)
+
+    def refine_all(self):
+        return self.reconstruct_tree(np.full(self.size(), REFINE, dtype=np.int8))
+
+    def compute_flags(self, crit: Criterion):
+        out = np.zeros(self.size(), dtype=np.int8)
+        _check(self.lib.amr_tree_compute_flags(self.h, ctypes.byref(crit), _vp(out), out.size))
+        return out
+
+    # -- fields -----------------------------------------------------------
+    def set_interior(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        _check(self.lib.amr_tree_set_interior(self.h, _vp(v), v.size))
+
+    def get_interior(self):
+        out = np.zeros(self.size() * self.ni, dtype=np.float64)
+        _check(self.lib.amr_tree_get_interior(self.h, _vp(out), out.size))
+        return out.reshape((self.size(),) + (self.n,) * self.rank)
+
+    def get_patches(self):
+        """Padded patches after halo_exchange_update (corners 0)."""
+        out = np.zeros(self.size() * self.flat, dtype=np.float64)
+        _check(self.lib.amr_tree_get_patches(self.h, _vp(out), out.size))
+        return out.reshape((self.size(),) + (self.pad,) * self.rank)
+
+    def get_patch(self, i):
+        return self.get_patches()[i]
+
+    def halo_exchange_update(self):
+        _check(self.lib.amr_tree_halo_exchange_update(self.h))
+
+    def swap_buffers(self):
+        _check(self.lib.amr_tree_swap_buffers(self.h))
+
+    def device_interior_ptr(self):
+        p = ctypes.c_void_p()
+        _check(self.lib.amr_tree_device_interior(self.h, ctypes.byref(p)))
+        return p.value
+
+    def stream(self):
+        p = ctypes.c_void_p()
+        _check(self.lib.amr_tree_stream(self.h, ctypes.byref(p)))
+        return p.value
+
+    def synchronize(self):
+        _check(self.lib.amr_ctx_synchronize(self.h))
+
+    # -- geometry (physics_system.hpp:150-231) -----------------------------
+    def cell_sizes(self, levels):
+        lv = np.asarray(levels, dtype=np.int64)
+        out = np.zeros((lv.size, self.rank))
+        maxc = float(1 << self.depth)
+        for i in range(self.rank):
+            pm = np.array([float(1 << (self.depth - int(x))) for x in lv])
+            out[:, i] = (self.lengths[i] * pm / maxc) / float(self.n)
+        return out
+
+    def cell_centers(self):
+        """Cell centres of every interior cell: [size, n^rank, rank] (x first)."""
+        coords, lv = decode_keys(self.keys(), self.rank)
+        maxc = float(1 << self.depth)
+        cs = self.cell_sizes(lv)
+        idx = np.indices((self.n,) * self.rank).reshape(self.rank, -1)  # layout order
+        out = np.zeros((coords.shape[0], self.ni, self.rank))
+        for i in range(self.rank):
+            origin = self.lengths[i] * coords[:, i].astype(np.float64) / maxc
+            data_idx = idx[self.rank - 1 - i].astype(np.float64)
+            cell_origin = origin[:, None] + data_idx[None, :] * cs[:, i][:, None]
+            out[:, :, i] = cell_origin + 0.5 * cs[:, i][:, None]
+        return out
+
+
+class AmrSolver:
+    """amr_solver<Tree, Geometry, Advection, DIM> (amr_solver.hpp:28-544)."""
+
+    def __init__(self, tree: NdTree, velocity=(1.0, 0.5, 0.0), cfl=0.4):
+        self.tree = tree
+        self.configure(velocity, cfl)
+
+    def configure(self, velocity, cfl):
+        v = (ctypes.c_double * 3)(*(list(velocity) + [0.0] * (3 - len(velocity))))
+        _check(self.tree.lib.amr_solver_configure(self.tree.h, v, float(cfl)))
+        self.velocity, self.cfl = tuple(velocity), cfl
+
+    def get_tree(self):
+        return self.tree
+
+    def initialize(self, ic):
+        """initialize(ic) (amr_solver.hpp:105-145): ic(coords[..., rank]) -> values."""
+        cen = self.tree.cell_centers()
+        self.tree.set_interior(np.asarray(ic(cen), dtype=np.float64))
+
+    def compute_dt(self):
+        dt = ctypes.c_double()
+        _check(self.tree.lib.amr_solver_compute_dt(self.tree.h, ctypes.byref(dt)))
+        return dt.value
+
+    def advance_batch_async(self, step_count, remaining_time=np.finfo(np.float64).max):
+        _check(self.tree.lib.amr_solver_advance_batch_async(self.tree.h, int(step_count), float(remaining_time)))
+
+    def finish_advance_batch(self):
+        s, n = ctypes.c_double(), ctypes.c_uint64()
+        _check(self.tree.lib.amr_solver_finish_advance_batch(self.tree.h, ctypes.byref(s), ctypes.byref(n)))
+        return s.value, int(n.value)
+
+    def advance(self):
+        dt = ctypes.c_double()
+        _check(self.tree.lib.amr_solver_advance(self.tree.h, ctypes.byref(dt)))
+        return dt.value
+
+
+def make_criterion(kind, refine_threshold, coarsen_threshold, min_level, max_level):
+    c = Criterion()
+    c.kind, c.min_level, c.max_level = kind, min_level, max_level
+    c.refine_threshold, c.coarsen_threshold = refine_threshold, coarsen_threshold
+    return c

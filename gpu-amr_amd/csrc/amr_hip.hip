@@ -1,0 +1,703 @@
+// amr_hip.hip -- the amr_ctx context and the C ABI declared in amr_hip.h.
+//
+// The context owns the device state of one periodic tree (Morton keys,
+// neighbour table, two compact interior buffers, regrid scratch) and one HIP
+// stream.  Host code here only sequences launches and forms per-step
+// constants; all per-cell and per-leaf work runs in the kernels of
+// kernels_step.hpp / kernels_tree.hpp.
+#include "amr_hip.h"
+#include "device_common.hpp"
+#include "kernels_step.hpp"
+#include "kernels_tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace amrhip;
+
+namespace
+{
+
+thread_local std::string g_err;
+
+struct amr_error : std::runtime_error
+{
+    int code;
+    amr_error(int c, std::string const& m)
+        : std::runtime_error(m), code(c)
+    {
+    }
+};
+
+#define AMR_HIP_CHECK(expr)                                                                       \
+    do                                                                                            \
+    {                                                                                             \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw amr_error(AMR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+inline void check_launch(char const* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw amr_error(AMR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr int block_threads(int cells)
+{
+    int t = 64;
+    while (t < cells && t < 256) t *= 2;
+    return t;
+}
+
+// Kernel table of one (rank, patch size, halo width) instance.
+struct ops
+{
+    int rank, n, h;
+    void (*step)(const double*, double*, const uint64_t*, const face_link*, const uint32_t*, step_params const&,
+                 uint32_t, hipStream_t);
+    void (*padded)(const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
+    void (*criterion)(int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double, double,
+                      int, int, int8_t*, uint32_t, hipStream_t);
+    void (*emit)(const uint64_t*, const double*, uint32_t, int, const uint8_t*, const uint8_t*, const uint32_t*,
+                 const uint32_t*, uint64_t*, double*, regrid_summary*, hipStream_t);
+};
+
+template <int R, int N, int H>
+struct instance
+{
+    static constexpr int NT  = block_threads(patch_shape<R, N>::NI);
+    static constexpr int P   = N + 2 * H;
+    static constexpr int NTP = block_threads((R == 2) ? P * P : P * P * P);
+
+    static void step(const double* u, double* un, const uint64_t* keys, const face_link* links,
+                     const uint32_t* metas, step_params const& prm, uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_step<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, keys, links, metas, prm, n);
+        check_launch("k_step");
+    }
+    static void padded(const double* u, double* pad, const face_link* links, const uint32_t* metas, uint32_t n,
+                       hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_padded_view<R, N, H, NTP>), dim3(n), dim3(NTP), 0, s, u, pad, links, metas, n);
+        check_launch("k_padded_view");
+    }
+    static void criterion(int kind, const double* u, const uint64_t* keys, const face_link* links,
+                          const uint32_t* metas, double r, double c, int mn, int mx, int8_t* flags, uint32_t n,
+                          hipStream_t s)
+    {
+        if (n == 0) return;
+        if (kind == AMR_CRITERION_GRADIENT)
+            hipLaunchKernelGGL((k_criterion_gradient<R, N, NT>), dim3(n), dim3(NT), 0, s, u, keys, links, metas, r, c,
+                               mn, mx, flags, n);
+        else
+            hipLaunchKernelGGL((k_criterion_max<R, N, H, NTP>), dim3(n), dim3(NTP), 0, s, u, keys, links, metas, r, c,
+                               mn, mx, flags, n);
+        check_launch("k_criterion");
+    }
+    static void emit(const uint64_t* keys, const double* u, uint32_t n, int depth, const uint8_t* mark,
+                     const uint8_t* head, const uint32_t* counts, const uint32_t* pos, uint64_t* keys_out,
+                     double* u_out, regrid_summary* summary, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_emit<R, N, NT>), dim3(n), dim3(NT), 0, s, keys, u, n, depth, mark, head, counts, pos,
+                           keys_out, u_out, summary);
+        check_launch("k_emit");
+    }
+    static constexpr ops table() { return ops{ R, N, H, &step, &padded, &criterion, &emit }; }
+};
+
+// Patch shapes built into this library: the configurations' 16^2 and 8^3
+// patches with one ghost layer, the reference test's 10^2 and 50^2 patches
+// with two, and small shapes for exhaustive parity tests.
+const ops kOps[] = {
+    instance<2, 16, 1>::table(), instance<3, 8, 1>::table(), instance<2, 10, 2>::table(),
+    instance<2, 50, 2>::table(), instance<2, 4, 1>::table(), instance<2, 4, 2>::table(),
+    instance<2, 8, 1>::table(),  instance<3, 4, 1>::table(),
+};
+
+ops const* find_ops(int rank, int n, int h)
+{
+    for (auto const& o : kOps)
+        if (o.rank == rank && o.n == n && o.h == h) return &o;
+    return nullptr;
+}
+
+template <typename Fn>
+int guarded(Fn&& fn)
+{
+    try
+    {
+        fn();
+        g_err.clear();
+        return AMR_OK;
+    }
+    catch (amr_error const& e)
+    {
+        g_err = e.what();
+        return e.code;
+    }
+    catch (std::bad_alloc const&)
+    {
+        g_err = "host allocation failed";
+        return AMR_ERR_CAPACITY;
+    }
+    catch (std::exception const& e)
+    {
+        g_err = e.what();
+        return AMR_ERR_ARGUMENT;
+    }
+}
+
+template <typename T>
+T* dev_alloc(size_t count)
+{
+    void* p = nullptr;
+    AMR_HIP_CHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+inline unsigned grid1d(uint64_t items, unsigned threads) { return static_cast<unsigned>((items + threads - 1) / threads); }
+
+} // namespace
+
+struct amr_ctx
+{
+    amr_tree_config cfg{};
+    ops const*      op = nullptr;
+    hipStream_t     stream{};
+    int             rank = 0, depth = 0, ni = 0, flat = 0, ndir = 0;
+    uint64_t        cap = 0;
+
+    // topology + state
+    uint64_t        n         = 0;
+    int             max_level = 0;
+    int             cur       = 0; // current interior buffer (flips every step and regrid)
+    int             kc        = 0; // current key buffer (flips every regrid)
+    double*         u[2]{};
+    uint64_t*       keys[2]{};
+    face_link*      links = nullptr;
+    uint32_t*       metas = nullptr;
+    double*         padded = nullptr; // lazily allocated padded view
+
+    // regrid scratch
+    int8_t*         flags = nullptr;
+    uint8_t*        mark = nullptr;
+    uint8_t*        head = nullptr;
+    uint32_t*       counts = nullptr;
+    uint32_t*       pos = nullptr;
+    uint32_t*       block_sums = nullptr;
+    regrid_summary* summary = nullptr;
+    regrid_summary* h_summary = nullptr; // pinned
+
+    // solver
+    double          vel[3] = { 1.0, 0.5, 0.0 };
+    double          cfl    = 0.4;
+    double          pending_dt    = 0.0;
+    uint64_t        pending_steps = 0;
+    bool            broken        = false;
+
+    ~amr_ctx()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : { (void*)u[0], (void*)u[1], (void*)keys[0], (void*)keys[1], (void*)links, (void*)metas,
+                         (void*)padded, (void*)flags, (void*)mark, (void*)head, (void*)counts, (void*)pos,
+                         (void*)block_sums, (void*)summary })
+            if (p) (void)hipFree(p);
+        if (h_summary) (void)hipHostFree(h_summary);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ---- geometry (include/solver/physics_system.hpp:150-178) ----------
+    double cell_size(int level, int axis) const
+    {
+        const double pm   = static_cast<double>(1u << (depth - level));
+        const double maxc = static_cast<double>(1u << depth);
+        const double ps   = cfg.domain_length[axis] * pm / maxc;
+        return ps / static_cast<double>(cfg.patch_size);
+    }
+
+    // compute_time_step_cpu (amr_solver.hpp:355-413): the per-cell minimum is
+    // attained on the finest level present.
+    double compute_dt() const
+    {
+        double dt = std::numeric_limits<double>::max();
+        for (int d = 0; d < rank; ++d)
+        {
+            const double speed = std::abs(vel[d]);
+            if (speed > 1e-12) dt = std::min(dt, cell_size(max_level, d) / speed);
+        }
+        return cfl * dt;
+    }
+
+    void sync() { AMR_HIP_CHECK(hipStreamSynchronize(stream)); }
+
+    void build_table()
+    {
+        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
+        const uint64_t items = n * static_cast<uint64_t>(ndir);
+        if (rank == 2)
+            hipLaunchKernelGGL(k_build_table<2>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[kc],
+                               static_cast<uint32_t>(n), depth, links, metas, summary);
+        else
+            hipLaunchKernelGGL(k_build_table<3>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[kc],
+                               static_cast<uint32_t>(n), depth, links, metas, summary);
+        check_launch("k_build_table");
+        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
+        sync();
+        if (h_summary->err & 1u)
+        {
+            // the table may hold -1 entries: no kernel may run on it again
+            broken = true;
+            throw amr_error(AMR_ERR_TOPOLOGY, "leaves do not tile the domain or violate 2:1 face balance");
+        }
+    }
+
+    void ensure_usable() const
+    {
+        if (broken) throw amr_error(AMR_ERR_TOPOLOGY, "context unusable after a topology error");
+    }
+
+    void step(double dt)
+    {
+        ensure_usable();
+        step_params prm{};
+        for (int d = 0; d < 3; ++d)
+        {
+            prm.vel[d] = vel[d];
+            prm.hs[d]  = 0.5 * std::abs(vel[d]);
+        }
+        for (int l = 0; l <= depth; ++l)
+            for (int d = 0; d < rank; ++d) prm.dtdx[l][d] = dt / cell_size(l, d);
+        op->step(u[cur], u[1 - cur], keys[kc], links, metas, prm, static_cast<uint32_t>(n), stream);
+        cur = 1 - cur;
+    }
+
+    void scan(uint32_t count)
+    {
+        const unsigned nb = grid1d(count, kScanBlock);
+        hipLaunchKernelGGL(k_scan_blocks, dim3(std::max(nb, 1u)), dim3(kScanThreads), 0, stream, counts, pos, count,
+                           block_sums);
+        check_launch("k_scan_blocks");
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, stream, block_sums, std::max(nb, 1u), summary);
+        check_launch("k_scan_sums");
+        hipLaunchKernelGGL(k_scan_add, dim3(grid1d(count, 256)), dim3(256), 0, stream, pos, count, block_sums);
+        check_launch("k_scan_add");
+    }
+
+    // ndtree::reconstruct_tree (ndtree.hpp:1249-1271) with the flags already
+    // in device memory.
+    bool regrid()
+    {
+        ensure_usable();
+        const uint32_t nn = static_cast<uint32_t>(n);
+        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
+        hipLaunchKernelGGL(k_mark_refine, dim3(grid1d(nn, 256)), dim3(256), 0, stream, flags, keys[kc], nn, depth, mark);
+        check_launch("k_mark_refine");
+        for (int l = std::min(max_level, depth); l >= 1; --l)
+        {
+            if (rank == 2)
+                hipLaunchKernelGGL(k_close_level<2>, dim3(grid1d(nn, 256)), dim3(256), 0, stream, keys[kc], nn, l,
+                                   links, metas, mark);
+            else
+                hipLaunchKernelGGL(k_close_level<3>, dim3(grid1d(nn, 256)), dim3(256), 0, stream, keys[kc], nn, l,
+                                   links, metas, mark);
+            check_launch("k_close_level");
+        }
+        if (rank == 2)
+        {
+            hipLaunchKernelGGL(k_coarsen_families<2>, dim3(grid1d(nn, 256)), dim3(256), 0, stream, flags, keys[kc], nn,
+                               depth, links, metas, mark, head, summary);
+            hipLaunchKernelGGL(k_emit_counts<2>, dim3(grid1d(nn, 256)), dim3(256), 0, stream, keys[kc], nn, depth,
+                               mark, head, counts, summary);
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_coarsen_families<3>, dim3(grid1d(nn, 256)), dim3(256), 0, stream, flags, keys[kc], nn,
+                               depth, links, metas, mark, head, summary);
+            hipLaunchKernelGGL(k_emit_counts<3>, dim3(grid1d(nn, 256)), dim3(256), 0, stream, keys[kc], nn, depth,
+                               mark, head, counts, summary);
+        }
+        check_launch("k_coarsen_families / k_emit_counts");
+        scan(nn);
+        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
+        sync();
+        const regrid_summary s = *h_summary;
+        if (s.err & 2u) throw amr_error(AMR_ERR_UNSUPPORTED, "coarsening to the root level is not supported");
+        if (s.n_refine == 0 && s.n_coarsen == 0) return false;
+        if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
+        op->emit(keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, stream);
+        cur = 1 - cur;
+        kc  = 1 - kc;
+        n   = s.total;
+        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
+        sync();
+        max_level = static_cast<int>(h_summary->max_level);
+        build_table();
+        return true;
+    }
+};
+
+extern "C"
+{
+
+const char* amr_last_error(void) { return g_err.c_str(); }
+
+int amr_tree_create(const amr_tree_config* cfg, amr_ctx** out)
+{
+    return guarded(
+        [&]
+        {
+            if (!cfg || !out) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *out = nullptr;
+            if (cfg->rank != 2 && cfg->rank != 3) throw amr_error(AMR_ERR_ARGUMENT, "rank must be 2 or 3");
+            if (cfg->max_depth < 0 || cfg->max_depth > kMaxDepth || cfg->rank * cfg->max_depth + 6 > 64)
+                throw amr_error(AMR_ERR_ARGUMENT, "max_depth out of range");
+            if (cfg->capacity < 1 || cfg->capacity > 0xFFFFFFFFull / 8)
+                throw amr_error(AMR_ERR_ARGUMENT, "capacity out of range");
+            ops const* op = find_ops(cfg->rank, cfg->patch_size, cfg->halo_width);
+            if (!op) throw amr_error(AMR_ERR_UNSUPPORTED, "no kernels built for this (rank, patch_size, halo_width)");
+            auto c   = std::make_unique<amr_ctx>();
+            c->cfg   = *cfg;
+            c->op    = op;
+            c->rank  = cfg->rank;
+            c->depth = cfg->max_depth;
+            c->ndir  = 2 * cfg->rank;
+            c->cap   = cfg->capacity;
+            c->ni    = cfg->rank == 2 ? cfg->patch_size * cfg->patch_size
+                                      : cfg->patch_size * cfg->patch_size * cfg->patch_size;
+            const int P = cfg->patch_size + 2 * cfg->halo_width;
+            c->flat     = cfg->rank == 2 ? P * P : P * P * P;
+            AMR_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            const size_t cap = cfg->capacity;
+            for (int b = 0; b < 2; ++b)
+            {
+                c->u[b]    = dev_alloc<double>(cap * c->ni);
+                c->keys[b] = dev_alloc<uint64_t>(cap);
+                AMR_HIP_CHECK(hipMemsetAsync(c->u[b], 0, cap * c->ni * sizeof(double), c->stream));
+            }
+            c->links      = dev_alloc<face_link>(cap * c->ndir);
+            c->metas      = dev_alloc<uint32_t>(cap * c->ndir);
+            c->flags      = dev_alloc<int8_t>(cap);
+            c->mark       = dev_alloc<uint8_t>(cap);
+            c->head       = dev_alloc<uint8_t>(cap);
+            c->counts     = dev_alloc<uint32_t>(cap);
+            c->pos        = dev_alloc<uint32_t>(cap);
+            c->block_sums = dev_alloc<uint32_t>(cap / kScanBlock + 2);
+            c->summary    = dev_alloc<regrid_summary>(1);
+            AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_summary), sizeof(regrid_summary), 0));
+            // the periodic root (ndtree.hpp:411-421)
+            const uint64_t root = 0;
+            AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
+            c->n         = 1;
+            c->max_level = 0;
+            c->build_table();
+            *out = c.release();
+        }
+    );
+}
+
+int amr_tree_destroy(amr_ctx* ctx)
+{
+    return guarded([&] { delete ctx; });
+}
+
+int amr_tree_size(const amr_ctx* ctx, uint64_t* n)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !n) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *n = ctx->n;
+        }
+    );
+}
+
+int amr_tree_get_keys(amr_ctx* ctx, uint64_t* host_keys, uint64_t n)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host_keys && n)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (n < ctx->n) throw amr_error(AMR_ERR_ARGUMENT, "key buffer smaller than the leaf count");
+            AMR_HIP_CHECK(hipMemcpyAsync(host_keys, ctx->keys[ctx->kc], ctx->n * sizeof(uint64_t),
+                                         hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+        }
+    );
+}
+
+int amr_tree_get_neighbor_metadata(amr_ctx* ctx, amr_halo_direction_metadata* host, uint64_t count)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            const uint64_t m = ctx->n * ctx->ndir;
+            if (count < m) throw amr_error(AMR_ERR_ARGUMENT, "metadata buffer too small");
+            std::vector<face_link> links(m);
+            std::vector<uint32_t>  metas(m);
+            AMR_HIP_CHECK(hipMemcpyAsync(links.data(), ctx->links, m * sizeof(face_link), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            AMR_HIP_CHECK(hipMemcpyAsync(metas.data(), ctx->metas, m * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            ctx->sync();
+            const int k = 1 << (ctx->rank - 1);
+            for (uint64_t e = 0; e < m; ++e)
+            {
+                amr_halo_direction_metadata r{};
+                r.neighbor = -1;
+                for (int j = 0; j < 4; ++j) r.finer_ids[j] = -1;
+                const uint32_t rel = rel_of(metas[e]);
+                r.relation         = static_cast<int8_t>(rel);
+                if (rel == REL_SAME) r.neighbor = links[e].ids[0];
+                if (rel == REL_FINER)
+                    for (int j = 0; j < k; ++j) r.finer_ids[j] = links[e].ids[j];
+                if (rel == REL_COARSER)
+                {
+                    r.neighbor = links[e].ids[0];
+                    for (int d = 0; d < ctx->rank; ++d) r.quadrant[d] = static_cast<int32_t>(quad_of(metas[e], d));
+                }
+                host[e] = r;
+            }
+        }
+    );
+}
+
+static void run_criterion(amr_ctx* ctx, amr_criterion const* crit)
+{
+    ctx->ensure_usable();
+    if (!crit) throw amr_error(AMR_ERR_ARGUMENT, "null criterion");
+    if (crit->kind != AMR_CRITERION_MAX_VALUE && crit->kind != AMR_CRITERION_GRADIENT)
+        throw amr_error(AMR_ERR_ARGUMENT, "unknown criterion kind");
+    ctx->op->criterion(crit->kind, ctx->u[ctx->cur], ctx->keys[ctx->kc], ctx->links, ctx->metas,
+                       crit->refine_threshold, crit->coarsen_threshold, crit->min_level, crit->max_level, ctx->flags,
+                       static_cast<uint32_t>(ctx->n), ctx->stream);
+}
+
+int amr_tree_compute_flags(amr_ctx* ctx, const amr_criterion* crit, int8_t* host_flags, uint64_t n)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host_flags && n)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (n < ctx->n) throw amr_error(AMR_ERR_ARGUMENT, "flag buffer smaller than the leaf count");
+            run_criterion(ctx, crit);
+            AMR_HIP_CHECK(hipMemcpyAsync(host_flags, ctx->flags, ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+        }
+    );
+}
+
+int amr_tree_reconstruct_flags(amr_ctx* ctx, const int8_t* host_flags, uint64_t n, int* changed)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host_flags && n)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (n != ctx->n) throw amr_error(AMR_ERR_ARGUMENT, "one flag per leaf is required");
+            AMR_HIP_CHECK(hipMemcpyAsync(ctx->flags, host_flags, n, hipMemcpyHostToDevice, ctx->stream));
+            const bool c = ctx->regrid();
+            if (changed) *changed = c ? 1 : 0;
+        }
+    );
+}
+
+int amr_tree_reconstruct_criterion(amr_ctx* ctx, const amr_criterion* crit, int* changed)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            run_criterion(ctx, crit);
+            const bool c = ctx->regrid();
+            if (changed) *changed = c ? 1 : 0;
+        }
+    );
+}
+
+int amr_tree_set_interior(amr_ctx* ctx, const double* host, uint64_t count)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (count != ctx->n * ctx->ni) throw amr_error(AMR_ERR_ARGUMENT, "count must be size * n^rank");
+            AMR_HIP_CHECK(hipMemcpyAsync(ctx->u[ctx->cur], host, count * sizeof(double), hipMemcpyHostToDevice,
+                                         ctx->stream));
+            ctx->sync();
+        }
+    );
+}
+
+int amr_tree_get_interior(amr_ctx* ctx, double* host, uint64_t count)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (count < ctx->n * ctx->ni) throw amr_error(AMR_ERR_ARGUMENT, "buffer smaller than size * n^rank");
+            AMR_HIP_CHECK(hipMemcpyAsync(host, ctx->u[ctx->cur], ctx->n * ctx->ni * sizeof(double),
+                                         hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+        }
+    );
+}
+
+static void materialize_padded(amr_ctx* ctx)
+{
+    ctx->ensure_usable();
+    if (!ctx->padded) ctx->padded = dev_alloc<double>(ctx->cap * ctx->flat);
+    ctx->op->padded(ctx->u[ctx->cur], ctx->padded, ctx->links, ctx->metas, static_cast<uint32_t>(ctx->n), ctx->stream);
+}
+
+int amr_tree_get_patches(amr_ctx* ctx, double* host, uint64_t count)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (count < ctx->n * ctx->flat) throw amr_error(AMR_ERR_ARGUMENT, "buffer smaller than size * (n+2h)^rank");
+            materialize_padded(ctx);
+            AMR_HIP_CHECK(hipMemcpyAsync(host, ctx->padded, ctx->n * ctx->flat * sizeof(double), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            ctx->sync();
+        }
+    );
+}
+
+int amr_tree_halo_exchange_update(amr_ctx* ctx)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            materialize_padded(ctx);
+        }
+    );
+}
+
+int amr_tree_swap_buffers(amr_ctx* ctx)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            ctx->cur = 1 - ctx->cur;
+        }
+    );
+}
+
+int amr_tree_device_interior(amr_ctx* ctx, double** dev_ptr)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !dev_ptr) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *dev_ptr = ctx->u[ctx->cur];
+        }
+    );
+}
+
+int amr_tree_stream(amr_ctx* ctx, void** hip_stream)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !hip_stream) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *hip_stream = reinterpret_cast<void*>(ctx->stream);
+        }
+    );
+}
+
+int amr_solver_configure(amr_ctx* ctx, const double velocity[3], double cfl)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !velocity) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (!(cfl > 0.0)) throw amr_error(AMR_ERR_ARGUMENT, "cfl must be positive");
+            for (int d = 0; d < 3; ++d) ctx->vel[d] = velocity[d];
+            ctx->cfl = cfl;
+        }
+    );
+}
+
+int amr_solver_compute_dt(amr_ctx* ctx, double* dt)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !dt) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *dt = ctx->compute_dt();
+        }
+    );
+}
+
+// advance_batch_async, CPU branch semantics (amr_solver.hpp:221-241): dt is
+// known on the host for constant-velocity advection, so only the step
+// kernels are queued.
+int amr_solver_advance_batch_async(amr_ctx* ctx, uint64_t step_count, double remaining_time)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            ctx->pending_dt    = 0.0;
+            ctx->pending_steps = 0;
+            double remaining   = remaining_time;
+            for (uint64_t s = 0; s < step_count; ++s)
+            {
+                if (remaining <= 0.0) break;
+                const double dt      = ctx->compute_dt();
+                const double step_dt = std::min(dt, remaining);
+                if (step_dt <= 0.0) break;
+                ctx->step(step_dt);
+                ctx->pending_dt += step_dt;
+                remaining -= step_dt;
+                ++ctx->pending_steps;
+            }
+        }
+    );
+}
+
+int amr_solver_finish_advance_batch(amr_ctx* ctx, double* dt_sum, uint64_t* executed_steps)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            ctx->sync();
+            if (dt_sum) *dt_sum = ctx->pending_dt;
+            if (executed_steps) *executed_steps = ctx->pending_steps;
+        }
+    );
+}
+
+int amr_solver_advance(amr_ctx* ctx, double* dt)
+{
+    int rc = amr_solver_advance_batch_async(ctx, 1, std::numeric_limits<double>::max());
+    if (rc != AMR_OK) return rc;
+    return amr_solver_finish_advance_batch(ctx, dt, nullptr);
+}
+
+int amr_ctx_synchronize(amr_ctx* ctx)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            ctx->sync();
+        }
+    );
+}
+
+} // extern "C"

@@ -1,0 +1,175 @@
+/*
+ * amr_hip.h -- C ABI of the MI355X-native patch-AMR advection step.
+ *
+ * One opaque context (amr_ctx) owns one periodic quad-/octree of leaf patches,
+ * all of its device memory and one HIP stream.  Every entry point returns an
+ * int status (AMR_OK on success) and never throws; amr_last_error() gives the
+ * message of the last failure on the calling thread.  Arguments are plain
+ * pointers and sizes; host arrays are owned by the caller.
+ *
+ * Each entry point names the reference interface it replaces
+ * (bgce-cse/gpu-amr, paths relative to the reference root).
+ *
+ * Data conventions (the reference's):
+ *   - key = (interleave(x, y[, z]) << 6) | level, x in bit 0 of each group
+ *     (include/morton/morton_id.hpp:132-140, :350-359); leaves are kept
+ *     sorted by key (include/ndtree/ndtree.hpp:1335-1414).
+ *   - a patch is n^rank interior cells, row-major with the last index fastest;
+ *     physical x is the last index (include/containers/static_layout.hpp:28-37,
+ *     include/solver/physics_system.hpp:126-135).  The padded view adds
+ *     halo_width ghost layers per side ((n + 2h)^rank doubles per patch).
+ *   - direction k addresses layout dim k/2, negative when k is even
+ *     (include/ndtree/neighbor.hpp:156-211).
+ */
+#ifndef AMR_HIP_H
+#define AMR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference throws std::runtime_error on device errors
+ * (src/cuda/device_buffer.cu:17-27) and leaves capacity unchecked
+ * (include/ndtree/ndtree.hpp:1304-1316); here both become codes. */
+enum amr_status
+{
+    AMR_OK              = 0,
+    AMR_ERR_HIP         = 1, /* a HIP runtime call failed                      */
+    AMR_ERR_CAPACITY    = 2, /* the leaf count would exceed the capacity        */
+    AMR_ERR_ARGUMENT    = 3, /* bad pointer / size / configuration              */
+    AMR_ERR_UNSUPPORTED = 4, /* configuration or operation not provided         */
+    AMR_ERR_TOPOLOGY    = 5  /* leaves do not tile the domain or break 2:1      */
+};
+
+typedef struct amr_ctx amr_ctx;
+
+/* Template parameters of ndtree<Cell, morton_id<Depth, Rank>,
+ * patch_layout<static_layout<n...>, Halo>, linear_interpolator> and
+ * physics_system<..., Domain_Sizes> (include/ndtree/ndtree.hpp:94-99,
+ * include/solver/physics_system.hpp:106-110), as run-time values. */
+typedef struct amr_tree_config
+{
+    int32_t  rank;             /* 2 or 3                                        */
+    int32_t  max_depth;        /* Morton depth (7 in 2D, 5 in 3D in the configs) */
+    int32_t  patch_size;       /* interior cells per dimension                  */
+    int32_t  halo_width;       /* ghost layers of the padded view (1 or 2)      */
+    uint64_t capacity;         /* maximum number of leaf patches                */
+    double   domain_length[3]; /* physical lengths of x, y, z                   */
+} amr_tree_config;
+
+/* One (leaf, direction) neighbour record in the reference's flattened form
+ * (include/cuda/halo_exchange.hpp:11-26): relation 0 none, 1 same, 2 finer,
+ * 3 coarser; finer ids ordered little-endian over the non-normal layout
+ * dims; quadrant = contact quadrant in the coarser leaf, per layout dim. */
+typedef struct amr_halo_direction_metadata
+{
+    int32_t neighbor;
+    int32_t finer_ids[4];
+    int32_t quadrant[3];
+    int8_t  relation;
+    int8_t  padding[3];
+} amr_halo_direction_metadata;
+
+/* Refinement criterion evaluated on the device (the fill_refine_flags form,
+ * include/ndtree/ndtree.hpp:889-894).  kind 0: maximum cell value over the
+ * interior and face halos, starting from 0 (examples/fvm_solver_advection.e.cpp:75-101,
+ * src/cuda/fvm_refinement_criterion.cu:27-67).  kind 1: maximum undivided
+ * gradient max(|u[c+s]-u[c]|, |u[c]-u[c-s]|) over interior cells and dims.
+ * Refine if value > refine_threshold and level < max_level, else Coarsen if
+ * value < coarsen_threshold and level > min_level, else Stable. */
+typedef struct amr_criterion
+{
+    int32_t kind;
+    int32_t min_level;
+    int32_t max_level;
+    int32_t reserved;
+    double  refine_threshold;
+    double  coarsen_threshold;
+} amr_criterion;
+
+enum amr_criterion_kind
+{
+    AMR_CRITERION_MAX_VALUE = 0,
+    AMR_CRITERION_GRADIENT  = 1
+};
+
+/* Message of the last failed call on this thread (empty if none). */
+const char* amr_last_error(void);
+
+/* ---- tree (include/ndtree/ndtree.hpp) ---------------------------------- */
+
+/* ndtree(capacity): allocates the capacity and appends the periodic root
+ * (ndtree.hpp:320-425). */
+int amr_tree_create(const amr_tree_config* cfg, amr_ctx** out);
+/* ~ndtree (ndtree.hpp:427-468). */
+int amr_tree_destroy(amr_ctx* ctx);
+/* ndtree::size (ndtree.hpp:471-475). */
+int amr_tree_size(const amr_ctx* ctx, uint64_t* n);
+/* ndtree::get_node_index_at for every leaf, in linear order
+ * (ndtree.hpp:1290-1295). */
+int amr_tree_get_keys(amr_ctx* ctx, uint64_t* host_keys, uint64_t n);
+/* ndtree::rebuild_halo_exchange_metadata + device copy-out
+ * (ndtree.hpp:1606-1660); count = size * 2 * rank records. */
+int amr_tree_get_neighbor_metadata(amr_ctx* ctx, amr_halo_direction_metadata* host, uint64_t count);
+/* ndtree::reconstruct_tree(fn) with the per-leaf functor already evaluated
+ * into host_flags (0 Stable, 1 Refine, 2 Coarsen; ndtree.hpp:886-902,
+ * 1249-1271).  *changed is set to 1 when the topology changed. */
+int amr_tree_reconstruct_flags(amr_ctx* ctx, const int8_t* host_flags, uint64_t n, int* changed);
+/* ndtree::reconstruct_tree(criterion) with fill_refine_flags on the device
+ * (ndtree.hpp:889-894; src/cuda/fvm_refinement_criterion.cu:71-108). */
+int amr_tree_reconstruct_criterion(amr_ctx* ctx, const amr_criterion* crit, int* changed);
+/* The flags the criterion produces for the current leaves (copied out;
+ * the reference's sync_refine_status_from_device, ndtree.hpp:639-656). */
+int amr_tree_compute_flags(amr_ctx* ctx, const amr_criterion* crit, int8_t* host_flags, uint64_t n);
+
+/* ---- fields ------------------------------------------------------------ */
+
+/* Write / read the interiors of all leaves (size * n^rank doubles, leaf
+ * after leaf): the host side of amr_solver::initialize (amr_solver.hpp:105-145)
+ * and sync_current_to_device / sync_current_from_device (ndtree.hpp:583-591). */
+int amr_tree_set_interior(amr_ctx* ctx, const double* host, uint64_t count);
+int amr_tree_get_interior(amr_ctx* ctx, double* host, uint64_t count);
+/* Padded patches after ndtree::halo_exchange_update (ndtree.hpp:1862-1877):
+ * size * (n+2h)^rank doubles; corner cells are 0 (never written). */
+int amr_tree_get_patches(amr_ctx* ctx, double* host, uint64_t count);
+/* ndtree::halo_exchange_update (ndtree.hpp:1862-1877): fills the face halos
+ * of the device padded view.  Time steps gather ghosts from neighbour
+ * interiors directly, so stepping never needs this call. */
+int amr_tree_halo_exchange_update(amr_ctx* ctx);
+/* ndtree::swap_buffers (ndtree.hpp:1558-1579). */
+int amr_tree_swap_buffers(amr_ctx* ctx);
+/* Device pointer of the current interiors (size * n^rank doubles) and of the
+ * context's stream, for callers that share device memory (PyTorch). */
+int amr_tree_device_interior(amr_ctx* ctx, double** dev_ptr);
+int amr_tree_stream(amr_ctx* ctx, void** hip_stream);
+
+/* ---- solver (include/solver/amr_solver.hpp) ---------------------------- */
+
+/* amr_solver(capacity, gamma, cfl) parameters plus the advection velocity
+ * that AdvectionPhysics hard-codes (AdvectionPhysics.hpp:24,54,82). */
+int amr_solver_configure(amr_ctx* ctx, const double velocity[3], double cfl);
+/* compute_time_step_cpu / compute_dt_kernel: cfl * min dx_d / |v_d| over the
+ * leaves present (amr_solver.hpp:355-413). */
+int amr_solver_compute_dt(amr_ctx* ctx, double* dt);
+/* advance_batch_async(step_count, remaining_time) (amr_solver.hpp:155-243):
+ * enqueues up to step_count steps on the context stream, each clamped to the
+ * remaining time; returns without waiting. */
+int amr_solver_advance_batch_async(amr_ctx* ctx, uint64_t step_count, double remaining_time);
+/* finish_advance_batch (amr_solver.hpp:245-262): waits for the batch and
+ * returns the summed dt and the executed step count. */
+int amr_solver_finish_advance_batch(amr_ctx* ctx, double* dt_sum, uint64_t* executed_steps);
+/* advance() = advance_batch_async(1) + finish_advance_batch
+ * (amr_solver.hpp:149-153). */
+int amr_solver_advance(amr_ctx* ctx, double* dt);
+
+/* Waits for all work queued on the context stream. */
+int amr_ctx_synchronize(amr_ctx* ctx);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* AMR_HIP_H */

@@ -1,0 +1,211 @@
+// kernel_emulation.hip -- runs the HIP kernels' per-item bodies on the CPU
+// (host-only build), one "workgroup" after another, so that the device logic
+// can be compared with the oracle without a GPU.  Test infrastructure only.
+#include "device_common.hpp"
+#include "kernels_step.hpp"
+#include "kernels_tree.hpp"
+
+#include <cstring>
+#include <vector>
+
+using namespace amrhip;
+
+namespace
+{
+
+template <int R>
+int build_table(int depth, const uint64_t* keys, uint32_t n, face_link* links, uint32_t* metas)
+{
+    int err = 0;
+    for (uint32_t idx = 0; idx < n * 2 * R; ++idx)
+        if (!table_entry<R>(keys, n, depth, idx / (2 * R), static_cast<int>(idx % (2 * R)), &links[idx], &metas[idx]))
+            err = 1;
+    return err;
+}
+
+template <int R, int N>
+void step(const uint64_t* keys, uint32_t n, const face_link* links, const uint32_t* metas, const double* u, double* un,
+          const double* dtdx_tab, const double* vel, const double* hs)
+{
+    using S = patch_shape<R, N>;
+    std::vector<double> tile(tile_shape<R, N>::TS, 0.0);
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+    {
+        for (int i = 0; i < S::NI; ++i) stage_interior<R, N>(tile.data(), u + static_cast<size_t>(leaf) * S::NI, i);
+        for (int g = 0; g < S::NDIR * S::FACE; ++g) stage_ghost<R, N>(tile.data(), u, leaf, links, metas, g);
+        const int lv = key_level(keys[leaf]);
+        double    dtdx[3];
+        for (int d = 0; d < R; ++d) dtdx[d] = dtdx_tab[lv * 3 + d];
+        for (int i = 0; i < S::NI; ++i)
+            un[static_cast<size_t>(leaf) * S::NI + i] = upwind_cell<R, N>(tile.data(), i, dtdx, vel, hs);
+    }
+}
+
+template <int R, int N, int H>
+void padded(uint32_t n, const face_link* links, const uint32_t* metas, const double* u, double* pad)
+{
+    constexpr int P    = N + 2 * H;
+    constexpr int FLAT = (R == 2) ? P * P : P * P * P;
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+        for (int i = 0; i < FLAT; ++i)
+        {
+            int kind;
+            pad[static_cast<size_t>(leaf) * FLAT + i] = padded_value<R, N, H>(u, leaf, links, metas, i, &kind);
+        }
+}
+
+template <int R, int N, int H>
+void flags(int kind, const uint64_t* keys, uint32_t n, const face_link* links, const uint32_t* metas, const double* u,
+           double rthr, double cthr, int mn, int mx, int8_t* out)
+{
+    using S            = patch_shape<R, N>;
+    constexpr int P    = N + 2 * H;
+    constexpr int FLAT = (R == 2) ? P * P : P * P * P;
+    std::vector<double> tile(tile_shape<R, N>::TS, 0.0);
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+    {
+        double v = 0.0;
+        if (kind == 1)
+        {
+            for (int i = 0; i < S::NI; ++i) stage_interior<R, N>(tile.data(), u + static_cast<size_t>(leaf) * S::NI, i);
+            for (int g = 0; g < S::NDIR * S::FACE; ++g) stage_ghost<R, N>(tile.data(), u, leaf, links, metas, g);
+            for (int i = 0; i < S::NI; ++i) v = fmax(v, gradient_cell<R, N>(tile.data(), i));
+        }
+        else
+        {
+            for (int i = 0; i < FLAT; ++i)
+            {
+                int          k;
+                const double x = padded_value<R, N, H>(u, leaf, links, metas, i, &k);
+                if (k < 2) v = fmax(v, x);
+            }
+        }
+        out[leaf] = decide(v, key_level(keys[leaf]), rthr, cthr, mn, mx);
+    }
+}
+
+// The regrid kernel sequence of amr_ctx::regrid, serialised.
+template <int R, int N>
+int regrid(int depth, const uint64_t* keys, uint32_t n, const int8_t* fl, const face_link* links,
+           const uint32_t* metas, const double* u, int max_level, uint64_t* keys_out, double* u_out, uint32_t cap,
+           uint32_t* summary)
+{
+    using S = patch_shape<R, N>;
+    std::vector<uint8_t>  mark(n), head(n);
+    std::vector<uint32_t> counts(n), pos(n);
+    for (uint32_t i = 0; i < n; ++i) mark[i] = refine_mark(fl[i], keys[i], depth);
+    for (int l = std::min(max_level, depth); l >= 1; --l)
+        for (uint32_t i = 0; i < n; ++i) close_item<R>(keys, l, links, metas, mark.data(), i);
+    uint32_t nc = 0, nr = 0, err = 0;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        bool root = false;
+        head[i]   = coarsen_family_head<R>(fl, keys, n, depth, links, metas, mark.data(), i, &root) ? 1 : 0;
+        if (root) err |= 2u;
+        nc += head[i];
+    }
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        counts[i] = emit_count<R>(keys, depth, mark.data(), head.data(), i);
+        nr += mark[i];
+        pos[i] = total;
+        total += counts[i];
+    }
+    summary[0] = total;
+    summary[1] = nr;
+    summary[2] = nc;
+    summary[3] = err;
+    if (err || (nr == 0 && nc == 0) || total > cap) return 0;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        for (uint32_t j = 0; j < counts[i]; ++j)
+        {
+            double* out = u_out + static_cast<size_t>(pos[i] + j) * S::NI;
+            for (int t = 0; t < S::NI; ++t)
+            {
+                if (mark[i])
+                    out[t] = refine_value<R, N>(u + static_cast<size_t>(i) * S::NI, static_cast<int>(j), t);
+                else if (head[i])
+                    out[t] = restrict_value<R, N>(u, i, t);
+                else
+                    out[t] = u[static_cast<size_t>(i) * S::NI + t];
+            }
+            keys_out[pos[i] + j] = emit_key<R>(keys[i], depth, mark[i] != 0, head[i] != 0, static_cast<int>(j));
+        }
+    }
+    return 1;
+}
+
+} // namespace
+
+#define EMU_SHAPES(X) X(2, 4, 1) X(2, 4, 2) X(2, 8, 1) X(2, 16, 1) X(2, 10, 2) X(2, 50, 2) X(3, 4, 1) X(3, 8, 1)
+
+extern "C"
+{
+
+int emu_build_table(int rank, int depth, const uint64_t* keys, uint32_t n, void* links, uint32_t* metas)
+{
+    auto* L = static_cast<face_link*>(links);
+    return rank == 2 ? build_table<2>(depth, keys, n, L, metas) : build_table<3>(depth, keys, n, L, metas);
+}
+
+int emu_step(int rank, int nn, const uint64_t* keys, uint32_t n, const void* links, const uint32_t* metas,
+             const double* u, double* un, const double* dtdx, const double* vel, const double* hs)
+{
+    auto const* L = static_cast<const face_link*>(links);
+#define X(R_, N_, H_)                                                   \
+    if (rank == R_ && nn == N_)                                         \
+    {                                                                   \
+        step<R_, N_>(keys, n, L, metas, u, un, dtdx, vel, hs);          \
+        return 0;                                                       \
+    }
+    EMU_SHAPES(X)
+#undef X
+    return -1;
+}
+
+int emu_padded(int rank, int nn, int h, uint32_t n, const void* links, const uint32_t* metas, const double* u,
+               double* pad)
+{
+    auto const* L = static_cast<const face_link*>(links);
+#define X(R_, N_, H_)                                  \
+    if (rank == R_ && nn == N_ && h == H_)             \
+    {                                                  \
+        padded<R_, N_, H_>(n, L, metas, u, pad);       \
+        return 0;                                      \
+    }
+    EMU_SHAPES(X)
+#undef X
+    return -1;
+}
+
+int emu_flags(int rank, int nn, int h, int kind, const uint64_t* keys, uint32_t n, const void* links,
+              const uint32_t* metas, const double* u, double rthr, double cthr, int mn, int mx, int8_t* out)
+{
+    auto const* L = static_cast<const face_link*>(links);
+#define X(R_, N_, H_)                                                          \
+    if (rank == R_ && nn == N_ && h == H_)                                     \
+    {                                                                          \
+        flags<R_, N_, H_>(kind, keys, n, L, metas, u, rthr, cthr, mn, mx, out); \
+        return 0;                                                              \
+    }
+    EMU_SHAPES(X)
+#undef X
+    return -1;
+}
+
+int emu_regrid(int rank, int nn, int depth, const uint64_t* keys, uint32_t n, const int8_t* fl, const void* links,
+               const uint32_t* metas, const double* u, int max_level, uint64_t* keys_out, double* u_out, uint32_t cap,
+               uint32_t* summary)
+{
+    auto const* L = static_cast<const face_link*>(links);
+#define X(R_, N_, H_)                                                                                 \
+    if (rank == R_ && nn == N_)                                                                       \
+        return regrid<R_, N_>(depth, keys, n, fl, L, metas, u, max_level, keys_out, u_out, cap, summary);
+    EMU_SHAPES(X)
+#undef X
+    return -1;
+}
+
+} // extern "C"

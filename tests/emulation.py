@@ -1,0 +1,121 @@
+"""ctypes face of tests/cpp/libkernel_emulation.so: the HIP kernels' item
+bodies executed on the CPU (test infrastructure)."""
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+ROOT = HERE.parent
+SRC = HERE / "cpp" / "kernel_emulation.hip"
+LIB = HERE / "cpp" / "libkernel_emulation.so"
+
+FACE_LINK = np.dtype([("ids", "<i4", (4,))])
+_lib = None
+
+
+def build():
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [
+        "/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++20", "-fPIC", "-shared",
+        "-ffp-contract=off", f"-I{ROOT / 'gpu-amr_amd' / 'csrc'}", f"-I{ROOT / 'include'}", str(SRC), "-o", str(tmp),
+    ]
+    subprocess.run(cmd, check=True, capture_output=True)
+    tmp.replace(LIB)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists() or LIB.stat().st_mtime < max(p.stat().st_mtime for p in (ROOT / "gpu-amr_amd" / "csrc").glob("*.hpp")):
+            build()
+        _lib = ctypes.CDLL(str(LIB))
+        for name in ("emu_build_table", "emu_step", "emu_padded", "emu_flags", "emu_regrid"):
+            getattr(_lib, name).restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def build_table(rank, depth, keys):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    n = keys.size
+    links = np.zeros(n * 2 * rank, dtype=FACE_LINK)
+    metas = np.zeros(n * 2 * rank, dtype=np.uint32)
+    err = load().emu_build_table(rank, depth, _p(keys), ctypes.c_uint32(n), _p(links), _p(metas))
+    return links, metas, err
+
+
+def to_reference_metadata(rank, links, metas):
+    from oracle import HALO_META_DTYPE
+
+    out = np.zeros(metas.size, dtype=HALO_META_DTYPE)
+    out["neighbor"] = -1
+    out["finer_ids"] = -1
+    rel = (metas & 3).astype(np.int8)
+    out["relation"] = rel
+    same_or_coarse = (rel == 1) | (rel == 3)
+    out["neighbor"][same_or_coarse] = links["ids"][same_or_coarse, 0]
+    fin = rel == 2
+    k = 1 << (rank - 1)
+    out["finer_ids"][fin, :k] = links["ids"][fin, :k]
+    co = rel == 3
+    for d in range(rank):
+        out["quadrant"][co, d] = (metas[co] >> (2 + d)) & 1
+    return out
+
+
+def step_params(rank, n, depth, max_level, vel, dt, lengths=(1.0, 1.0, 1.0)):
+    dtdx = np.zeros((17, 3))
+    for lv in range(depth + 1):
+        for d in range(rank):
+            cs = (lengths[d] * float(1 << (depth - lv)) / float(1 << depth)) / float(n)
+            dtdx[lv, d] = dt / cs
+    v = np.zeros(3)
+    v[:rank] = vel
+    hs = 0.5 * np.abs(v)
+    return dtdx, v, hs
+
+
+def step(rank, n, keys, links, metas, u, dtdx, vel, hs):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    un = np.zeros_like(u)
+    rc = load().emu_step(rank, n, _p(keys), ctypes.c_uint32(keys.size), _p(links), _p(metas), _p(u), _p(un),
+                         _p(np.ascontiguousarray(dtdx)), _p(vel), _p(hs))
+    assert rc == 0
+    return un
+
+
+def padded(rank, n, h, links, metas, u, nleaves):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    pad = np.zeros(nleaves * (n + 2 * h) ** rank)
+    rc = load().emu_padded(rank, n, h, ctypes.c_uint32(nleaves), _p(links), _p(metas), _p(u), _p(pad))
+    assert rc == 0
+    return pad.reshape((nleaves,) + (n + 2 * h,) * rank)
+
+
+def flags(rank, n, h, kind, keys, links, metas, u, rthr, cthr, mn, mx):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(keys.size, dtype=np.int8)
+    rc = load().emu_flags(rank, n, h, kind, _p(keys), ctypes.c_uint32(keys.size), _p(links), _p(metas), _p(u),
+                          ctypes.c_double(rthr), ctypes.c_double(cthr), mn, mx, _p(out))
+    assert rc == 0
+    return out
+
+
+def regrid(rank, n, depth, keys, fl, links, metas, u, max_level, cap):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    fl = np.ascontiguousarray(fl, dtype=np.int8)
+    keys_out = np.zeros(cap, dtype=np.uint64)
+    u_out = np.zeros(cap * n**rank)
+    summary = np.zeros(4, dtype=np.uint32)
+    changed = load().emu_regrid(rank, n, depth, _p(keys), ctypes.c_uint32(keys.size), _p(fl), _p(links), _p(metas),
+                                _p(u), max_level, _p(keys_out), _p(u_out), ctypes.c_uint32(cap), _p(summary))
+    total = int(summary[0])
+    return bool(changed), keys_out[:total], u_out[: total * n**rank], summary

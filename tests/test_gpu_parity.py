@@ -1,0 +1,316 @@
+"""HIP path vs the CPU oracle, through the C ABI.
+
+Bar: keys, neighbour tables and flags bit-exact; FP64 interiors and halos
+bit-exact (both sides compute without contraction); the stated tolerance of
+1e-12 relative is asserted as well so a failure report shows the size of any
+deviation.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import brute
+from amrlib import load_package
+from oracle import OracleTree
+from test_oracle_tree import gaussian_ic, random_flags
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def amr():
+    return load_package()
+
+
+def pair(amr, rank, n, h, depth, cap, vel, cfl):
+    g = amr.NdTree(rank, depth, n, h, cap)
+    s = amr.AmrSolver(g, vel, cfl)
+    # the oracle appends children before compacting (ndtree.hpp:727-778):
+    # twice the final capacity covers the transient
+    o = OracleTree(rank, n, h, depth, 2 * cap + 64)
+    o.set_params(vel, cfl, threads=8)
+    return g, s, o
+
+
+def assert_fields(got, exp, what):
+    got = np.asarray(got)
+    exp = np.asarray(exp)
+    assert got.shape == exp.shape, what
+    scale = max(1.0, float(np.max(np.abs(exp)))) if exp.size else 1.0
+    err = float(np.max(np.abs(got - exp))) if exp.size else 0.0
+    assert err <= REL_TOL * scale, f"{what}: max abs err {err}"
+    assert np.array_equal(got.view(np.uint64), exp.view(np.uint64)), f"{what}: not bit-exact (max err {err})"
+
+
+def assert_topology(g, o, what):
+    np.testing.assert_array_equal(g.keys(), o.keys(), err_msg=f"{what}: keys")
+    gm, om = g.neighbor_metadata(), o.metadata()
+    assert gm.tobytes() == om.tobytes(), f"{what}: neighbour table"
+
+
+def test_c1_uniform_2d_100_steps(amr):
+    say("case: C1 2D uniform depth 4, 16^2, 100 steps")
+    g, s, o = pair(amr, 2, 16, 1, 7, 256, (1.0, 0.5), 0.4)
+    for _ in range(4):
+        g.refine_all()
+        o.refine_all()
+    assert_topology(g, o, "C1")
+    cen = o.cell_centers()
+    assert np.array_equal(g.cell_centers().view(np.uint64), cen.view(np.uint64))
+    u0 = np.exp(-((cen[..., 0] - 0.3) ** 2 + (cen[..., 1] - 0.3) ** 2) / (2 * 0.1**2))
+    g.set_interior(u0)
+    o.set_interior(u0)
+    o.halo_exchange()
+    assert s.compute_dt() == o.compute_dt() == 0.4 / 256.0
+    s.advance_batch_async(100)
+    tot, ex = s.finish_advance_batch()
+    otot, oex = o.advance_batch(100)
+    assert ex == oex == 100 and tot == otot
+    assert_fields(g.get_interior(), o.get_interior(), "C1 after 100 steps")
+    m0 = u0.sum()
+    assert abs(g.get_interior().sum() - m0) <= 1e-12 * m0
+
+
+def test_padded_view_matches_halo_exchange(amr):
+    say("case: padded view 2D 16^2 depth 3")
+    g, s, o = pair(amr, 2, 16, 1, 7, 256, (1.0, 0.5), 0.4)
+    for _ in range(3):
+        g.refine_all()
+        o.refine_all()
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(g.size() * 256)
+    g.set_interior(v)
+    o.set_interior(v)
+    o.halo_exchange()
+    gp, op = g.get_patches(), o.get_padded()
+    mask = np.ones(gp.shape[1:], dtype=bool)
+    mask[0, 0] = mask[0, -1] = mask[-1, 0] = mask[-1, -1] = False  # corners are never written
+    assert_fields(gp[:, mask], op[:, mask], "padded view")
+
+
+def test_3d_uniform_steps(amr):
+    say("case: 3D uniform depth 2, 8^3, v=(1,.5,.25), 20 steps")
+    g, s, o = pair(amr, 3, 8, 1, 5, 64, (1.0, 0.5, 0.25), 0.3)
+    for _ in range(2):
+        g.refine_all()
+        o.refine_all()
+    assert_topology(g, o, "3D uniform")
+    rng = np.random.default_rng(7)
+    v = rng.random(g.size() * 512)
+    g.set_interior(v)
+    o.set_interior(v)
+    o.halo_exchange()
+    s.advance_batch_async(20)
+    s.finish_advance_batch()
+    o.advance_batch(20)
+    assert_fields(g.get_interior(), o.get_interior(), "3D after 20 steps")
+
+
+@pytest.mark.parametrize(
+    "rank,n,h,depth,start,rounds,seed",
+    [(2, 4, 1, 5, 2, 10, 1), (2, 4, 2, 5, 2, 6, 2), (2, 16, 1, 6, 2, 8, 3), (3, 4, 1, 4, 1, 6, 4), (3, 8, 1, 4, 2, 4, 5)],
+)
+def test_regrid_sequences(amr, rank, n, h, depth, start, rounds, seed):
+    say(f"case: regrid sequence rank={rank} n={n} h={h} depth={depth} seed={seed}")
+    vel = (1.0, 0.5, 0.25)[:rank]
+    cap = (1 << (rank * depth)) + 64
+    g, s, o = pair(amr, rank, n, h, depth, cap, vel, 0.4)
+    for _ in range(start):
+        g.refine_all()
+        o.refine_all()
+    rng = np.random.default_rng(seed)
+    for rd in range(rounds):
+        keys = o.keys()
+        v = rng.standard_normal(o.size() * n**rank)
+        g.set_interior(v)
+        o.set_interior(v)
+        flags = random_flags(rng, keys, rank, depth)
+        cg = g.reconstruct_tree(flags)
+        co = o.reconstruct(flags)
+        assert cg == co, f"round {rd}: changed flag"
+        assert_topology(g, o, f"round {rd}")
+        assert_fields(g.get_interior(), o.get_interior(), f"round {rd}: transfer")
+        o.halo_exchange()
+        gp, op = g.get_patches(), o.get_padded()
+        eh = brute.expected_halo(o.keys(), o.get_interior(), rank, n, h, depth)
+        m = ~np.isnan(eh)
+        assert_fields(gp[m], op[m], f"round {rd}: halos")
+        s.advance_batch_async(3)
+        s.finish_advance_batch()
+        o.advance_batch(3)
+        assert_fields(g.get_interior(), o.get_interior(), f"round {rd}: 3 steps")
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_criterion_flags(amr, kind):
+    say(f"case: criterion kind={kind}")
+    g, s, o = pair(amr, 2, 16, 1, 7, 4200, (1.0, 0.5), 0.4)
+    for _ in range(3):
+        g.refine_all()
+        o.refine_all()
+    rng = np.random.default_rng(11)
+    for rd in range(4):
+        cen = o.cell_centers()
+        c0 = rng.random(2)
+        u = np.exp(-((cen[..., 0] - c0[0]) ** 2 + (cen[..., 1] - c0[1]) ** 2) / (2 * 0.05**2))
+        g.set_interior(u)
+        o.set_interior(u)
+        o.halo_exchange()
+        thr = (0.5, 0.05) if kind == 0 else (0.05, 0.01)
+        crit = amr.make_criterion(kind, thr[0], thr[1], 2, 6)
+        fg = g.compute_flags(crit)
+        fo = o.criterion(kind, thr[0], thr[1], 2, 6)
+        np.testing.assert_array_equal(fg, fo)
+        assert g.reconstruct_tree(crit) == o.reconstruct(fo)
+        assert_topology(g, o, f"criterion round {rd}")
+
+
+@pytest.mark.parametrize("n", [10, 50])
+def test_reference_pulse_center_movement(amr, n):
+    """tests/advection_equation_amr.t.cpp:50-130 through the C ABI."""
+    say(f"case: PulseCenterMovement n={n}")
+    g, s, o = pair(amr, 2, n, 2, 7, 2000, (1.0, 0.5), 0.4)
+    s.initialize(gaussian_ic)
+    o.set_interior(gaussian_ic(o.cell_centers()))
+    o.halo_exchange()
+    t, target = 0.0, 0.5
+    while t < target:
+        dt = s.advance()
+        o.advance()
+        if t + dt > target:
+            dt = target - t
+        g.halo_exchange_update()
+        t += dt
+    u = g.get_interior()
+    assert_fields(u, o.get_interior(), "pulse")
+    cen = g.cell_centers()[0].reshape(n, n, 2)
+    iy, ix = np.unravel_index(np.argmax(u[0]), u[0].shape)
+    for d, v in enumerate((1.0, 0.5)):
+        assert abs(cen[iy, ix][d] - (0.2 + v * target)) <= 1.0 / n
+
+
+def test_reference_pulse_center_movement_amr(amr):
+    """tests/advection_equation_amr.t.cpp:132-231 (10x10 patches, halo 2,
+    max-value criterion refine > 0.1 below level 5, coarsen < 0.05 above 1,
+    regrid after every step)."""
+    say("case: PulseCenterMovementAMR n=10")
+    n = 10
+    g, s, o = pair(amr, 2, n, 2, 7, 2000, (1.0, 0.5), 0.4)
+    s.initialize(gaussian_ic)
+    o.set_interior(gaussian_ic(o.cell_centers()))
+    o.halo_exchange()
+    crit = amr.make_criterion(0, 0.1, 0.05, 1, 5)
+    t, target, steps = 0.0, 0.5, 0
+    while t < target:
+        dt = s.advance()
+        assert dt == o.advance()
+        fo = o.criterion(0, 0.1, 0.05, 1, 5)
+        np.testing.assert_array_equal(g.compute_flags(crit), fo, err_msg=f"step {steps}")
+        g.reconstruct_tree(crit)
+        o.reconstruct(fo)
+        o.halo_exchange()
+        t += dt
+        steps += 1
+        if steps % 10 == 0:
+            assert_topology(g, o, f"step {steps}")
+            assert_fields(g.get_interior(), o.get_interior(), f"step {steps}")
+    assert_topology(g, o, "final")
+    u = g.get_interior()
+    assert_fields(u, o.get_interior(), "final field")
+    keys = g.keys()
+    cen = g.cell_centers()
+    i = int(np.argmax(u.reshape(-1)))
+    pos = cen.reshape(-1, 2)[i]
+    _, lv = amr.decode_keys(keys[:1], 2)
+    cs = g.cell_sizes(lv)[0]
+    for d, v in enumerate((1.0, 0.5)):
+        assert abs(pos[d] - (0.2 + v * target)) <= cs[d]
+
+
+def test_active_amr_gradient_short(amr):
+    """C2-style run (seeded 8-bump mixture, gradient criterion every 10 steps,
+    levels 3..7) for 60 steps, compared with the oracle at every regrid."""
+    say("case: active AMR gradient, 60 steps")
+    g, s, o = pair(amr, 2, 16, 1, 7, 20000, (1.0, 0.5), 0.4)
+    for _ in range(3):
+        g.refine_all()
+        o.refine_all()
+    rng = np.random.default_rng(2026)
+    amp, sig, cx = rng.uniform(0.5, 1, 8), rng.uniform(0.03, 0.1, 8), rng.uniform(0, 1, (8, 2))
+
+    def ic(cen):
+        u = np.zeros(cen.shape[:-1])
+        for a, sg, c in zip(amp, sig, cx):
+            u += a * np.exp(-((cen[..., 0] - c[0]) ** 2 + (cen[..., 1] - c[1]) ** 2) / (2 * sg * sg))
+        return u
+
+    s.initialize(ic)
+    o.set_interior(ic(o.cell_centers()))
+    o.halo_exchange()
+    crit = amr.make_criterion(1, 0.05, 0.01, 3, 7)
+    for it in range(6):
+        fo = o.criterion(1, 0.05, 0.01, 3, 7)
+        np.testing.assert_array_equal(g.compute_flags(crit), fo)
+        g.reconstruct_tree(crit)
+        o.reconstruct(fo)
+        o.halo_exchange()
+        assert_topology(g, o, f"regrid {it}")
+        s.advance_batch_async(10)
+        s.finish_advance_batch()
+        o.advance_batch(10)
+        assert_fields(g.get_interior(), o.get_interior(), f"after batch {it}")
+    say("leaves at end:", g.size())
+
+
+def test_c3_full_size_periodic_tiling(amr):
+    """Full C3 shape (32768 leaves of 8^3, 16.8M cells): a field with period
+    1/4 evolves exactly like the oracle's 1/4-size periodic box, tiled."""
+    say("case: C3 full size, tiling property, 10 steps")
+    small = OracleTree(3, 8, 1, 5, 1200, lengths=(0.25, 0.25, 0.25))
+    small.set_params((1.0, 0.5, 0.25), 0.3, threads=8)
+    for _ in range(3):
+        small.refine_all()
+    rng = np.random.default_rng(42)
+    small.set_interior(rng.random(small.size() * 512))
+    small.halo_exchange()
+    dense_small = brute.dense_from_uniform(small.keys(), small.get_interior(), 3, 8, 5)
+    g = amr.NdTree(3, 5, 8, 1, 32768)
+    s = amr.AmrSolver(g, (1.0, 0.5, 0.25), 0.3)
+    for _ in range(5):
+        g.refine_all()
+    assert g.size() == 32768
+    keys = g.keys()
+    big = np.tile(dense_small, (4, 4, 4))
+    g.set_interior(brute.split_uniform(keys, big, 3, 8, 5))
+    assert s.compute_dt() == small.compute_dt() == 0.3 / 256.0
+    s.advance_batch_async(10)
+    s.finish_advance_batch()
+    small.advance_batch(10)
+    got = brute.dense_from_uniform(keys, g.get_interior(), 3, 8, 5)
+    exp = np.tile(brute.dense_from_uniform(small.keys(), small.get_interior(), 3, 8, 5), (4, 4, 4))
+    assert_fields(got, exp, "C3 tiling")
+    assert abs(got.sum() - big.sum()) <= 1e-12 * big.sum()
+
+
+def test_capacity_and_root_errors(amr):
+    say("case: error paths")
+    g = amr.NdTree(2, 7, 16, 1, 10)
+    g.refine_all()
+    with pytest.raises(amr.AmrError) as e:
+        g.refine_all()  # 16 > capacity 10
+    assert e.value.code == amr.AMR_ERR_CAPACITY
+    assert g.size() == 4
+    with pytest.raises(amr.AmrError) as e:
+        g.reconstruct_tree(np.full(4, 2, dtype=np.int8))
+    assert e.value.code == amr.AMR_ERR_UNSUPPORTED
+    with pytest.raises(amr.AmrError) as e:
+        amr.NdTree(2, 7, 12, 1, 10)
+    assert e.value.code == amr.AMR_ERR_UNSUPPORTED

@@ -1,0 +1,89 @@
+"""The HIP kernels' per-item bodies, executed on the CPU, against the oracle.
+
+These run without a GPU: they pin the device-side logic (geometric neighbour
+table, ghost gather, upwind update, criteria, regrid expand/contract) to the
+oracle bit for bit; the GPU tests then add the launch configuration and
+host sequencing on real hardware.
+"""
+import numpy as np
+import pytest
+
+import brute
+import emulation as emu
+from oracle import OracleTree
+from test_oracle_tree import random_flags
+
+
+def emulated_sequence(rank, n, h, depth, start, rounds, seed, vel):
+    rng = np.random.default_rng(seed)
+    cap = (1 << (rank * depth)) + 64
+    o = OracleTree(rank, n, h, depth, 2 * cap + 64)
+    o.set_params(vel, 0.4)
+    for _ in range(start):
+        o.refine_all()
+    keys = o.keys()
+    u = rng.standard_normal(o.size() * n**rank)
+    o.set_interior(u)
+    o.halo_exchange()
+    for rd in range(rounds):
+        links, metas, err = emu.build_table(rank, depth, keys)
+        assert err == 0
+        assert emu.to_reference_metadata(rank, links, metas).tobytes() == o.metadata().tobytes(), f"round {rd}: table"
+        # padded view vs the oracle's halo exchange (corners excluded)
+        pad = emu.padded(rank, n, h, links, metas, u, keys.size)
+        eh = brute.expected_halo(keys, u, rank, n, h, depth)
+        m = ~np.isnan(eh)
+        assert np.array_equal(pad[m].view(np.uint64), o.get_padded()[m].view(np.uint64)), f"round {rd}: halo"
+        # criteria
+        for kind, thr in ((0, (0.5, -1.0)), (1, (1.0, 0.2))):
+            fe = emu.flags(rank, n, h, kind, keys, links, metas, u, thr[0], thr[1], 1, depth)
+            fo = o.criterion(kind, thr[0], thr[1], 1, depth)
+            np.testing.assert_array_equal(fe, fo, err_msg=f"round {rd}: criterion {kind}")
+        # regrid
+        flags = random_flags(rng, keys, rank, depth)
+        max_level = int(max(int(k) & 63 for k in keys))
+        changed, k2, u2, summ = emu.regrid(rank, n, depth, keys, flags, links, metas, u, max_level, cap)
+        assert summ[3] == 0
+        co = o.reconstruct(flags)
+        assert changed == co, f"round {rd}: changed"
+        if changed:
+            keys, u = k2, u2
+        np.testing.assert_array_equal(keys, o.keys(), err_msg=f"round {rd}: keys")
+        assert np.array_equal(u.view(np.uint64), o.get_interior().reshape(-1).view(np.uint64)), f"round {rd}: transfer"
+        o.halo_exchange()
+        # two steps
+        links, metas, err = emu.build_table(rank, depth, keys)
+        max_level = int(max(int(k) & 63 for k in keys))
+        for _ in range(2):
+            dt = o.compute_dt()
+            dtdx, v, hs = emu.step_params(rank, n, depth, max_level, vel, dt)
+            u = emu.step(rank, n, keys, links, metas, u, dtdx, v, hs)
+            o.step(dt)
+        assert np.array_equal(u.view(np.uint64), o.get_interior().reshape(-1).view(np.uint64)), f"round {rd}: step"
+
+
+@pytest.mark.parametrize(
+    "rank,n,h,depth,start,rounds,seed",
+    [(2, 4, 1, 5, 2, 8, 1), (2, 4, 2, 5, 2, 6, 2), (2, 16, 1, 5, 2, 5, 3), (3, 4, 1, 4, 1, 5, 4), (3, 8, 1, 3, 1, 3, 5)],
+)
+def test_emulated_kernels_match_oracle(rank, n, h, depth, start, rounds, seed):
+    emulated_sequence(rank, n, h, depth, start, rounds, seed, (1.0, 0.5, 0.25)[:rank])
+
+
+def test_emulated_table_detects_unbalanced_tree():
+    # a level-1 leaf next to level-3 leaves violates 2:1
+    keys = [brute.make_key((0, 0), 1)]
+    s = 1 << 4  # depth 5, level 1 -> size 16; level 3 -> size 4
+    for x in range(0, 32, 16):
+        for y in range(0, 32, 16):
+            if (x, y) == (0, 0):
+                continue
+            if (x, y) == (16, 0):
+                for xx in range(16, 32, 4):
+                    for yy in range(0, 16, 4):
+                        keys.append(brute.make_key((xx, yy), 3))
+            else:
+                keys.append(brute.make_key((x, y), 1))
+    keys = np.array(sorted(keys), dtype=np.uint64)
+    _, _, err = emu.build_table(2, 5, keys)
+    assert err == 1
