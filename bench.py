@@ -9,8 +9,9 @@ all leaves with the state resident in HBM.
 
 Prints one JSON line with the throughput, the roofline of the step kernel
 (16 algorithmic bytes per cell update: one FP64 read of u^n, one FP64 write of
-u^(n+1)) measured with HIP events on the context stream, and the CPU oracle
-timed on this host as a baseline.
+u^(n+1)) measured with HIP events on the context stream, the CPU oracle timed
+on this host as a baseline, and the batched form (64 independent 2D
+instances, one graph launch) against 64 sequential plain solves.
 """
 from __future__ import annotations
 
@@ -69,30 +70,9 @@ class HipEvents:
         return float(ms.value)
 
 
-def c3_dense_ic(ncell=256, seed=42):
-    """Seeded 3D Gaussian bump (amplitude 1, sigma 0.1, centre U[.25,.75]^3)
-    evaluated at cell centres of the uniform 256^3 grid; array order z, y, x."""
-    c = np.random.default_rng(seed).uniform(0.25, 0.75, 3)
-    h = 1.0 / ncell
-    x = (np.arange(ncell) * h) + 0.5 * h
-    gx = np.exp(-((x - c[0]) ** 2) / (2 * 0.1**2))
-    gy = np.exp(-((x - c[1]) ** 2) / (2 * 0.1**2))
-    gz = np.exp(-((x - c[2]) ** 2) / (2 * 0.1**2))
-    return gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
-
-
-def dense_to_leaves(amr, keys, dense, n, depth, rank=3):
-    coords, lv = amr.decode_keys(keys, rank)
-    s = 1 << (depth - int(lv[0]))
-    nb = dense.shape[0] // n
-    blocks = dense.reshape(nb, n, nb, n, nb, n).transpose(0, 2, 4, 1, 3, 5)
-    bz, by, bx = coords[:, 2] // s, coords[:, 1] // s, coords[:, 0] // s
-    return np.ascontiguousarray(blocks[bz, by, bx])
-
-
 def traffic_from_profiles():
     """HBM bytes per step-kernel launch from the committed PMC summary."""
-    p = ROOT / "profiles" / "k_step_3d_c3_traffic.md"
+    p = ROOT / "profiles" / "k_step_3d8_c3_traffic.md"
     if not p.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
@@ -106,19 +86,83 @@ def cpu_baseline(steps, threads):
         sys.path.insert(0, str(ROOT))
     from oracle import OracleTree
 
+    from gpu_amr_amd import workloads
+
     o = OracleTree(3, 8, 1, 5, 36864 + 64)
     o.set_params((1.0, 0.5, 0.25), 0.3, threads=threads)
     t0 = time.perf_counter()
     for _ in range(5):
         o.refine_all()
-    amr = package()
-    o.set_interior(dense_to_leaves(amr, o.keys(), c3_dense_ic(), 8, 5))
+    o.set_interior(workloads.dense_to_leaves(package(), o.keys(), workloads.c3_dense_ic(), 8, 5))
     o.halo_exchange()
     log(f"cpu baseline: tree built in {time.perf_counter() - t0:.1f}s, timing {steps} steps on {threads} threads")
     t0 = time.perf_counter()
     o.advance_batch(steps)
     dt = time.perf_counter() - t0
     return steps * o.size() * 512 / dt, dt
+
+
+def copy_calibration(mib, reps):
+    """Plain device-to-device copies of a known size: the reference point
+    that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_calibration_bytes_per_dispatch": nbytes, "reps": reps}), flush=True)
+
+
+def batched_section(amr, instances, steps):
+    """C4: `instances` independent 2D trees (depth 3..7, own velocity and dt)
+    stepped as one graph launch vs the same solves one after another with the
+    plain per-tree kernel; results must agree bit for bit."""
+    from gpu_amr_amd import workloads
+
+    t0 = time.perf_counter()
+    built = [workloads.c4_instance(amr, i) for i in range(instances)]
+    trees = [b[0] for b in built]
+    solvers = [b[1] for b in built]
+    leaves = [t.size() for t in trees]
+    log(f"batched: {instances} instances, {sum(leaves)} leaves (min {min(leaves)}, max {max(leaves)}), built in {time.perf_counter() - t0:.1f}s")
+    batch = amr.Batch(trees)  # copies the initial state
+    variant, _ = batch.autotune(10)
+    batch.advance(steps, use_graph=True)  # graph capture + first run
+    batch.synchronize()
+    # sequential plain solves of the same instances (one launch per step each)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.advance_batch_async(steps)
+        s.finish_advance_batch()
+    seq = time.perf_counter() - t0
+    seq_out = [t.get_interior().copy() for t in trees]
+    batch.scatter()
+    same = all(np.array_equal(t.get_interior().view(np.uint64), u.view(np.uint64)) for t, u in zip(trees, seq_out))
+    # timed: replay of the captured graph (`steps` is even, so the buffer
+    # parity matches the capture)
+    t0 = time.perf_counter()
+    batch.advance(steps, use_graph=True)
+    batch.synchronize()
+    bat = time.perf_counter() - t0
+    batch.close()
+    cells = sum(leaves) * 256 * steps
+    log(f"batched: sequential {seq * 1e3:.1f} ms, batched {bat * 1e3:.1f} ms, speedup {seq / bat:.2f}x, bit-identical {same}")
+    return {
+        "workload": f"{instances} independent 2D instances, 16^2 FP64 patches, depth 3..7, own velocity, CFL 0.4",
+        "instances": instances,
+        "steps": steps,
+        "leaves": int(sum(leaves)),
+        "sequential_ms": seq * 1e3,
+        "batched_ms": bat * 1e3,
+        "speedup": seq / bat,
+        "batched_cell_updates_per_s": cells / bat,
+        "bit_identical": bool(same),
+        "step_variant": variant,
+    }
 
 
 def main():
@@ -128,20 +172,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-instances", type=int, default=64)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 5)
+        return
 
     amr = package()
-    depth, n = 5, 8
-    g = amr.NdTree(3, depth, n, 1, 32768)
-    s = amr.AmrSolver(g, (1.0, 0.5, 0.25), 0.3)
-    for _ in range(depth):
-        g.refine_all()
+    from gpu_amr_amd import workloads
+
+    g, s = workloads.build_c3(amr)
     leaves = g.size()
-    cells = leaves * n**3
-    g.set_interior(dense_to_leaves(amr, g.keys(), c3_dense_ic(), n, depth))
+    cells = leaves * 512
     log(f"tree: {leaves} leaves, {cells} cells; warmup {args.warmup} steps")
     s.advance_batch_async(args.warmup)
     s.finish_advance_batch()
@@ -149,6 +196,20 @@ def main():
     ev = HipEvents()
     e0, e1 = ev.create(), ev.create()
     stream = g.stream()
+    # per-variant timing (diagnostic), then the autotuner's choice
+    names = s.step_variants()
+    per_variant = {}
+    for v, name in enumerate(names):
+        s.set_step_variant(v)
+        g.synchronize()
+        ev.record(e0, stream)
+        s.advance_batch_async(20)
+        ev.record(e1, stream)
+        s.finish_advance_batch()
+        per_variant[name] = round(ev.elapsed_ms(e0, e1) / 20 * 1e3, 2)
+    best, best_us = s.autotune(20)
+    log(f"step variants (us/step): {per_variant}; autotuner -> {names[best]} ({best_us:.1f} us)")
+
     g.synchronize()
     t0 = time.perf_counter()
     ev.record(e0, stream)
@@ -160,7 +221,12 @@ def main():
     assert executed == args.steps
     value = args.steps * cells / wall
     achieved = BYTES_PER_CELL_UPDATE * cells / (kernel_ms * 1e-3) / 1e9
-    log(f"gpu: {wall * 1e3 / args.steps:.2f} ms/step wall, {kernel_ms * 1e3:.1f} us/step kernel, {achieved:.0f} GB/s")
+    log(f"gpu: {wall * 1e3 / args.steps:.3f} ms/step wall, {kernel_ms * 1e3:.1f} us/step kernel, {achieved:.0f} GB/s")
+    g.close()
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_section(amr, args.batched_instances, 200)
 
     cpu = None
     if not args.no_cpu:
@@ -201,10 +267,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(),
-            "kernel": "k_step<3,8,256>",
+            "kernel": f"step kernel variant '{names[best]}' (autotuned)",
             "kernel_us": kernel_ms * 1e3,
+            "variants_us": per_variant,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -84,7 +84,21 @@ _SIGS = {
     "amr_solver_advance_batch_async": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double],
     "amr_solver_finish_advance_batch": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)],
     "amr_solver_advance": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)],
+    "amr_solver_step_variants": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)],
+    "amr_solver_step_variant_name": [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)],
+    "amr_solver_set_step_variant": [ctypes.c_void_p, ctypes.c_int32],
+    "amr_solver_get_step_variant": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)],
+    "amr_solver_autotune": [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
     "amr_ctx_synchronize": [ctypes.c_void_p],
+    "amr_batch_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
+    "amr_batch_destroy": [ctypes.c_void_p],
+    "amr_batch_size": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)],
+    "amr_batch_advance": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int],
+    "amr_batch_synchronize": [ctypes.c_void_p],
+    "amr_batch_scatter": [ctypes.c_void_p],
+    "amr_batch_stream": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
+    "amr_batch_set_step_variant": [ctypes.c_void_p, ctypes.c_int32],
+    "amr_batch_autotune": [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
 }
 
 
@@ -304,6 +318,80 @@ class AmrSolver:
         dt = ctypes.c_double()
         _check(self.tree.lib.amr_solver_advance(self.tree.h, ctypes.byref(dt)))
         return dt.value
+
+    # -- step-kernel autotuner ----------------------------------------------
+    def step_variants(self):
+        n = ctypes.c_int32()
+        _check(self.tree.lib.amr_solver_step_variants(self.tree.h, ctypes.byref(n)))
+        names = []
+        for v in range(n.value):
+            s = ctypes.c_char_p()
+            _check(self.tree.lib.amr_solver_step_variant_name(self.tree.h, v, ctypes.byref(s)))
+            names.append(s.value.decode())
+        return names
+
+    def set_step_variant(self, v):
+        _check(self.tree.lib.amr_solver_set_step_variant(self.tree.h, int(v)))
+
+    def step_variant(self):
+        v = ctypes.c_int32()
+        _check(self.tree.lib.amr_solver_get_step_variant(self.tree.h, ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, trial_steps=10):
+        """Times every step-kernel variant on the current tree, keeps the
+        fastest; returns (variant index, microseconds per step)."""
+        v, us = ctypes.c_int32(), ctypes.c_float()
+        _check(self.tree.lib.amr_solver_autotune(self.tree.h, int(trial_steps), ctypes.byref(v), ctypes.byref(us)))
+        return v.value, us.value
+
+
+class Batch:
+    """Many independent trees stepped as one launch per step (optionally one
+    hipGraph launch for many steps); each keeps its own velocity and dt."""
+
+    def __init__(self, trees):
+        self.lib = load()
+        self.trees = list(trees)
+        arr = (ctypes.c_void_p * len(self.trees))(*[t.h.value for t in self.trees])
+        h = ctypes.c_void_p()
+        _check(self.lib.amr_batch_create(arr, len(self.trees), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.amr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def size(self):
+        n = ctypes.c_uint64()
+        _check(self.lib.amr_batch_size(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def advance(self, steps, use_graph=True):
+        _check(self.lib.amr_batch_advance(self.h, int(steps), 1 if use_graph else 0))
+
+    def synchronize(self):
+        _check(self.lib.amr_batch_synchronize(self.h))
+
+    def scatter(self):
+        _check(self.lib.amr_batch_scatter(self.h))
+
+    def stream(self):
+        p = ctypes.c_void_p()
+        _check(self.lib.amr_batch_stream(self.h, ctypes.byref(p)))
+        return p.value
+
+    def set_step_variant(self, v):
+        _check(self.lib.amr_batch_set_step_variant(self.h, int(v)))
+
+    def autotune(self, trial_steps=10):
+        v, us = ctypes.c_int32(), ctypes.c_float()
+        _check(self.lib.amr_batch_autotune(self.h, int(trial_steps), ctypes.byref(v), ctypes.byref(us)))
+        return v.value, us.value
 
 
 def make_criterion(kind, refine_threshold, coarsen_threshold, min_level, max_level):

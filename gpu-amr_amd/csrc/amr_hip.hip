@@ -60,8 +60,10 @@ constexpr int block_threads(int cells)
 struct ops
 {
     int rank, n, h;
-    void (*step)(const double*, double*, const uint64_t*, const face_link*, const uint32_t*, step_params const&,
-                 uint32_t, hipStream_t);
+    void (*step)(int variant, const double*, double*, const uint64_t*, const face_link*, const uint32_t*,
+                 step_params const&, const step_params*, const uint16_t*, uint32_t, hipStream_t);
+    int (*variant_count)();
+    char const* (*variant_name)(int);
     void (*padded)(const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
     void (*criterion)(int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double, double,
                       int, int, int8_t*, uint32_t, hipStream_t);
@@ -76,11 +78,57 @@ struct instance
     static constexpr int P   = N + 2 * H;
     static constexpr int NTP = block_threads((R == 2) ? P * P : P * P * P);
 
-    static void step(const double* u, double* un, const uint64_t* keys, const face_link* links,
-                     const uint32_t* metas, step_params const& prm, uint32_t n, hipStream_t s)
+    // Step-kernel variants (all bit-identical; they differ in how leaves map
+    // to workgroups and in how many workgroups share a CU, set through extra
+    // dynamic LDS).  Variant 0 is the default until the autotuner picks.
+    static constexpr char const* kVariants3d8[]  = { "wave",          "wave-nt",     "group8", "group8-nt",
+                                                     "group8-lds56k", "wave-lds10k", "tile256" };
+    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256" };
+    static constexpr char const* kVariantsAny[]  = { "tile" };
+    static int variant_count()
+    {
+        if constexpr (R == 3 && N == 8) return 7;
+        if constexpr (R == 2 && N == 16) return 4;
+        return 1;
+    }
+    static char const* variant_name(int v)
+    {
+        if constexpr (R == 3 && N == 8) return kVariants3d8[v];
+        if constexpr (R == 2 && N == 16) return kVariants2d16[v];
+        return kVariantsAny[v];
+    }
+    static void step(int variant, const double* u, double* un, const uint64_t* keys, const face_link* links,
+                     const uint32_t* metas, step_params const& prm, const step_params* table,
+                     const uint16_t* inst, uint32_t n, hipStream_t s)
     {
         if (n == 0) return;
-        hipLaunchKernelGGL((k_step<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, keys, links, metas, prm, n);
+        const dim3 per_leaf(xcd_grid(n));
+        if constexpr (R == 3 && N == 8)
+        {
+            const dim3 per_group(xcd_grid((n + kGroup3d - 1) / kGroup3d));
+            switch (variant)
+            {
+                case 1: hipLaunchKernelGGL(k_step_3d8<true>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                case 2: hipLaunchKernelGGL(k_step_3d8_group<false>, per_group, dim3(512), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                case 3: hipLaunchKernelGGL(k_step_3d8_group<true>, per_group, dim3(512), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                case 4: hipLaunchKernelGGL(k_step_3d8_group<false>, per_group, dim3(512), 16 << 10, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                case 5: hipLaunchKernelGGL(k_step_3d8<false>, per_leaf, dim3(64), 8 << 10, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                case 6: hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                default: hipLaunchKernelGGL(k_step_3d8<false>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+            }
+        }
+        else if constexpr (R == 2 && N == 16)
+        {
+            switch (variant)
+            {
+                case 1: hipLaunchKernelGGL(k_step_2d16<true>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                case 2: hipLaunchKernelGGL(k_step_2d16<false>, per_leaf, dim3(64), 10 << 10, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                case 3: hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
+                default: hipLaunchKernelGGL(k_step_2d16<false>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+            }
+        }
+        else
+            hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm, table, inst, n);
         check_launch("k_step");
     }
     static void padded(const double* u, double* pad, const face_link* links, const uint32_t* metas, uint32_t n,
@@ -112,7 +160,10 @@ struct instance
                            keys_out, u_out, summary);
         check_launch("k_emit");
     }
-    static constexpr ops table() { return ops{ R, N, H, &step, &padded, &criterion, &emit }; }
+    static constexpr ops table()
+    {
+        return ops{ R, N, H, &step, &variant_count, &variant_name, &padded, &criterion, &emit };
+    }
 };
 
 // Patch shapes built into this library: the configurations' 16^2 and 8^3
@@ -167,6 +218,22 @@ T* dev_alloc(size_t count)
 
 inline unsigned grid1d(uint64_t items, unsigned threads) { return static_cast<unsigned>((items + threads - 1) / threads); }
 
+// Batch packing: shift an instance's neighbour slots by its leaf offset, and
+// tag its leaves with the instance index.
+__global__ void k_offset_links(face_link* __restrict__ links, uint64_t count, int32_t offset)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= count) return;
+    for (int j = 0; j < 4; ++j)
+        if (links[i].ids[j] >= 0) links[i].ids[j] += offset;
+}
+
+__global__ void k_fill_instance(uint16_t* __restrict__ inst, uint64_t count, uint16_t value)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < count) inst[i] = value;
+}
+
 } // namespace
 
 struct amr_ctx
@@ -204,6 +271,7 @@ struct amr_ctx
     double          pending_dt    = 0.0;
     uint64_t        pending_steps = 0;
     bool            broken        = false;
+    int             variant       = 0; // step-kernel variant (autotuner)
 
     ~amr_ctx()
     {
@@ -266,9 +334,8 @@ struct amr_ctx
         if (broken) throw amr_error(AMR_ERR_TOPOLOGY, "context unusable after a topology error");
     }
 
-    void step(double dt)
+    step_params make_params(double dt) const
     {
-        ensure_usable();
         step_params prm{};
         for (int d = 0; d < 3; ++d)
         {
@@ -277,8 +344,51 @@ struct amr_ctx
         }
         for (int l = 0; l <= depth; ++l)
             for (int d = 0; d < rank; ++d) prm.dtdx[l][d] = dt / cell_size(l, d);
-        op->step(u[cur], u[1 - cur], keys[kc], links, metas, prm, static_cast<uint32_t>(n), stream);
+        return prm;
+    }
+
+    void step(double dt)
+    {
+        ensure_usable();
+        const step_params prm = make_params(dt);
+        op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), stream);
         cur = 1 - cur;
+    }
+
+    // Times every step-kernel variant on the current tree and keeps the
+    // fastest.  Trial launches read the current interiors and write only the
+    // other buffer, so the state is unchanged.
+    int autotune(uint32_t trials, float* best_us)
+    {
+        ensure_usable();
+        const step_params prm = make_params(compute_dt());
+        hipEvent_t        e0, e1;
+        AMR_HIP_CHECK(hipEventCreate(&e0));
+        AMR_HIP_CHECK(hipEventCreate(&e1));
+        int   best   = 0;
+        float best_t = std::numeric_limits<float>::max();
+        for (int v = 0; v < op->variant_count(); ++v)
+        {
+            op->step(v, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), stream);
+            AMR_HIP_CHECK(hipEventRecord(e0, stream));
+            for (uint32_t t = 0; t < trials; ++t)
+                op->step(v, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), stream);
+            AMR_HIP_CHECK(hipEventRecord(e1, stream));
+            AMR_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            AMR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            const float us = 1000.0f * ms / static_cast<float>(std::max(trials, 1u));
+            if (us < best_t)
+            {
+                best_t = us;
+                best   = v;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        variant = best;
+        if (best_us) *best_us = best_t;
+        return best;
     }
 
     void scan(uint32_t count)
@@ -343,6 +453,92 @@ struct amr_ctx
         max_level = static_cast<int>(h_summary->max_level);
         build_table();
         return true;
+    }
+};
+
+// A batch of independent trees with the same patch shape, concatenated into
+// one leaf list so that one launch steps all of them: each instance keeps its
+// own velocity, cfl and dt (through the per-instance constant table).
+struct amr_batch
+{
+    ops const*               op = nullptr;
+    hipStream_t              stream{};
+    int                      ni = 0, ndir = 0;
+    uint64_t                 total = 0;
+    std::vector<amr_ctx*>    src;
+    std::vector<uint64_t>    off;
+    std::vector<step_params> host_table;
+    double*                  u[2]{};
+    int                      cur     = 0;
+    int                      variant = 0;
+    uint64_t*                keys    = nullptr;
+    face_link*               links   = nullptr;
+    uint32_t*                metas   = nullptr;
+    uint16_t*                inst    = nullptr;
+    step_params*             table   = nullptr;
+    struct graph_entry
+    {
+        uint64_t        steps;
+        int             parity;
+        int             variant;
+        hipGraphExec_t  exec;
+    };
+    std::vector<graph_entry> graphs;
+
+    ~amr_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        for (void* p : { (void*)u[0], (void*)u[1], (void*)keys, (void*)links, (void*)metas, (void*)inst, (void*)table })
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void launch(int v)
+    {
+        op->step(v, u[cur], u[1 - cur], keys, links, metas, host_table[0], table, inst, static_cast<uint32_t>(total),
+                 stream);
+        cur = 1 - cur;
+    }
+
+    // Every instance advances `steps` steps with its own dt.  With use_graph
+    // the launches are captured once per (steps, buffer parity, variant) in a
+    // hipGraph and replayed as one graph launch.
+    void advance(uint64_t steps, bool use_graph)
+    {
+        if (!use_graph)
+        {
+            for (uint64_t s = 0; s < steps; ++s) launch(variant);
+            return;
+        }
+        hipGraphExec_t exec = nullptr;
+        for (auto const& g : graphs)
+            if (g.steps == steps && g.parity == cur && g.variant == variant) exec = g.exec;
+        const int start = cur;
+        if (!exec)
+        {
+            hipGraph_t graph = nullptr;
+            AMR_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try
+            {
+                for (uint64_t s = 0; s < steps; ++s) launch(variant);
+            }
+            catch (...)
+            {
+                (void)hipStreamEndCapture(stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                cur = start;
+                throw;
+            }
+            AMR_HIP_CHECK(hipStreamEndCapture(stream, &graph));
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) throw amr_error(AMR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            graphs.push_back({ steps, start, variant, exec });
+            cur = start;
+        }
+        AMR_HIP_CHECK(hipGraphLaunch(exec, stream));
+        cur = (steps % 2) ? 1 - start : start;
     }
 };
 
@@ -687,6 +883,243 @@ int amr_solver_advance(amr_ctx* ctx, double* dt)
     int rc = amr_solver_advance_batch_async(ctx, 1, std::numeric_limits<double>::max());
     if (rc != AMR_OK) return rc;
     return amr_solver_finish_advance_batch(ctx, dt, nullptr);
+}
+
+int amr_solver_step_variants(amr_ctx* ctx, int32_t* count)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !count) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *count = ctx->op->variant_count();
+        }
+    );
+}
+
+int amr_solver_step_variant_name(amr_ctx* ctx, int32_t variant, const char** name)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !name) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (variant < 0 || variant >= ctx->op->variant_count()) throw amr_error(AMR_ERR_ARGUMENT, "no such variant");
+            *name = ctx->op->variant_name(variant);
+        }
+    );
+}
+
+int amr_solver_set_step_variant(amr_ctx* ctx, int32_t variant)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (variant < 0 || variant >= ctx->op->variant_count()) throw amr_error(AMR_ERR_ARGUMENT, "no such variant");
+            ctx->variant = variant;
+        }
+    );
+}
+
+int amr_solver_get_step_variant(amr_ctx* ctx, int32_t* variant)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !variant) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *variant = ctx->variant;
+        }
+    );
+}
+
+int amr_solver_autotune(amr_ctx* ctx, uint32_t trial_steps, int32_t* chosen, float* best_us)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            const int v = ctx->autotune(trial_steps, best_us);
+            if (chosen) *chosen = v;
+        }
+    );
+}
+
+int amr_batch_create(amr_ctx* const* trees, uint32_t count, amr_batch** out)
+{
+    return guarded(
+        [&]
+        {
+            if (!trees || !out || count == 0) throw amr_error(AMR_ERR_ARGUMENT, "null argument or empty batch");
+            if (count > 65535) throw amr_error(AMR_ERR_ARGUMENT, "at most 65535 instances per batch");
+            *out = nullptr;
+            auto b  = std::make_unique<amr_batch>();
+            b->op   = trees[0]->op;
+            b->ni   = trees[0]->ni;
+            b->ndir = trees[0]->ndir;
+            for (uint32_t i = 0; i < count; ++i)
+            {
+                amr_ctx* t = trees[i];
+                if (!t) throw amr_error(AMR_ERR_ARGUMENT, "null tree");
+                if (t->op != b->op || t->depth != trees[0]->depth)
+                    throw amr_error(AMR_ERR_ARGUMENT, "batched trees must share rank, depth, patch size and halo");
+                t->ensure_usable();
+                t->sync();
+                b->src.push_back(t);
+                b->off.push_back(b->total);
+                b->total += t->n;
+            }
+            if (b->total > 0xFFFFFFFFull / 8) throw amr_error(AMR_ERR_CAPACITY, "batch too large");
+            AMR_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+            for (int k = 0; k < 2; ++k) b->u[k] = dev_alloc<double>(b->total * b->ni);
+            b->keys  = dev_alloc<uint64_t>(b->total);
+            b->links = dev_alloc<face_link>(b->total * b->ndir);
+            b->metas = dev_alloc<uint32_t>(b->total * b->ndir);
+            b->inst  = dev_alloc<uint16_t>(b->total);
+            b->table = dev_alloc<step_params>(count);
+            for (uint32_t i = 0; i < count; ++i)
+            {
+                amr_ctx const* t = b->src[i];
+                const uint64_t o = b->off[i];
+                AMR_HIP_CHECK(hipMemcpyAsync(b->u[0] + o * b->ni, t->u[t->cur], t->n * b->ni * sizeof(double),
+                                             hipMemcpyDeviceToDevice, b->stream));
+                AMR_HIP_CHECK(hipMemcpyAsync(b->keys + o, t->keys[t->kc], t->n * sizeof(uint64_t),
+                                             hipMemcpyDeviceToDevice, b->stream));
+                AMR_HIP_CHECK(hipMemcpyAsync(b->links + o * b->ndir, t->links, t->n * b->ndir * sizeof(face_link),
+                                             hipMemcpyDeviceToDevice, b->stream));
+                AMR_HIP_CHECK(hipMemcpyAsync(b->metas + o * b->ndir, t->metas, t->n * b->ndir * sizeof(uint32_t),
+                                             hipMemcpyDeviceToDevice, b->stream));
+                const uint64_t nl = t->n * b->ndir;
+                hipLaunchKernelGGL(k_offset_links, dim3(grid1d(nl, 256)), dim3(256), 0, b->stream, b->links + o * b->ndir,
+                                   nl, static_cast<int32_t>(o));
+                hipLaunchKernelGGL(k_fill_instance, dim3(grid1d(t->n, 256)), dim3(256), 0, b->stream, b->inst + o, t->n,
+                                   static_cast<uint16_t>(i));
+                check_launch("batch packing");
+                b->host_table.push_back(t->make_params(t->compute_dt()));
+            }
+            AMR_HIP_CHECK(hipMemcpyAsync(b->table, b->host_table.data(), count * sizeof(step_params),
+                                         hipMemcpyHostToDevice, b->stream));
+            AMR_HIP_CHECK(hipStreamSynchronize(b->stream));
+            *out = b.release();
+        }
+    );
+}
+
+int amr_batch_destroy(amr_batch* b)
+{
+    return guarded([&] { delete b; });
+}
+
+int amr_batch_size(amr_batch* b, uint64_t* total_leaves)
+{
+    return guarded(
+        [&]
+        {
+            if (!b || !total_leaves) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *total_leaves = b->total;
+        }
+    );
+}
+
+int amr_batch_advance(amr_batch* b, uint64_t steps, int use_graph)
+{
+    return guarded(
+        [&]
+        {
+            if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (steps) b->advance(steps, use_graph != 0);
+        }
+    );
+}
+
+int amr_batch_synchronize(amr_batch* b)
+{
+    return guarded(
+        [&]
+        {
+            if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            AMR_HIP_CHECK(hipStreamSynchronize(b->stream));
+        }
+    );
+}
+
+int amr_batch_scatter(amr_batch* b)
+{
+    return guarded(
+        [&]
+        {
+            if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            for (size_t i = 0; i < b->src.size(); ++i)
+            {
+                amr_ctx* t = b->src[i];
+                AMR_HIP_CHECK(hipMemcpyAsync(t->u[t->cur], b->u[b->cur] + b->off[i] * b->ni, t->n * b->ni * sizeof(double),
+                                             hipMemcpyDeviceToDevice, b->stream));
+            }
+            AMR_HIP_CHECK(hipStreamSynchronize(b->stream));
+        }
+    );
+}
+
+int amr_batch_stream(amr_batch* b, void** hip_stream)
+{
+    return guarded(
+        [&]
+        {
+            if (!b || !hip_stream) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *hip_stream = reinterpret_cast<void*>(b->stream);
+        }
+    );
+}
+
+int amr_batch_set_step_variant(amr_batch* b, int32_t variant)
+{
+    return guarded(
+        [&]
+        {
+            if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (variant < 0 || variant >= b->op->variant_count()) throw amr_error(AMR_ERR_ARGUMENT, "no such variant");
+            b->variant = variant;
+        }
+    );
+}
+
+int amr_batch_autotune(amr_batch* b, uint32_t trial_steps, int32_t* chosen, float* best_us)
+{
+    return guarded(
+        [&]
+        {
+            if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            hipEvent_t e0, e1;
+            AMR_HIP_CHECK(hipEventCreate(&e0));
+            AMR_HIP_CHECK(hipEventCreate(&e1));
+            int   best   = 0;
+            float best_t = std::numeric_limits<float>::max();
+            for (int v = 0; v < b->op->variant_count(); ++v)
+            {
+                auto trial = [&]
+                {
+                    b->op->step(v, b->u[b->cur], b->u[1 - b->cur], b->keys, b->links, b->metas, b->host_table[0],
+                                b->table, b->inst, static_cast<uint32_t>(b->total), b->stream);
+                };
+                trial();
+                AMR_HIP_CHECK(hipEventRecord(e0, b->stream));
+                for (uint32_t t = 0; t < trial_steps; ++t) trial();
+                AMR_HIP_CHECK(hipEventRecord(e1, b->stream));
+                AMR_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                AMR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                const float us = 1000.0f * ms / static_cast<float>(std::max(trial_steps, 1u));
+                if (us < best_t)
+                {
+                    best_t = us;
+                    best   = v;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            b->variant = best;
+            if (chosen) *chosen = best;
+            if (best_us) *best_us = best_t;
+        }
+    );
 }
 
 int amr_ctx_synchronize(amr_ctx* ctx)

@@ -29,6 +29,26 @@ struct step_params
     double dtdx[kMaxLevels][3];
 };
 
+// Output store of a step kernel.  The non-temporal form streams the new
+// state past L2 so that L2 keeps the input lines that neighbouring leaves'
+// ghost gathers re-read.
+template <bool NT>
+__device__ inline void store_cell(double* p, double v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+// Step constants of a leaf: the launch's own, or, for a batch of
+// concatenated trees, those of the leaf's instance.
+__device__ inline const step_params& params_for(const step_params& prm, const step_params* __restrict__ table,
+                                                const uint16_t* __restrict__ inst, uint32_t leaf)
+{
+    return table ? table[inst[leaf]] : prm;
+}
+
 // Ghost value of the cell with layout coordinates c (c[dim] lies outside
 // [0, N): >= N on the positive side, < 0 on the negative side) of the leaf
 // whose direction-`dir` neighbour record is (L, meta).  Restates
@@ -136,11 +156,28 @@ __host__ __device__ inline void stage_ghost(double* tile, const double* __restri
     tile[tile_shape<R, N>::idx(c)] = ghost_value<R, N>(u, links[e], metas[e], dir, c);
 }
 
-// Upwind update of interior cell i from the staged tile: the expression
-// order of amr_solver::time_step_cpu (amr_solver.hpp:302-347) and
+// Upwind update of one cell from its neighbours along each physical axis d
+// (lo[d] = u_L, hi[d] = u_R): the expression order of
+// amr_solver::time_step_cpu (amr_solver.hpp:302-347) and
 // AdvectionPhysics::rusanovFlux (AdvectionPhysics.hpp:45-66):
 //   total = 0; total -= dt/dx_d * (F(u, u_R) - F(u_L, u)) for d = x, y(, z);
 //   F(a, b) = 0.5 * (a v + b v) - (0.5 |v|) (b - a); out = u + total.
+template <int R>
+__host__ __device__ inline double upwind_update(double uc, const double* lo, const double* hi, const double* dtdx,
+                                                const double* vel, const double* hs)
+{
+    double total = 0.0;
+    for (int d = 0; d < R; ++d)
+    {
+        const double v  = vel[d];
+        const double fL = 0.5 * (lo[d] * v + uc * v) - hs[d] * (uc - lo[d]);
+        const double fR = 0.5 * (uc * v + hi[d] * v) - hs[d] * (hi[d] - uc);
+        total -= dtdx[d] * (fR - fL);
+    }
+    return uc + total;
+}
+
+// Upwind update of interior cell i from the staged tile.
 template <int R, int N>
 __host__ __device__ inline double upwind_cell(const double* tile, int i, const double* dtdx, const double* vel,
                                               const double* hs)
@@ -148,20 +185,15 @@ __host__ __device__ inline double upwind_cell(const double* tile, int i, const d
     using T = tile_shape<R, N>;
     int c[R];
     patch_shape<R, N>::coords(i, c);
-    const int    t     = T::idx(c);
-    const double uc    = tile[t];
-    double       total = 0.0;
+    const int t = T::idx(c);
+    double    lo[R], hi[R];
     for (int d = 0; d < R; ++d)
     {
-        const int    s  = T::stride(R - 1 - d); // physical axis d
-        const double ul = tile[t - s];
-        const double ur = tile[t + s];
-        const double v  = vel[d];
-        const double fL = 0.5 * (ul * v + uc * v) - hs[d] * (uc - ul);
-        const double fR = 0.5 * (uc * v + ur * v) - hs[d] * (ur - uc);
-        total -= dtdx[d] * (fR - fL);
+        const int s = T::stride(R - 1 - d); // physical axis d
+        lo[d]       = tile[t - s];
+        hi[d]       = tile[t + s];
     }
-    return uc + total;
+    return upwind_update<R>(tile[t], lo, hi, dtdx, vel, hs);
 }
 
 // max over layout dims of max(|u[c+s]-u[c]|, |u[c]-u[c-s]|) for cell i.
@@ -240,7 +272,9 @@ template <int R, int N, int NT>
 __global__ __launch_bounds__(NT) void k_step(const double* __restrict__ u, double* __restrict__ un,
                                              const uint64_t* __restrict__ keys,
                                              const face_link* __restrict__ links,
-                                             const uint32_t* __restrict__ metas, step_params prm, uint32_t nleaves)
+                                             const uint32_t* __restrict__ metas, step_params prm,
+                                             const step_params* __restrict__ table,
+                                             const uint16_t* __restrict__ inst, uint32_t nleaves)
 {
     using S = patch_shape<R, N>;
     __shared__ double tile[tile_shape<R, N>::TS];
@@ -248,16 +282,276 @@ __global__ __launch_bounds__(NT) void k_step(const double* __restrict__ u, doubl
     if (leaf >= nleaves) return;
     stage_tile<R, N, NT>(tile, u, leaf, links, metas);
     __syncthreads();
-    const int lv = key_level(keys[leaf]);
-    double    dtdx[R], vel[R], hs[R];
+    const step_params& P  = params_for(prm, table, inst, leaf);
+    const int          lv = key_level(keys[leaf]);
+    double             dtdx[R], vel[R], hs[R];
     for (int d = 0; d < R; ++d)
     {
-        dtdx[d] = prm.dtdx[lv][d];
-        vel[d]  = prm.vel[d];
-        hs[d]   = prm.hs[d];
+        dtdx[d] = P.dtdx[lv][d];
+        vel[d]  = P.vel[d];
+        hs[d]   = P.hs[d];
     }
     double* out = un + static_cast<size_t>(leaf) * S::NI;
     for (int i = threadIdx.x; i < S::NI; i += NT) out[i] = upwind_cell<R, N>(tile, i, dtdx, vel, hs);
+}
+
+// 3D 8^3 patches, one wave (64 lanes) per leaf.  Lane l owns the z-column at
+// (y, x) = (l / 8, l % 8) in registers: x- and y-neighbours come from lanes
+// l -+ 1 and l -+ 8, z-neighbours from the lane's own registers.  The six
+// ghost faces are gathered one face cell per lane; the -+z faces land in the
+// lane that needs them, the -+y / -+x faces go through 2 KiB of LDS.  With
+// no interior tile in LDS, 32 leaves are in flight per CU.
+template <bool NT>
+__global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, double* __restrict__ un,
+                                                 const uint64_t* __restrict__ keys,
+                                                 const face_link* __restrict__ links,
+                                                 const uint32_t* __restrict__ metas, step_params prm,
+                                                 const step_params* __restrict__ table,
+                                                 const uint16_t* __restrict__ inst, uint32_t nleaves)
+{
+    constexpr int NI = 512;
+    __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
+    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    if (leaf >= nleaves) return;
+    const int     lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
+    const double* up   = u + static_cast<size_t>(leaf) * NI;
+    double        r[8];
+#pragma unroll
+    for (int z = 0; z < 8; ++z) r[z] = up[z * 64 + lane];
+    const face_link* L = links + static_cast<size_t>(leaf) * 6;
+    const uint32_t*  M = metas + static_cast<size_t>(leaf) * 6;
+    int              c[3];
+    c[0] = -1, c[1] = hi3, c[2] = lo3; // (z, y, x) of the lane's -z ghost
+    const double gzm = ghost_value<3, 8>(u, L[0], M[0], 0, c);
+    c[0]             = 8;
+    const double gzp = ghost_value<3, 8>(u, L[1], M[1], 1, c);
+    c[0] = hi3, c[1] = -1, c[2] = lo3; // y faces: face cell (z, x)
+    gface[0][lane] = ghost_value<3, 8>(u, L[2], M[2], 2, c);
+    c[1]           = 8;
+    gface[1][lane] = ghost_value<3, 8>(u, L[3], M[3], 3, c);
+    c[0] = hi3, c[1] = lo3, c[2] = -1; // x faces: face cell (z, y)
+    gface[2][lane] = ghost_value<3, 8>(u, L[4], M[4], 4, c);
+    c[2]           = 8;
+    gface[3][lane] = ghost_value<3, 8>(u, L[5], M[5], 5, c);
+    __syncthreads();
+    const step_params& P       = params_for(prm, table, inst, leaf);
+    const int          lv      = key_level(keys[leaf]);
+    const double       dtdx[3] = { P.dtdx[lv][0], P.dtdx[lv][1], P.dtdx[lv][2] };
+    const int    y = hi3, x = lo3;
+    double*      out = un + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+    for (int z = 0; z < 8; ++z)
+    {
+        const double uc = r[z];
+        const double xl = __shfl_up(uc, 1, 64);
+        const double xr = __shfl_down(uc, 1, 64);
+        const double yl = __shfl_up(uc, 8, 64);
+        const double yr = __shfl_down(uc, 8, 64);
+        double       lo[3], hi[3];
+        lo[0] = (x == 0) ? gface[2][z * 8 + y] : xl;
+        hi[0] = (x == 7) ? gface[3][z * 8 + y] : xr;
+        lo[1] = (y == 0) ? gface[0][z * 8 + x] : yl;
+        hi[1] = (y == 7) ? gface[1][z * 8 + x] : yr;
+        if (z == 0)
+            lo[2] = gzm;
+        else
+            lo[2] = r[z > 0 ? z - 1 : 0];
+        if (z == 7)
+            hi[2] = gzp;
+        else
+            hi[2] = r[z < 7 ? z + 1 : 7];
+        store_cell<NT>(out + z * 64 + lane, upwind_update<3>(uc, lo, hi, dtdx, P.vel, P.hs));
+    }
+}
+
+// 3D 8^3 patches, one workgroup of 8 waves per group of 8 consecutive
+// leaves (a sibling octet wherever the tree is locally uniform), one wave per
+// leaf as in k_step_3d8.  Every wave publishes its six boundary faces in LDS;
+// a ghost face whose neighbour is a same-level leaf of the same group is read
+// from there, and only the faces leaving the group are gathered from HBM/L2.
+// In a uniform octree that halves the global ghost traffic.
+constexpr int kGroup3d = 8;
+
+template <bool NT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_3d8_group(
+                                                        const double* __restrict__ u, double* __restrict__ un,
+                                                        const uint64_t* __restrict__ keys,
+                                                        const face_link* __restrict__ links,
+                                                        const uint32_t* __restrict__ metas, step_params prm,
+                                                        const step_params* __restrict__ table,
+                                                        const uint16_t* __restrict__ inst, uint32_t nleaves)
+{
+    constexpr int NI = 512;
+    // own boundary faces, [wave][dir][cell]: dirs 0/1 = z=0/z=7 planes (cell
+    // y*8+x), 2/3 = y=0/y=7 (cell z*8+x), 4/5 = x=0/x=7 (cell z*8+y)
+    __shared__ double faces[kGroup3d][6][64];
+    // gathered ghost faces for the -+y / -+x directions, same cell indexing
+    __shared__ double gbuf[kGroup3d][4][64];
+    const uint32_t ngroups = (nleaves + kGroup3d - 1) / kGroup3d;
+    const uint32_t group   = xcd_item(blockIdx.x, gridDim.x);
+    if (group >= ngroups) return; // uniform over the workgroup
+    const int      w      = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t first  = group * kGroup3d;
+    const uint32_t leaf   = first + w;
+    const bool     active = leaf < nleaves;
+    const int      y = lane >> 3, x = lane & 7;
+    double         r[8];
+    double         gzm = 0.0, gzp = 0.0;
+    int            nb_wave[6]; // -1: gathered, else the group wave holding the neighbour
+    if (active)
+    {
+        const double* up = u + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+        for (int z = 0; z < 8; ++z) r[z] = up[z * 64 + lane];
+        faces[w][0][lane] = r[0];
+        faces[w][1][lane] = r[7];
+#pragma unroll
+        for (int z = 0; z < 8; ++z)
+        {
+            if (y == 0) faces[w][2][z * 8 + x] = r[z];
+            if (y == 7) faces[w][3][z * 8 + x] = r[z];
+            if (x == 0) faces[w][4][z * 8 + y] = r[z];
+            if (x == 7) faces[w][5][z * 8 + y] = r[z];
+        }
+        const face_link* L = links + static_cast<size_t>(leaf) * 6;
+        const uint32_t*  M = metas + static_cast<size_t>(leaf) * 6;
+#pragma unroll
+        for (int dir = 0; dir < 6; ++dir)
+        {
+            const int32_t nb = L[dir].ids[0];
+            nb_wave[dir]     = (rel_of(M[dir]) == REL_SAME && nb >= static_cast<int32_t>(first) &&
+                            nb < static_cast<int32_t>(first + kGroup3d))
+                                   ? nb - static_cast<int32_t>(first)
+                                   : -1;
+        }
+        int c[3];
+        if (nb_wave[0] < 0)
+        {
+            c[0] = -1, c[1] = y, c[2] = x;
+            gzm  = ghost_value<3, 8>(u, L[0], M[0], 0, c);
+        }
+        if (nb_wave[1] < 0)
+        {
+            c[0] = 8, c[1] = y, c[2] = x;
+            gzp  = ghost_value<3, 8>(u, L[1], M[1], 1, c);
+        }
+        // -+y faces: face cell (z, x) = (lane / 8, lane % 8)
+        if (nb_wave[2] < 0)
+        {
+            c[0] = y, c[1] = -1, c[2] = x;
+            gbuf[w][0][lane] = ghost_value<3, 8>(u, L[2], M[2], 2, c);
+        }
+        if (nb_wave[3] < 0)
+        {
+            c[0] = y, c[1] = 8, c[2] = x;
+            gbuf[w][1][lane] = ghost_value<3, 8>(u, L[3], M[3], 3, c);
+        }
+        // -+x faces: face cell (z, y) = (lane / 8, lane % 8)
+        if (nb_wave[4] < 0)
+        {
+            c[0] = y, c[1] = x, c[2] = -1;
+            gbuf[w][2][lane] = ghost_value<3, 8>(u, L[4], M[4], 4, c);
+        }
+        if (nb_wave[5] < 0)
+        {
+            c[0] = y, c[1] = x, c[2] = 8;
+            gbuf[w][3][lane] = ghost_value<3, 8>(u, L[5], M[5], 5, c);
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // a ghost of direction dir is the neighbour's face on the opposite side
+    if (nb_wave[0] >= 0) gzm = faces[nb_wave[0]][1][lane];
+    if (nb_wave[1] >= 0) gzp = faces[nb_wave[1]][0][lane];
+    const double* gym = nb_wave[2] >= 0 ? faces[nb_wave[2]][3] : gbuf[w][0];
+    const double* gyp = nb_wave[3] >= 0 ? faces[nb_wave[3]][2] : gbuf[w][1];
+    const double* gxm = nb_wave[4] >= 0 ? faces[nb_wave[4]][5] : gbuf[w][2];
+    const double* gxp = nb_wave[5] >= 0 ? faces[nb_wave[5]][4] : gbuf[w][3];
+    const step_params& P       = params_for(prm, table, inst, leaf);
+    const int          lv      = key_level(keys[leaf]);
+    const double       dtdx[3] = { P.dtdx[lv][0], P.dtdx[lv][1], P.dtdx[lv][2] };
+    double*       out     = un + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+    for (int z = 0; z < 8; ++z)
+    {
+        const double uc = r[z];
+        const double xl = __shfl_up(uc, 1, 64);
+        const double xr = __shfl_down(uc, 1, 64);
+        const double yl = __shfl_up(uc, 8, 64);
+        const double yr = __shfl_down(uc, 8, 64);
+        double       lo[3], hi[3];
+        lo[0] = (x == 0) ? gxm[z * 8 + y] : xl;
+        hi[0] = (x == 7) ? gxp[z * 8 + y] : xr;
+        lo[1] = (y == 0) ? gym[z * 8 + x] : yl;
+        hi[1] = (y == 7) ? gyp[z * 8 + x] : yr;
+        if (z == 0)
+            lo[2] = gzm;
+        else
+            lo[2] = r[z > 0 ? z - 1 : 0];
+        if (z == 7)
+            hi[2] = gzp;
+        else
+            hi[2] = r[z < 7 ? z + 1 : 7];
+        store_cell<NT>(out + z * 64 + lane, upwind_update<3>(uc, lo, hi, dtdx, P.vel, P.hs));
+    }
+}
+
+// 2D 16^2 patches, one wave per leaf.  Lane l owns x = l % 16 of rows
+// y = 4 (l / 16) .. 4 (l / 16) + 3; x-neighbours from lanes l -+ 1, the row
+// above / below a lane's four from lanes l -+ 16; the four ghost faces are
+// gathered one face cell per lane into 512 B of LDS.
+template <bool NT>
+__global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, double* __restrict__ un,
+                                                  const uint64_t* __restrict__ keys,
+                                                  const face_link* __restrict__ links,
+                                                  const uint32_t* __restrict__ metas, step_params prm,
+                                                  const step_params* __restrict__ table,
+                                                  const uint16_t* __restrict__ inst, uint32_t nleaves)
+{
+    constexpr int NI = 256;
+    __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
+    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    if (leaf >= nleaves) return;
+    const int     lane = threadIdx.x, x = lane & 15, q = lane >> 4;
+    const double* up   = u + static_cast<size_t>(leaf) * NI;
+    double        r[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = up[(4 * q + j) * 16 + x];
+    {
+        const int    f = q, t = x; // face f = direction f, face cell t
+        int          c[2];
+        const size_t e = static_cast<size_t>(leaf) * 4 + f;
+        c[0]           = (f == 0) ? -1 : (f == 1) ? 16 : t;
+        c[1]           = (f == 2) ? -1 : (f == 3) ? 16 : t;
+        gface[f][t]    = ghost_value<2, 16>(u, links[e], metas[e], f, c);
+    }
+    __syncthreads();
+    const step_params& P       = params_for(prm, table, inst, leaf);
+    const int          lv      = key_level(keys[leaf]);
+    const double       dtdx[2] = { P.dtdx[lv][0], P.dtdx[lv][1] };
+    const double above   = __shfl_up(r[3], 16, 64);  // row 4q - 1
+    const double below   = __shfl_down(r[0], 16, 64); // row 4q + 4
+    double*      out     = un + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+        const int    y  = 4 * q + j;
+        const double uc = r[j];
+        const double xl = __shfl_up(uc, 1, 64);
+        const double xr = __shfl_down(uc, 1, 64);
+        double       lo[2], hi[2];
+        lo[0] = (x == 0) ? gface[2][y] : xl;
+        hi[0] = (x == 15) ? gface[3][y] : xr;
+        if (j > 0)
+            lo[1] = r[j > 0 ? j - 1 : 0];
+        else
+            lo[1] = (q == 0) ? gface[0][x] : above;
+        if (j < 3)
+            hi[1] = r[j < 3 ? j + 1 : 3];
+        else
+            hi[1] = (q == 3) ? gface[1][x] : below;
+        store_cell<NT>(out + y * 16 + x, upwind_update<2>(uc, lo, hi, dtdx, P.vel, P.hs));
+    }
 }
 
 // Padded view with H ghost layers (the reference's patch after

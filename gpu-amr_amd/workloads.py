@@ -1,0 +1,91 @@
+"""Synthetic workloads of the benchmark configurations, built through the
+public API (used by bench.py and the tests).
+
+C3: 3D uniform octree of depth 5, 8^3 FP64 patches, seeded Gaussian bump,
+    v = (1, 0.5, 0.25), CFL 0.3.
+C4: 64 independent 2D instances; instance i (seed 1000 + i) starts uniform at
+    depth 3 and is refined four times with each leaf flagged Refine with
+    probability 0.35 (levels capped at 7, 2:1 enforced by the regrid), carries
+    a seeded mixture of 8 Gaussian bumps (amplitudes U[0.5, 1], sigmas
+    U[0.03, 0.1], centres U[0, 1]^2) and a velocity drawn from U[-1, 1]^2
+    (redrawn while min |v_d| < 1e-3); CFL 0.4.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def c3_dense_ic(ncell=256, seed=42):
+    """Seeded 3D Gaussian bump (amplitude 1, sigma 0.1, centre U[.25,.75]^3)
+    at the cell centres of the uniform ncell^3 grid; array order z, y, x."""
+    c = np.random.default_rng(seed).uniform(0.25, 0.75, 3)
+    h = 1.0 / ncell
+    x = (np.arange(ncell) * h) + 0.5 * h
+    gx = np.exp(-((x - c[0]) ** 2) / (2 * 0.1**2))
+    gy = np.exp(-((x - c[1]) ** 2) / (2 * 0.1**2))
+    gz = np.exp(-((x - c[2]) ** 2) / (2 * 0.1**2))
+    return gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+
+
+def dense_to_leaves(amr, keys, dense, n, depth, rank=3):
+    """Cut a dense uniform-level array (layout order) into leaf interiors."""
+    coords, lv = amr.decode_keys(keys, rank)
+    s = 1 << (depth - int(lv[0]))
+    nb = dense.shape[0] // n
+    if rank == 3:
+        blocks = dense.reshape(nb, n, nb, n, nb, n).transpose(0, 2, 4, 1, 3, 5)
+        return np.ascontiguousarray(blocks[coords[:, 2] // s, coords[:, 1] // s, coords[:, 0] // s])
+    blocks = dense.reshape(nb, n, nb, n).transpose(0, 2, 1, 3)
+    return np.ascontiguousarray(blocks[coords[:, 1] // s, coords[:, 0] // s])
+
+
+def build_c3(amr, depth=5, n=8):
+    tree = amr.NdTree(3, depth, n, 1, 8 ** depth)
+    solver = amr.AmrSolver(tree, (1.0, 0.5, 0.25), 0.3)
+    for _ in range(depth):
+        tree.refine_all()
+    tree.set_interior(dense_to_leaves(amr, tree.keys(), c3_dense_ic(n * (1 << depth)), n, depth))
+    return tree, solver
+
+
+def mixture(rng, bumps=8):
+    amp = rng.uniform(0.5, 1.0, bumps)
+    sig = rng.uniform(0.03, 0.1, bumps)
+    cen = rng.uniform(0.0, 1.0, (bumps, 2))
+
+    def ic(c):
+        u = np.zeros(c.shape[:-1])
+        for a, s, p in zip(amp, sig, cen):
+            u += a * np.exp(-((c[..., 0] - p[0]) ** 2 + (c[..., 1] - p[1]) ** 2) / (2 * s * s))
+        return u
+
+    return ic
+
+
+def c4_velocity(rng):
+    while True:
+        v = rng.uniform(-1.0, 1.0, 2)
+        if np.min(np.abs(v)) >= 1e-3:
+            return (float(v[0]), float(v[1]))
+
+
+def c4_instance(amr, i, max_depth=7, rounds=4, p_refine=0.35, start_level=3, capacity=16384 + 64):
+    """Instance i of the batched configuration.  Returns (tree, solver,
+    flag history, velocity, ic) so that the same tree can be rebuilt
+    elsewhere (the oracle) by replaying the flags."""
+    rng = np.random.default_rng(1000 + i)
+    tree = amr.NdTree(2, 7, 16, 1, capacity)
+    for _ in range(start_level):
+        tree.refine_all()
+    history = []
+    for _ in range(rounds):
+        keys = tree.keys()
+        lv = (keys & np.uint64(63)).astype(np.int64)
+        flags = np.where((rng.random(keys.size) < p_refine) & (lv < max_depth), 1, 0).astype(np.int8)
+        history.append(flags)
+        tree.reconstruct_tree(flags)
+    vel = c4_velocity(rng)
+    ic = mixture(rng)
+    solver = amr.AmrSolver(tree, vel, 0.4)
+    solver.initialize(ic)
+    return tree, solver, history, vel, ic

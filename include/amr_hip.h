@@ -165,8 +165,47 @@ int amr_solver_finish_advance_batch(amr_ctx* ctx, double* dt_sum, uint64_t* exec
  * (amr_solver.hpp:149-153). */
 int amr_solver_advance(amr_ctx* ctx, double* dt);
 
+/* ---- step-kernel autotuner (no counterpart in the reference) ----------- */
+
+/* Number / name of the step-kernel variants built for this patch shape.  All
+ * variants compute bit-identical results; they differ in the leaf-to-
+ * workgroup mapping and in how many workgroups share a CU. */
+int amr_solver_step_variants(amr_ctx* ctx, int32_t* count);
+int amr_solver_step_variant_name(amr_ctx* ctx, int32_t variant, const char** name);
+int amr_solver_set_step_variant(amr_ctx* ctx, int32_t variant);
+int amr_solver_get_step_variant(amr_ctx* ctx, int32_t* variant);
+/* Times every variant for trial_steps launches on the current tree (reading
+ * the current state, writing only the scratch buffer: the solution is not
+ * changed), selects the fastest and returns its index and time per step. */
+int amr_solver_autotune(amr_ctx* ctx, uint32_t trial_steps, int32_t* chosen, float* best_us);
+
 /* Waits for all work queued on the context stream. */
 int amr_ctx_synchronize(amr_ctx* ctx);
+
+/* ---- batched instances (no counterpart in the reference, whose drivers run
+ * one tree per process) -------------------------------------------------- */
+
+typedef struct amr_batch amr_batch;
+
+/* Concatenates the current state of `count` trees of the same rank, depth,
+ * patch size and halo into one leaf list whose time steps run as one launch
+ * over all leaves; every instance keeps its own velocity, cfl and dt (fixed
+ * at creation from its finest level).  The trees are not modified until
+ * amr_batch_scatter. */
+int amr_batch_create(amr_ctx* const* trees, uint32_t count, amr_batch** out);
+int amr_batch_destroy(amr_batch* batch);
+int amr_batch_size(amr_batch* batch, uint64_t* total_leaves);
+/* Every instance advances `steps` steps.  use_graph != 0 captures the steps
+ * in a hipGraph once (per step count, buffer parity and kernel variant) and
+ * replays it as a single graph launch.  Returns without waiting. */
+int amr_batch_advance(amr_batch* batch, uint64_t steps, int use_graph);
+int amr_batch_synchronize(amr_batch* batch);
+/* Writes each instance's current interiors back into its source tree. */
+int amr_batch_scatter(amr_batch* batch);
+int amr_batch_stream(amr_batch* batch, void** hip_stream);
+int amr_batch_set_step_variant(amr_batch* batch, int32_t variant);
+/* As amr_solver_autotune, over the whole batch. */
+int amr_batch_autotune(amr_batch* batch, uint32_t trial_steps, int32_t* chosen, float* best_us);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -16,13 +16,7 @@ _lib = None
 
 
 def build():
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [
-        "/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++20", "-fPIC", "-shared",
-        "-ffp-contract=off", f"-I{ROOT / 'gpu-amr_amd' / 'csrc'}", f"-I{ROOT / 'include'}", str(SRC), "-o", str(tmp),
-    ]
-    subprocess.run(cmd, check=True, capture_output=True)
-    tmp.replace(LIB)
+    subprocess.run(["make", "-s", "-C", str(HERE / "cpp"), str(LIB)], check=True, capture_output=True)
 
 
 def load():

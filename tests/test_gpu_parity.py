@@ -314,3 +314,55 @@ def test_capacity_and_root_errors(amr):
     with pytest.raises(amr.AmrError) as e:
         amr.NdTree(2, 7, 12, 1, 10)
     assert e.value.code == amr.AMR_ERR_UNSUPPORTED
+
+
+def test_reference_tests_through_cpp_api():
+    """tests/advection_equation_amr.t.cpp ported to the amr::hip C++ API."""
+    import subprocess
+
+    from amrlib import ROOT
+
+    say("case: C++ host API port of the reference's advection tests")
+    exe = ROOT / "tests" / "cpp" / "reference_advection_tests.bin"
+    assert exe.exists(), "build the native tests first (__graft_entry__.build())"
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
+    say(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("rank,n,levels,mixed", [(3, 8, 3, False), (3, 8, 3, True), (2, 16, 4, True)])
+def test_step_variants_bit_identical(amr, rank, n, levels, mixed):
+    """Every step-kernel variant (and the autotuner's pick) reproduces the
+    oracle bit for bit, on uniform and on mixed-level trees."""
+    say(f"case: step variants rank={rank} n={n} levels={levels} mixed={mixed}")
+    depth = 5 if rank == 3 else 7
+    vel = (1.0, 0.5, 0.25)[:rank]
+    cap = 8**levels + 4096 if rank == 3 else 4**levels + 4096
+    g, s, o = pair(amr, rank, n, 1, depth, cap, vel, 0.3)
+    for _ in range(levels):
+        g.refine_all()
+        o.refine_all()
+    if mixed:
+        flags = np.zeros(g.size(), dtype=np.int8)
+        flags[::7] = 1
+        g.reconstruct_tree(flags)
+        o.reconstruct(flags)
+        assert_topology(g, o, "mixed tree")
+    rng = np.random.default_rng(17)
+    u0 = rng.random(g.size() * n**rank)
+    o.set_interior(u0)
+    o.halo_exchange()
+    o.advance_batch(3)
+    exp = o.get_interior()
+    for v, name in enumerate(s.step_variants()):
+        s.set_step_variant(v)
+        g.set_interior(u0)
+        s.advance_batch_async(3)
+        s.finish_advance_batch()
+        assert_fields(g.get_interior(), exp, f"variant {name}")
+    best, us = s.autotune(5)
+    assert s.step_variant() == best and us > 0
+    g.set_interior(u0)
+    s.advance_batch_async(3)
+    s.finish_advance_batch()
+    assert_fields(g.get_interior(), exp, "autotuned variant")
