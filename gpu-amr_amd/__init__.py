@@ -99,7 +99,27 @@ _SIGS = {
     "amr_batch_stream": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)],
     "amr_batch_set_step_variant": [ctypes.c_void_p, ctypes.c_int32],
     "amr_batch_autotune": [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
+    "amr_tape_record": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                        ctypes.POINTER(ctypes.c_void_p)],
+    "amr_tape_destroy": [ctypes.c_void_p],
+    "amr_tape_get_info": [ctypes.c_void_p, ctypes.c_void_p],
+    "amr_tape_get_dts": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tape_get_regrid": [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                            ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
+                            ctypes.c_uint64],
+    "amr_tape_get_keys": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                          ctypes.POINTER(ctypes.c_uint64)],
+    "amr_tape_forward": [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p],
+    "amr_tape_adjoint": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)],
 }
+
+
+class TapeInfo(ctypes.Structure):
+    _fields_ = [
+        (name, ctypes.c_uint64)
+        for name in ("steps", "checkpoint_every", "checkpoints", "regrids", "topologies", "initial_leaves",
+                     "final_leaves", "max_leaves", "cells_per_leaf")
+    ]
 
 
 def exported_symbols():
@@ -392,6 +412,69 @@ class Batch:
         v, us = ctypes.c_int32(), ctypes.c_float()
         _check(self.lib.amr_batch_autotune(self.h, int(trial_steps), ctypes.byref(v), ctypes.byref(us)))
         return v.value, us.value
+
+
+class Tape:
+    """Recorded forward solve for the discrete adjoint (amr_tape_* in
+    include/amr_hip.h).  Recording advances the solver's tree `steps` steps,
+    regridding with `criterion` before every step k > 0 with
+    k % regrid_every == 0, and keeps the state before every
+    checkpoint_every-th step.  forward()/adjoint() take device pointers
+    (ints) -- torch tensors on the GPU pass tensor.data_ptr()."""
+
+    def __init__(self, solver: AmrSolver, steps, criterion: Criterion | None = None, regrid_every=0,
+                 checkpoint_every=20):
+        self.lib = load()
+        self.tree = solver.tree
+        h = ctypes.c_void_p()
+        crit = ctypes.byref(criterion) if criterion is not None else None
+        _check(self.lib.amr_tape_record(self.tree.h, crit, int(steps), int(regrid_every), int(checkpoint_every),
+                                        ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.amr_tape_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        i = TapeInfo()
+        _check(self.lib.amr_tape_get_info(self.h, ctypes.byref(i)))
+        return {name: int(getattr(i, name)) for name, _ in TapeInfo._fields_}
+
+    def dts(self):
+        n = self.info()["steps"]
+        out = np.zeros(n)
+        _check(self.lib.amr_tape_get_dts(self.h, _vp(out), n))
+        return out
+
+    def regrid(self, index):
+        """(step, leaves before, changed, flags) of recorded regrid `index`."""
+        step, nb, ch = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        _check(self.lib.amr_tape_get_regrid(self.h, int(index), ctypes.byref(step), ctypes.byref(nb), ctypes.byref(ch),
+                                            None, 0))
+        flags = np.zeros(nb.value, dtype=np.int8)
+        _check(self.lib.amr_tape_get_regrid(self.h, int(index), None, None, None, _vp(flags), flags.size))
+        return int(step.value), int(nb.value), bool(ch.value), flags
+
+    def keys(self, topology):
+        n = ctypes.c_uint64()
+        _check(self.lib.amr_tape_get_keys(self.h, int(topology), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.uint64)
+        _check(self.lib.amr_tape_get_keys(self.h, int(topology), _vp(out), out.size, ctypes.byref(n)))
+        return out
+
+    def forward(self, u0_ptr, velocity, uT_ptr):
+        v = (ctypes.c_double * 3)(*(list(velocity) + [0.0] * (3 - len(velocity))))
+        _check(self.lib.amr_tape_forward(self.h, ctypes.c_void_p(u0_ptr), v, ctypes.c_void_p(uT_ptr)))
+
+    def adjoint(self, lamT_ptr, grad_u0_ptr):
+        g = (ctypes.c_double * 3)()
+        _check(self.lib.amr_tape_adjoint(self.h, ctypes.c_void_p(lamT_ptr), ctypes.c_void_p(grad_u0_ptr), g))
+        return np.array(g[:])
 
 
 def make_criterion(kind, refine_threshold, coarsen_threshold, min_level, max_level):

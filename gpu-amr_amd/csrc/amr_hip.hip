@@ -7,6 +7,7 @@
 // kernels_step.hpp / kernels_tree.hpp.
 #include "amr_hip.h"
 #include "device_common.hpp"
+#include "kernels_adjoint.hpp"
 #include "kernels_step.hpp"
 #include "kernels_tree.hpp"
 
@@ -69,6 +70,10 @@ struct ops
                       int, int, int8_t*, uint32_t, hipStream_t);
     void (*emit)(const uint64_t*, const double*, uint32_t, int, const uint8_t*, const uint8_t*, const uint32_t*,
                  const uint32_t*, uint64_t*, double*, regrid_summary*, hipStream_t);
+    void (*adj_step)(const double*, const double*, const uint64_t*, const face_link*, const uint32_t*,
+                     adjoint_params const&, double*, double*, uint32_t, hipStream_t);
+    void (*adj_regrid)(const double*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*, double*, uint32_t,
+                       hipStream_t);
 };
 
 template <int R, int N, int H>
@@ -160,9 +165,26 @@ struct instance
                            keys_out, u_out, summary);
         check_launch("k_emit");
     }
+    static void adj_step(const double* lam_next, const double* u, const uint64_t* keys, const face_link* links,
+                         const uint32_t* metas, adjoint_params const& prm, double* lam, double* vpart, uint32_t n,
+                         hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_adj_step<R, N, NT>), dim3(n), dim3(NT), 0, s, lam_next, u, keys, links, metas, prm, lam,
+                           vpart, n);
+        check_launch("k_adj_step");
+    }
+    static void adj_regrid(const double* lam_new, const uint8_t* mark, const uint8_t* head, const uint32_t* counts,
+                           const uint32_t* pos, double* lam_old, uint32_t n_old, hipStream_t s)
+    {
+        if (n_old == 0) return;
+        hipLaunchKernelGGL((k_adj_regrid<R, N, NT>), dim3(n_old), dim3(NT), 0, s, lam_new, mark, head, counts, pos,
+                           lam_old, n_old);
+        check_launch("k_adj_regrid");
+    }
     static constexpr ops table()
     {
-        return ops{ R, N, H, &step, &variant_count, &variant_name, &padded, &criterion, &emit };
+        return ops{ R, N, H, &step, &variant_count, &variant_name, &padded, &criterion, &emit, &adj_step, &adj_regrid };
     }
 };
 
@@ -217,6 +239,37 @@ T* dev_alloc(size_t count)
 }
 
 inline unsigned grid1d(uint64_t items, unsigned threads) { return static_cast<unsigned>((items + threads - 1) / threads); }
+
+// Times `trials` launches of every variant, three interleaved rounds, and
+// returns the variant with the lowest per-launch time (min over rounds).
+template <typename Launch>
+int pick_fastest(int nvar, Launch&& launch, hipStream_t stream, uint32_t trials, float* best_us)
+{
+    hipEvent_t e0, e1;
+    AMR_HIP_CHECK(hipEventCreate(&e0));
+    AMR_HIP_CHECK(hipEventCreate(&e1));
+    std::vector<float> best(static_cast<size_t>(nvar), std::numeric_limits<float>::max());
+    trials = std::max(trials, 1u);
+    for (int round = 0; round < 3; ++round)
+    {
+        for (int v = 0; v < nvar; ++v)
+        {
+            launch(v); // warm
+            AMR_HIP_CHECK(hipEventRecord(e0, stream));
+            for (uint32_t t = 0; t < trials; ++t) launch(v);
+            AMR_HIP_CHECK(hipEventRecord(e1, stream));
+            AMR_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            AMR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best[static_cast<size_t>(v)] = std::min(best[static_cast<size_t>(v)], 1000.0f * ms / static_cast<float>(trials));
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const auto it = std::min_element(best.begin(), best.end());
+    if (best_us) *best_us = *it;
+    return static_cast<int>(it - best.begin());
+}
 
 // Batch packing: shift an instance's neighbour slots by its leaf offset, and
 // tag its leaves with the instance index.
@@ -362,33 +415,15 @@ struct amr_ctx
     {
         ensure_usable();
         const step_params prm = make_params(compute_dt());
-        hipEvent_t        e0, e1;
-        AMR_HIP_CHECK(hipEventCreate(&e0));
-        AMR_HIP_CHECK(hipEventCreate(&e1));
-        int   best   = 0;
-        float best_t = std::numeric_limits<float>::max();
-        for (int v = 0; v < op->variant_count(); ++v)
-        {
-            op->step(v, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), stream);
-            AMR_HIP_CHECK(hipEventRecord(e0, stream));
-            for (uint32_t t = 0; t < trials; ++t)
-                op->step(v, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), stream);
-            AMR_HIP_CHECK(hipEventRecord(e1, stream));
-            AMR_HIP_CHECK(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            AMR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            const float us = 1000.0f * ms / static_cast<float>(std::max(trials, 1u));
-            if (us < best_t)
+        variant               = pick_fastest(
+            op->variant_count(),
+            [&](int v)
             {
-                best_t = us;
-                best   = v;
-            }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        variant = best;
-        if (best_us) *best_us = best_t;
-        return best;
+                op->step(v, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr,
+                         static_cast<uint32_t>(n), stream);
+            },
+            stream, trials, best_us);
+        return variant;
     }
 
     void scan(uint32_t count)
@@ -542,6 +577,292 @@ struct amr_batch
     }
 };
 
+// Refine flags of the criterion on the current state, into ctx->flags.
+static void run_criterion(amr_ctx* ctx, amr_criterion const* crit)
+{
+    ctx->ensure_usable();
+    if (!crit) throw amr_error(AMR_ERR_ARGUMENT, "null criterion");
+    if (crit->kind != AMR_CRITERION_MAX_VALUE && crit->kind != AMR_CRITERION_GRADIENT)
+        throw amr_error(AMR_ERR_ARGUMENT, "unknown criterion kind");
+    ctx->op->criterion(crit->kind, ctx->u[ctx->cur], ctx->keys[ctx->kc], ctx->links, ctx->metas,
+                       crit->refine_threshold, crit->coarsen_threshold, crit->min_level, crit->max_level, ctx->flags,
+                       static_cast<uint32_t>(ctx->n), ctx->stream);
+}
+
+// The recorded forward pass of a solve with regrids, for the discrete
+// adjoint: every topology the solve went through, the dt of every step, the
+// refine flags and old->new maps of every regrid, and the state entering
+// every `ckpt_every`-th step.  Independent of the context once recorded.
+// forward() replays the recorded topology and dt sequence from a given
+// initial state and velocity (refreshing the checkpoints); adjoint()
+// differentiates the most recent forward pass (the recording or a replay).
+struct amr_tape
+{
+    struct topology
+    {
+        uint64_t   n     = 0;
+        uint64_t*  keys  = nullptr;
+        face_link* links = nullptr;
+        uint32_t*  metas = nullptr;
+    };
+    struct regrid_record
+    {
+        uint64_t            step    = 0;
+        bool                changed = false;
+        uint64_t            n_old = 0, n_new = 0;
+        int                 topo_before = 0;
+        std::vector<int8_t> flags;
+        uint8_t*            mark   = nullptr;
+        uint8_t*            head   = nullptr;
+        uint32_t*           counts = nullptr;
+        uint32_t*           pos    = nullptr;
+    };
+
+    ops const*                 op = nullptr;
+    amr_tree_config            cfg{};
+    int                        rank = 0, depth = 0, ni = 0, ndir = 0, variant = 0;
+    hipStream_t                stream{};
+    uint64_t                   steps = 0, ckpt_every = 1;
+    double                     vel[3]{};
+    std::vector<topology>      topos;
+    std::vector<int>           topo_of_step;   // topology index of step k
+    std::vector<int>           regrid_at_step; // regrid record before step k, or -1
+    std::vector<double>        dts;
+    std::vector<regrid_record> regrids;
+    std::vector<double*>       ckpts; // state entering step s * ckpt_every
+    uint64_t                   max_n = 0;
+    // work buffers (max_n leaves each)
+    double*                    lam[2]{};
+    double*                    work  = nullptr;
+    double*                    vpart = nullptr; // ckpt_every * max_n triples
+    double*                    chunk = nullptr;
+    double*                    acc   = nullptr;
+    uint64_t*                  scratch_keys    = nullptr;
+    regrid_summary*            scratch_summary = nullptr;
+    std::vector<double*>       seg; // recomputed states of one segment
+
+    ~amr_tape()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        std::vector<void*> ptrs = { lam[0], lam[1], work, vpart, chunk, acc, scratch_keys, scratch_summary };
+        for (auto& t : topos) ptrs.insert(ptrs.end(), { t.keys, t.links, t.metas });
+        for (auto& r : regrids) ptrs.insert(ptrs.end(), { r.mark, r.head, r.counts, r.pos });
+        for (double* c : ckpts) ptrs.push_back(c);
+        for (double* s : seg) ptrs.push_back(s);
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    double cell_size(int level, int axis) const
+    {
+        const double pm   = static_cast<double>(1u << (depth - level));
+        const double maxc = static_cast<double>(1u << depth);
+        return (cfg.domain_length[axis] * pm / maxc) / static_cast<double>(cfg.patch_size);
+    }
+
+    step_params step_consts(double dt) const
+    {
+        step_params p{};
+        for (int d = 0; d < 3; ++d)
+        {
+            p.vel[d] = vel[d];
+            p.hs[d]  = 0.5 * std::abs(vel[d]);
+        }
+        for (int l = 0; l <= depth; ++l)
+            for (int d = 0; d < rank; ++d) p.dtdx[l][d] = dt / cell_size(l, d);
+        return p;
+    }
+
+    adjoint_params adjoint_consts(double dt) const
+    {
+        adjoint_params p{};
+        for (int d = 0; d < 3; ++d)
+        {
+            p.vp[d]  = 0.5 * (vel[d] + std::abs(vel[d]));
+            p.vm[d]  = 0.5 * (vel[d] - std::abs(vel[d]));
+            p.sgn[d] = vel[d] > 0.0 ? 1.0 : (vel[d] < 0.0 ? -1.0 : 0.0);
+        }
+        for (int l = 0; l <= depth; ++l)
+            for (int d = 0; d < rank; ++d) p.dtdx[l][d] = dt / cell_size(l, d);
+        return p;
+    }
+
+    topology const& topo(uint64_t k) const { return topos[static_cast<size_t>(topo_of_step[k])]; }
+    size_t          field_bytes(uint64_t leaves) const { return leaves * static_cast<size_t>(ni) * sizeof(double); }
+
+    void snapshot_topology(amr_ctx const* c)
+    {
+        topology t;
+        t.n     = c->n;
+        t.keys  = dev_alloc<uint64_t>(c->n);
+        t.links = dev_alloc<face_link>(c->n * ndir);
+        t.metas = dev_alloc<uint32_t>(c->n * ndir);
+        AMR_HIP_CHECK(hipMemcpyAsync(t.keys, c->keys[c->kc], c->n * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+        AMR_HIP_CHECK(hipMemcpyAsync(t.links, c->links, c->n * ndir * sizeof(face_link), hipMemcpyDeviceToDevice, c->stream));
+        AMR_HIP_CHECK(hipMemcpyAsync(t.metas, c->metas, c->n * ndir * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+        topos.push_back(t);
+        max_n = std::max(max_n, c->n);
+    }
+
+    // Records `count` steps of the context's solve, regridding with `crit`
+    // before every step k > 0 with k % regrid_every == 0.  The context ends
+    // at the final state.
+    void record(amr_ctx* c, const amr_criterion* crit, uint64_t count, uint64_t regrid_every)
+    {
+        steps = count;
+        for (int d = 0; d < 3; ++d) vel[d] = c->vel[d];
+        snapshot_topology(c);
+        topo_of_step.assign(count, 0);
+        regrid_at_step.assign(count, -1);
+        dts.assign(count, 0.0);
+        for (uint64_t k = 0; k < count; ++k)
+        {
+            if (crit && regrid_every > 0 && k > 0 && k % regrid_every == 0)
+            {
+                regrid_record r;
+                r.step        = k;
+                r.n_old       = c->n;
+                r.topo_before = static_cast<int>(topos.size()) - 1;
+                run_criterion(c, crit);
+                r.flags.resize(c->n);
+                AMR_HIP_CHECK(hipMemcpyAsync(r.flags.data(), c->flags, c->n, hipMemcpyDeviceToHost, c->stream));
+                c->sync();
+                r.changed = c->regrid();
+                r.n_new   = c->n;
+                if (r.changed)
+                {
+                    r.mark   = dev_alloc<uint8_t>(r.n_old);
+                    r.head   = dev_alloc<uint8_t>(r.n_old);
+                    r.counts = dev_alloc<uint32_t>(r.n_old);
+                    r.pos    = dev_alloc<uint32_t>(r.n_old);
+                    AMR_HIP_CHECK(hipMemcpyAsync(r.mark, c->mark, r.n_old, hipMemcpyDeviceToDevice, c->stream));
+                    AMR_HIP_CHECK(hipMemcpyAsync(r.head, c->head, r.n_old, hipMemcpyDeviceToDevice, c->stream));
+                    AMR_HIP_CHECK(hipMemcpyAsync(r.counts, c->counts, r.n_old * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+                    AMR_HIP_CHECK(hipMemcpyAsync(r.pos, c->pos, r.n_old * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+                    snapshot_topology(c);
+                }
+                regrid_at_step[k] = static_cast<int>(regrids.size());
+                regrids.push_back(std::move(r));
+            }
+            topo_of_step[k] = static_cast<int>(topos.size()) - 1;
+            if (k % ckpt_every == 0)
+            {
+                ckpts.push_back(dev_alloc<double>(c->n * ni));
+                AMR_HIP_CHECK(hipMemcpyAsync(ckpts.back(), c->u[c->cur], field_bytes(c->n), hipMemcpyDeviceToDevice, c->stream));
+            }
+            dts[k] = c->compute_dt();
+            c->step(dts[k]);
+        }
+        c->sync();
+        // work buffers
+        for (int b = 0; b < 2; ++b) lam[b] = dev_alloc<double>(max_n * ni);
+        work            = dev_alloc<double>(max_n * ni);
+        vpart           = dev_alloc<double>(ckpt_every * max_n * 3);
+        chunk           = dev_alloc<double>(kReduceBlocks * 3);
+        acc             = dev_alloc<double>(3);
+        scratch_keys    = dev_alloc<uint64_t>(max_n);
+        scratch_summary = dev_alloc<regrid_summary>(1);
+        for (uint64_t j = 1; j < ckpt_every; ++j) seg.push_back(dev_alloc<double>(max_n * ni));
+    }
+
+    regrid_record const* regrid_before(uint64_t k) const
+    {
+        const int r = regrid_at_step[k];
+        return (r >= 0 && regrids[static_cast<size_t>(r)].changed) ? &regrids[static_cast<size_t>(r)] : nullptr;
+    }
+
+    // The recorded regrid transfer (old -> new leaves) applied to src.
+    void replay_transfer(regrid_record const& r, const double* src, double* dst)
+    {
+        op->emit(topos[static_cast<size_t>(r.topo_before)].keys, src, static_cast<uint32_t>(r.n_old), depth, r.mark,
+                 r.head, r.counts, r.pos, scratch_keys, dst, scratch_summary, stream);
+    }
+
+    void forward_step(uint64_t k, const double* src, double* dst)
+    {
+        topology const& t = topo(k);
+        op->step(variant, src, dst, t.keys, t.links, t.metas, step_consts(dts[k]), nullptr, nullptr,
+                 static_cast<uint32_t>(t.n), stream);
+    }
+
+    void forward(const double* u0, const double* v, double* uT)
+    {
+        AMR_HIP_CHECK(hipDeviceSynchronize()); // inputs written on other streams
+        for (int d = 0; d < 3; ++d) vel[d] = d < rank ? v[d] : 0.0;
+        double* a = lam[0];
+        double* b = lam[1];
+        AMR_HIP_CHECK(hipMemcpyAsync(a, u0, field_bytes(topos[0].n), hipMemcpyDeviceToDevice, stream));
+        for (uint64_t k = 0; k < steps; ++k)
+        {
+            if (auto const* r = regrid_before(k))
+            {
+                replay_transfer(*r, a, b);
+                std::swap(a, b);
+            }
+            if (k % ckpt_every == 0)
+                AMR_HIP_CHECK(hipMemcpyAsync(ckpts[k / ckpt_every], a, field_bytes(topo(k).n), hipMemcpyDeviceToDevice, stream));
+            forward_step(k, a, b);
+            std::swap(a, b);
+        }
+        AMR_HIP_CHECK(hipMemcpyAsync(uT, a, field_bytes(topo(steps - 1).n), hipMemcpyDeviceToDevice, stream));
+        AMR_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // Gradient of lamT . u(T) with respect to u0 and v (dt held fixed),
+    // recomputing the states inside each checkpoint segment.
+    void adjoint(const double* lamT, double* grad_u0, double* grad_v)
+    {
+        AMR_HIP_CHECK(hipDeviceSynchronize());
+        double* l  = lam[0];
+        double* lo = lam[1];
+        AMR_HIP_CHECK(hipMemcpyAsync(l, lamT, field_bytes(topo(steps - 1).n), hipMemcpyDeviceToDevice, stream));
+        AMR_HIP_CHECK(hipMemsetAsync(acc, 0, 3 * sizeof(double), stream));
+        std::vector<const double*> states(ckpt_every);
+        const uint64_t             nseg = (steps + ckpt_every - 1) / ckpt_every;
+        for (uint64_t s = nseg; s-- > 0;)
+        {
+            const uint64_t c = s * ckpt_every, e = std::min(steps, c + ckpt_every);
+            states[0]        = ckpts[s];
+            for (uint64_t k = c; k + 1 < e; ++k)
+            {
+                double* next = seg[k - c];
+                if (auto const* r = regrid_before(k + 1))
+                {
+                    forward_step(k, states[k - c], work);
+                    replay_transfer(*r, work, next);
+                }
+                else
+                    forward_step(k, states[k - c], next);
+                states[k + 1 - c] = next;
+            }
+            uint64_t off = 0;
+            for (uint64_t k = e; k-- > c;)
+            {
+                topology const& t = topo(k);
+                op->adj_step(l, states[k - c], t.keys, t.links, t.metas, adjoint_consts(dts[k]), lo, vpart + 3 * off,
+                             static_cast<uint32_t>(t.n), stream);
+                off += t.n;
+                std::swap(l, lo);
+                if (auto const* r = regrid_before(k))
+                {
+                    op->adj_regrid(l, r->mark, r->head, r->counts, r->pos, lo, static_cast<uint32_t>(r->n_old), stream);
+                    std::swap(l, lo);
+                }
+            }
+            hipLaunchKernelGGL(k_sum3_chunks, dim3(kReduceBlocks), dim3(256), 0, stream, vpart, off, chunk);
+            check_launch("k_sum3_chunks");
+            hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(256), 0, stream, chunk, acc);
+            check_launch("k_sum3_final");
+        }
+        AMR_HIP_CHECK(hipMemcpyAsync(grad_u0, l, field_bytes(topos[0].n), hipMemcpyDeviceToDevice, stream));
+        double h[3];
+        AMR_HIP_CHECK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, stream));
+        AMR_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int d = 0; d < 3; ++d) grad_v[d] = d < rank ? h[d] : 0.0;
+    }
+};
+
 extern "C"
 {
 
@@ -666,17 +987,6 @@ int amr_tree_get_neighbor_metadata(amr_ctx* ctx, amr_halo_direction_metadata* ho
             }
         }
     );
-}
-
-static void run_criterion(amr_ctx* ctx, amr_criterion const* crit)
-{
-    ctx->ensure_usable();
-    if (!crit) throw amr_error(AMR_ERR_ARGUMENT, "null criterion");
-    if (crit->kind != AMR_CRITERION_MAX_VALUE && crit->kind != AMR_CRITERION_GRADIENT)
-        throw amr_error(AMR_ERR_ARGUMENT, "unknown criterion kind");
-    ctx->op->criterion(crit->kind, ctx->u[ctx->cur], ctx->keys[ctx->kc], ctx->links, ctx->metas,
-                       crit->refine_threshold, crit->coarsen_threshold, crit->min_level, crit->max_level, ctx->flags,
-                       static_cast<uint32_t>(ctx->n), ctx->stream);
 }
 
 int amr_tree_compute_flags(amr_ctx* ctx, const amr_criterion* crit, int8_t* host_flags, uint64_t n)
@@ -1087,37 +1397,15 @@ int amr_batch_autotune(amr_batch* b, uint32_t trial_steps, int32_t* chosen, floa
         [&]
         {
             if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            hipEvent_t e0, e1;
-            AMR_HIP_CHECK(hipEventCreate(&e0));
-            AMR_HIP_CHECK(hipEventCreate(&e1));
-            int   best   = 0;
-            float best_t = std::numeric_limits<float>::max();
-            for (int v = 0; v < b->op->variant_count(); ++v)
-            {
-                auto trial = [&]
+            b->variant = pick_fastest(
+                b->op->variant_count(),
+                [&](int v)
                 {
                     b->op->step(v, b->u[b->cur], b->u[1 - b->cur], b->keys, b->links, b->metas, b->host_table[0],
                                 b->table, b->inst, static_cast<uint32_t>(b->total), b->stream);
-                };
-                trial();
-                AMR_HIP_CHECK(hipEventRecord(e0, b->stream));
-                for (uint32_t t = 0; t < trial_steps; ++t) trial();
-                AMR_HIP_CHECK(hipEventRecord(e1, b->stream));
-                AMR_HIP_CHECK(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                AMR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                const float us = 1000.0f * ms / static_cast<float>(std::max(trial_steps, 1u));
-                if (us < best_t)
-                {
-                    best_t = us;
-                    best   = v;
-                }
-            }
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            b->variant = best;
-            if (chosen) *chosen = best;
-            if (best_us) *best_us = best_t;
+                },
+                b->stream, trial_steps, best_us);
+            if (chosen) *chosen = b->variant;
         }
     );
 }
@@ -1129,6 +1417,132 @@ int amr_ctx_synchronize(amr_ctx* ctx)
         {
             if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             ctx->sync();
+        }
+    );
+}
+
+int amr_tape_record(amr_ctx* ctx, const amr_criterion* criterion, uint64_t steps, uint64_t regrid_every,
+                    uint64_t checkpoint_every, amr_tape** out)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !out) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *out = nullptr;
+            ctx->ensure_usable();
+            if (steps == 0) throw amr_error(AMR_ERR_ARGUMENT, "at least one step is required");
+            if (checkpoint_every == 0) throw amr_error(AMR_ERR_ARGUMENT, "checkpoint_every must be positive");
+            if (ctx->n == 0) throw amr_error(AMR_ERR_ARGUMENT, "empty tree");
+            auto t        = std::make_unique<amr_tape>();
+            t->op         = ctx->op;
+            t->cfg        = ctx->cfg;
+            t->rank       = ctx->rank;
+            t->depth      = ctx->depth;
+            t->ni         = ctx->ni;
+            t->ndir       = ctx->ndir;
+            t->variant    = ctx->variant;
+            t->ckpt_every = std::min(checkpoint_every, steps);
+            AMR_HIP_CHECK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+            t->record(ctx, criterion, steps, regrid_every);
+            *out = t.release();
+        }
+    );
+}
+
+int amr_tape_destroy(amr_tape* tape)
+{
+    return guarded([&] { delete tape; });
+}
+
+int amr_tape_get_info(const amr_tape* tape, amr_tape_info* info)
+{
+    return guarded(
+        [&]
+        {
+            if (!tape || !info) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            info->steps            = tape->steps;
+            info->checkpoint_every = tape->ckpt_every;
+            info->checkpoints      = tape->ckpts.size();
+            info->regrids          = tape->regrids.size();
+            info->topologies       = tape->topos.size();
+            info->initial_leaves   = tape->topos.front().n;
+            info->final_leaves     = tape->topo(tape->steps - 1).n;
+            info->max_leaves       = tape->max_n;
+            info->cells_per_leaf   = static_cast<uint64_t>(tape->ni);
+        }
+    );
+}
+
+int amr_tape_get_dts(const amr_tape* tape, double* dts, uint64_t count)
+{
+    return guarded(
+        [&]
+        {
+            if (!tape || !dts) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (count < tape->steps) throw amr_error(AMR_ERR_ARGUMENT, "buffer smaller than the step count");
+            std::copy(tape->dts.begin(), tape->dts.end(), dts);
+        }
+    );
+}
+
+int amr_tape_get_regrid(const amr_tape* tape, uint64_t index, uint64_t* step, uint64_t* leaves_before, int* changed,
+                        int8_t* flags, uint64_t flags_count)
+{
+    return guarded(
+        [&]
+        {
+            if (!tape) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (index >= tape->regrids.size()) throw amr_error(AMR_ERR_ARGUMENT, "regrid index out of range");
+            auto const& r = tape->regrids[index];
+            if (step) *step = r.step;
+            if (leaves_before) *leaves_before = r.n_old;
+            if (changed) *changed = r.changed ? 1 : 0;
+            if (flags)
+            {
+                if (flags_count < r.n_old) throw amr_error(AMR_ERR_ARGUMENT, "flag buffer smaller than the leaf count");
+                std::copy(r.flags.begin(), r.flags.end(), flags);
+            }
+        }
+    );
+}
+
+int amr_tape_get_keys(const amr_tape* tape, uint64_t index, uint64_t* keys, uint64_t count, uint64_t* leaves)
+{
+    return guarded(
+        [&]
+        {
+            if (!tape) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (index >= tape->topos.size()) throw amr_error(AMR_ERR_ARGUMENT, "topology index out of range");
+            auto const& t = tape->topos[index];
+            if (leaves) *leaves = t.n;
+            if (keys)
+            {
+                if (count < t.n) throw amr_error(AMR_ERR_ARGUMENT, "key buffer smaller than the leaf count");
+                AMR_HIP_CHECK(hipMemcpy(keys, t.keys, t.n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+            }
+        }
+    );
+}
+
+int amr_tape_forward(amr_tape* tape, const double* u0_device, const double velocity[3], double* uT_device)
+{
+    return guarded(
+        [&]
+        {
+            if (!tape || !u0_device || !velocity || !uT_device) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            tape->forward(u0_device, velocity, uT_device);
+        }
+    );
+}
+
+int amr_tape_adjoint(amr_tape* tape, const double* lambda_T_device, double* grad_u0_device, double grad_velocity[3])
+{
+    return guarded(
+        [&]
+        {
+            if (!tape || !lambda_T_device || !grad_u0_device || !grad_velocity)
+                throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            tape->adjoint(lambda_T_device, grad_u0_device, grad_velocity);
         }
     );
 }

@@ -1,0 +1,387 @@
+// kernels_adjoint.hpp -- discrete adjoint of the advection step, the halo
+// fill and the regrid transfer (their exact transposes), in gather form so
+// that every sum has a fixed order and no atomics are needed.
+//
+// For fixed topology, velocity v and dt, one step is linear:
+//   u' = u - sum_d dt/dx_d [ F(u, u_R) - F(u_L, u) ],
+//   F(a, b) = 0.5 (a v + b v) - 0.5 |v| (b - a) = vp a + vm b,
+//   vp = 0.5 (v + |v|), vm = 0.5 (v - |v|),
+// so  du'/du = 1 - sum_d dt/dx_d |v_d|,  du'/du_L = dt/dx vp,
+//     du'/du_R = -dt/dx vm,
+// and du'/dv_d = -dt/dx_d [ G(u, u_R) - G(u_L, u) ],
+//     G(a, b) = 0.5 (a + b) - 0.5 sgn(v_d) (b - a)
+// (dt held fixed).  Ghosts are linear in the neighbours' interiors:
+// same = copy, coarser = injection, finer = mean of 2^R fine cells.  Their
+// transposes: a cell copied into a same-level neighbour's ghost receives that
+// ghost's adjoint; a cell injected into finer neighbours' ghosts receives the
+// sum of their adjoints; a cell averaged into a coarser neighbour's ghost
+// receives 1/2^R of it.  The adjoint of a ghost is the coefficient with
+// which the neighbour's boundary cell reads it, times that cell's lambda'.
+//
+// One adjoint step is one launch over the leaves of the step's topology
+// (k_adj_step): lambda = A^T lambda' with the transposed halo gathered from
+// the neighbours' boundary layers of lambda', plus the per-leaf partial sums
+// of lambda' . du'/dv.  A regrid transfer is transposed by k_adj_regrid.
+#pragma once
+
+#include "device_common.hpp"
+#include "kernels_step.hpp"
+
+#pragma clang fp contract(off)
+
+namespace amrhip
+{
+
+// Per-launch adjoint constants: dt/dx per level and dimension, vp, vm and
+// sgn(v) per dimension.
+struct adjoint_params
+{
+    double dtdx[kMaxLevels][3];
+    double vp[3];
+    double vm[3];
+    double sgn[3];
+};
+
+// In-patch part of (A^T lambda')_c.
+template <int R, int N>
+__host__ __device__ inline double adj_interior(const double* __restrict__ lam, int i, const double* dtdx,
+                                               const double* vp, const double* vm)
+{
+    using S = patch_shape<R, N>;
+    int c[R];
+    S::coords(i, c);
+    double diag = 1.0;
+    for (int d = 0; d < R; ++d) diag -= dtdx[d] * (vp[d] - vm[d]);
+    double acc = diag * lam[i];
+    for (int d = 0; d < R; ++d)
+    {
+        const int j = R - 1 - d; // layout dim of physical axis d
+        int       s = 1;
+        for (int k = R - 1; k > j; --k) s *= N;
+        if (c[j] + 1 < N) acc += (dtdx[d] * vp[d]) * lam[i + s];  // c is u_L of c + e_d
+        if (c[j] - 1 >= 0) acc -= (dtdx[d] * vm[d]) * lam[i - s]; // c is u_R of c - e_d
+    }
+    return acc;
+}
+
+// Transposed halo term for the leaf's boundary cells facing direction dir at
+// tangential layout coordinates c (c[dim] ignored), from the neighbours'
+// lambda'.  It is added to the depth-0 cell, and for a coarser neighbour also
+// to the depth-1 cell (both are averaged into the same coarse ghost).
+//   same    : the neighbour reads our cell as its ghost on side dir^1;
+//   finer   : the 2^(R-1) fine ghost cells injected from our cell;
+//   coarser : the coarse ghost cell that averages our cell, / 2^R.
+// The neighbour's boundary cell reads its ghost on side dir^1 with
+// coefficient dt/dx vp (ghost is its u_L, dir positive) or -dt/dx vm (ghost
+// is its u_R, dir negative), at the neighbour's own level.
+template <int R, int N>
+__host__ __device__ inline double adj_face_term(const double* __restrict__ lam, const face_link& L, uint32_t meta,
+                                                int dir, const int* c, int level, const adjoint_params& prm)
+{
+    using S        = patch_shape<R, N>;
+    const int  dim = dir >> 1;
+    const bool pos = (dir & 1) != 0;
+    const int  d   = R - 1 - dim;
+    const int  nb  = pos ? 0 : N - 1; // the neighbour's layer that touches us
+    switch (rel_of(meta))
+    {
+        case REL_SAME:
+        {
+            int cc[R];
+            for (int k = 0; k < R; ++k) cc[k] = (k == dim) ? nb : c[k];
+            const double coef = pos ? (prm.dtdx[level][d] * prm.vp[d]) : -(prm.dtdx[level][d] * prm.vm[d]);
+            return coef * lam[static_cast<size_t>(L.ids[0]) * S::NI + S::lin(cc)];
+        }
+        case REL_FINER:
+        {
+            int kid = 0, bit = 0, base[R];
+            for (int k = 0; k < R; ++k)
+            {
+                if (k == dim)
+                {
+                    base[k] = nb;
+                    continue;
+                }
+                kid |= (c[k] / S::HALF) << bit;
+                ++bit;
+                base[k] = (c[k] % S::HALF) * 2;
+            }
+            const double* q   = lam + static_cast<size_t>(L.ids[kid]) * S::NI;
+            double        sum = 0.0;
+            for (int o = 0; o < S::K; ++o)
+            {
+                int cc[R], b = 0;
+                for (int k = 0; k < R; ++k)
+                {
+                    if (k == dim)
+                    {
+                        cc[k] = base[k];
+                        continue;
+                    }
+                    cc[k] = base[k] + ((o >> (R - 2 - b)) & 1);
+                    ++b;
+                }
+                sum += q[S::lin(cc)];
+            }
+            const double coef =
+                pos ? (prm.dtdx[level + 1][d] * prm.vp[d]) : -(prm.dtdx[level + 1][d] * prm.vm[d]);
+            return coef * sum;
+        }
+        case REL_COARSER:
+        {
+            int cc[R];
+            for (int k = 0; k < R; ++k)
+                cc[k] = (k == dim) ? nb : static_cast<int>(quad_of(meta, k)) * S::HALF + c[k] / 2;
+            const double coef =
+                pos ? (prm.dtdx[level - 1][d] * prm.vp[d]) : -(prm.dtdx[level - 1][d] * prm.vm[d]);
+            return coef * lam[static_cast<size_t>(L.ids[0]) * S::NI + S::lin(cc)] / static_cast<double>(S::F);
+        }
+        default: return 0.0;
+    }
+}
+
+// Contribution of cell i to lambda' . du'/dv, from the staged u tile (with
+// the ghost layer).
+template <int R, int N>
+__host__ __device__ inline void adj_velocity_cell(const double* tile, int i, double lam_c, const double* dtdx,
+                                                  const double* sgn, double* acc)
+{
+    using T = tile_shape<R, N>;
+    int c[R];
+    patch_shape<R, N>::coords(i, c);
+    const int    t  = T::idx(c);
+    const double uc = tile[t];
+    for (int d = 0; d < R; ++d)
+    {
+        const int    s  = T::stride(R - 1 - d);
+        const double ul = tile[t - s], ur = tile[t + s];
+        const double gR = 0.5 * (uc + ur) - 0.5 * sgn[d] * (ur - uc);
+        const double gL = 0.5 * (ul + uc) - 0.5 * sgn[d] * (uc - ul);
+        acc[d] += lam_c * (-dtdx[d] * (gR - gL));
+    }
+}
+
+// Face index of tangential coordinates c (dims other than dim), row-major.
+template <int R, int N>
+__host__ __device__ inline int face_index(const int* c, int dim)
+{
+    int f = 0;
+    for (int k = 0; k < R; ++k)
+        if (k != dim) f = f * N + c[k];
+    return f;
+}
+
+// Tangential coordinates of face index f of direction dir (c[dim] = 0).
+template <int R, int N>
+__host__ __device__ inline void face_coords(int f, int dir, int* c)
+{
+    const int dim = dir >> 1;
+    for (int k = R - 1; k >= 0; --k)
+    {
+        if (k == dim)
+        {
+            c[k] = 0;
+            continue;
+        }
+        c[k] = f % N;
+        f /= N;
+    }
+}
+
+// lambda_c = (A^T lambda')_c + transposed halo terms h[dir][face] of the
+// directions whose boundary band holds c.
+template <int R, int N>
+__host__ __device__ inline double adj_cell(const double* __restrict__ lam, const double* __restrict__ h,
+                                           const uint32_t* __restrict__ M, int i, const double* dtdx,
+                                           const double* vp, const double* vm)
+{
+    using S    = patch_shape<R, N>;
+    double acc = adj_interior<R, N>(lam, i, dtdx, vp, vm);
+    int    c[R];
+    S::coords(i, c);
+    for (int dir = 0; dir < S::NDIR; ++dir)
+    {
+        const int dim   = dir >> 1;
+        const int depth = (dir & 1) ? (N - 1 - c[dim]) : c[dim];
+        if (depth == 0 || (depth == 1 && rel_of(M[dir]) == REL_COARSER))
+            acc += h[dir * S::FACE + face_index<R, N>(c, dim)];
+    }
+    return acc;
+}
+
+// Transposed regrid transfer, value of cell t of old leaf i.
+// Refined leaf: sum of lambda over the 2^R fine cells injected from t.
+template <int R, int N>
+__host__ __device__ inline double adj_refine_value(const double* __restrict__ lam_new, uint32_t first_child, int t)
+{
+    using S = patch_shape<R, N>;
+    int tc[R], base[R], kid = 0;
+    S::coords(t, tc);
+    for (int d = 0; d < R; ++d)
+    {
+        kid |= (tc[d] / S::HALF) << (R - 1 - d);
+        base[d] = (tc[d] % S::HALF) * 2;
+    }
+    const double* q   = lam_new + static_cast<size_t>(first_child + kid) * S::NI;
+    double        sum = 0.0;
+    for (int off = 0; off < S::F; ++off)
+    {
+        int cc[R];
+        for (int d = 0; d < R; ++d) cc[d] = base[d] + ((off >> (R - 1 - d)) & 1);
+        sum += q[S::lin(cc)];
+    }
+    return sum;
+}
+
+// Coarsened family: cell f of child kid receives lambda of the parent cell
+// that averaged it, divided by 2^R.
+template <int R, int N>
+__host__ __device__ inline double adj_restrict_value(const double* __restrict__ lam_parent, int kid, int f)
+{
+    using S = patch_shape<R, N>;
+    int fc[R], t[R];
+    S::coords(f, fc);
+    for (int d = 0; d < R; ++d) t[d] = ((kid >> (R - 1 - d)) & 1) * S::HALF + fc[d] / 2;
+    return lam_parent[S::lin(t)] / static_cast<double>(S::F);
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// One adjoint step, one workgroup per leaf: lam = transpose(step)(lam_next)
+// and vpart[leaf][d] = sum over the leaf's cells of lam_next . du'/dv_d.
+// u is the state the forward step started from.
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_adj_step(const double* __restrict__ lam_next, const double* __restrict__ u,
+                                                 const uint64_t* __restrict__ keys,
+                                                 const face_link* __restrict__ links,
+                                                 const uint32_t* __restrict__ metas, adjoint_params prm,
+                                                 double* __restrict__ lam, double* __restrict__ vpart,
+                                                 uint32_t nleaves)
+{
+    using S = patch_shape<R, N>;
+    __shared__ double tile[tile_shape<R, N>::TS];
+    __shared__ double lt[S::NI];
+    __shared__ double h[S::NDIR * S::FACE];
+    __shared__ double red[R][NT];
+    const uint32_t leaf = blockIdx.x;
+    if (leaf >= nleaves) return;
+    const face_link* L  = links + static_cast<size_t>(leaf) * S::NDIR;
+    const uint32_t*  M  = metas + static_cast<size_t>(leaf) * S::NDIR;
+    const int        lv = key_level(keys[leaf]);
+    stage_tile<R, N, NT>(tile, u, leaf, links, metas);
+    const double* lp = lam_next + static_cast<size_t>(leaf) * S::NI;
+    for (int i = threadIdx.x; i < S::NI; i += NT) lt[i] = lp[i];
+    for (int g = threadIdx.x; g < S::NDIR * S::FACE; g += NT)
+    {
+        const int dir = g / S::FACE;
+        int       c[R];
+        face_coords<R, N>(g % S::FACE, dir, c);
+        h[g] = adj_face_term<R, N>(lam_next, L[dir], M[dir], dir, c, lv, prm);
+    }
+    __syncthreads();
+    double dtdx[R], vp[R], vm[R], sgn[R], acc[R];
+    for (int d = 0; d < R; ++d)
+    {
+        dtdx[d] = prm.dtdx[lv][d];
+        vp[d]   = prm.vp[d];
+        vm[d]   = prm.vm[d];
+        sgn[d]  = prm.sgn[d];
+        acc[d]  = 0.0;
+    }
+    double* out = lam + static_cast<size_t>(leaf) * S::NI;
+    for (int i = threadIdx.x; i < S::NI; i += NT)
+    {
+        out[i] = adj_cell<R, N>(lt, h, M, i, dtdx, vp, vm);
+        adj_velocity_cell<R, N>(tile, i, lt[i], dtdx, sgn, acc);
+    }
+    // fixed-order block reduction of the velocity partials
+    for (int d = 0; d < R; ++d) red[d][threadIdx.x] = acc[d];
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+            for (int d = 0; d < R; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) vpart[static_cast<size_t>(leaf) * 3 + d] = d < R ? red[d][0] : 0.0;
+}
+
+// Transposed regrid transfer, one workgroup per leaf before the regrid
+// (mark / head / counts / pos recorded by the forward regrid).
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_adj_regrid(const double* __restrict__ lam_new, const uint8_t* __restrict__ mark,
+                                                   const uint8_t* __restrict__ head,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ pos, double* __restrict__ lam_old,
+                                                   uint32_t n_old)
+{
+    using S          = patch_shape<R, N>;
+    const uint32_t i = blockIdx.x;
+    if (i >= n_old || counts[i] == 0) return;
+    const uint32_t dst = pos[i];
+    double*        out = lam_old + static_cast<size_t>(i) * S::NI;
+    if (mark[i])
+    {
+        for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = adj_refine_value<R, N>(lam_new, dst, t);
+    }
+    else if (head[i])
+    {
+        const double* par = lam_new + static_cast<size_t>(dst) * S::NI;
+        for (int kid = 0; kid < S::F; ++kid)
+            for (int f = threadIdx.x; f < S::NI; f += NT)
+                lam_old[static_cast<size_t>(i + kid) * S::NI + f] = adj_restrict_value<R, N>(par, kid, f);
+    }
+    else
+    {
+        const double* src = lam_new + static_cast<size_t>(dst) * S::NI;
+        for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = src[t];
+    }
+}
+
+// Deterministic sum of `count` triples: stage 1 gives each of kReduceBlocks
+// workgroups a fixed contiguous chunk, stage 2 adds the chunk sums in order
+// into acc.
+constexpr int kReduceBlocks = 64;
+
+__device__ inline void block_sum3(double* s, double (*red)[256])
+{
+    for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = s[d];
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1)
+    {
+        if (threadIdx.x < k)
+            for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + k];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum3_chunks(const double* __restrict__ v, uint64_t count,
+                                                     double* __restrict__ chunk_sums)
+{
+    __shared__ double red[3][256];
+    const uint64_t    per = (count + kReduceBlocks - 1) / kReduceBlocks;
+    const uint64_t    b0  = blockIdx.x * per;
+    const uint64_t    b1  = b0 + per < count ? b0 + per : count;
+    double            s[3] = { 0.0, 0.0, 0.0 };
+    for (uint64_t l = b0 + threadIdx.x; l < b1; l += 256)
+        for (int d = 0; d < 3; ++d) s[d] += v[l * 3 + d];
+    block_sum3(s, red);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) chunk_sums[blockIdx.x * 3 + d] = red[d][0];
+}
+
+__global__ __launch_bounds__(256) void k_sum3_final(const double* __restrict__ chunk_sums, double* __restrict__ acc)
+{
+    __shared__ double red[3][256];
+    double            s[3] = { 0.0, 0.0, 0.0 };
+    if (threadIdx.x < kReduceBlocks)
+        for (int d = 0; d < 3; ++d) s[d] = chunk_sums[threadIdx.x * 3 + d];
+    block_sum3(s, red);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) acc[d] += red[d][0];
+}
+
+} // namespace amrhip

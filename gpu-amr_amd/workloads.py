@@ -3,6 +3,13 @@ public API (used by bench.py and the tests).
 
 C3: 3D uniform octree of depth 5, 8^3 FP64 patches, seeded Gaussian bump,
     v = (1, 0.5, 0.25), CFL 0.3.
+C2: 2D active AMR: uniform depth 3, seeded (2026) 8-bump mixture, v = (1, 0.5),
+    CFL 0.4, gradient criterion (refine > 0.05, coarsen < 0.01) every 10
+    steps, levels 3..7.
+C5: C2 with levels 3..6 and 300 steps; the tree is first adapted to the
+    initial field (criterion applied until it no longer changes the tree,
+    the field re-sampled after each regrid), then the solve is recorded with
+    a checkpoint every 20 steps.
 C4: 64 independent 2D instances; instance i (seed 1000 + i) starts uniform at
     depth 3 and is refined four times with each leaf flagged Refine with
     probability 0.35 (levels capped at 7, 2:1 enforced by the regrid), carries
@@ -89,3 +96,38 @@ def c4_instance(amr, i, max_depth=7, rounds=4, p_refine=0.35, start_level=3, cap
     solver = amr.AmrSolver(tree, vel, 0.4)
     solver.initialize(ic)
     return tree, solver, history, vel, ic
+
+
+def c2_mixture(seed=2026):
+    """The C2 initial field: 8 Gaussian bumps drawn in the order amplitudes,
+    sigmas, centres."""
+    rng = np.random.default_rng(seed)
+    amp, sig, cen = rng.uniform(0.5, 1.0, 8), rng.uniform(0.03, 0.1, 8), rng.uniform(0.0, 1.0, (8, 2))
+
+    def ic(c):
+        u = np.zeros(c.shape[:-1])
+        for a, s, p in zip(amp, sig, cen):
+            u += a * np.exp(-((c[..., 0] - p[0]) ** 2 + (c[..., 1] - p[1]) ** 2) / (2 * s * s))
+        return u
+
+    return ic
+
+
+def c5_setup(amr, max_level=6, capacity=4096 + 64, max_adapt=8):
+    """Tree, solver, criterion and the flag history of the initial adaptation
+    (so that the oracle can rebuild the same tree)."""
+    tree = amr.NdTree(2, 7, 16, 1, capacity)
+    for _ in range(3):
+        tree.refine_all()
+    solver = amr.AmrSolver(tree, (1.0, 0.5), 0.4)
+    ic = c2_mixture()
+    solver.initialize(ic)
+    crit = amr.make_criterion(amr.CRITERION_GRADIENT, 0.05, 0.01, 3, max_level)
+    history = []
+    for _ in range(max_adapt):
+        flags = tree.compute_flags(crit)
+        if not tree.reconstruct_tree(flags):
+            break
+        history.append(flags)
+        solver.initialize(ic)
+    return tree, solver, crit, history, ic

@@ -207,6 +207,59 @@ int amr_batch_set_step_variant(amr_batch* batch, int32_t variant);
 /* As amr_solver_autotune, over the whole batch. */
 int amr_batch_autotune(amr_batch* batch, uint32_t trial_steps, int32_t* chosen, float* best_us);
 
+/* ---- discrete adjoint (no counterpart in the reference, which has no
+ * differentiation; the forward pass it differentiates is the solve loop of
+ * include/solver/amr_solver.hpp:149-262 with reconstruct_tree calls,
+ * include/ndtree/ndtree.hpp:1249-1271, between steps) -------------------- */
+
+typedef struct amr_tape amr_tape;
+
+typedef struct amr_tape_info
+{
+    uint64_t steps;            /* recorded time steps                         */
+    uint64_t checkpoint_every; /* state kept before every k-th step           */
+    uint64_t checkpoints;
+    uint64_t regrids;          /* regrid calls (changed or not)               */
+    uint64_t topologies;       /* distinct trees the solve went through       */
+    uint64_t initial_leaves;
+    uint64_t final_leaves;
+    uint64_t max_leaves;
+    uint64_t cells_per_leaf;
+} amr_tape_info;
+
+/* Advances the context `steps` steps (dt from the cfl condition of the
+ * current finest level, as amr_solver_advance), calling the regrid of
+ * amr_tree_reconstruct_criterion with `criterion` before every step k > 0
+ * with k % regrid_every == 0 (criterion NULL or regrid_every 0: no regrid),
+ * and records what the adjoint needs: each topology, the dt of every step,
+ * every regrid's flags and leaf maps, and the state before every
+ * checkpoint_every-th step.  The context ends at the final state. */
+int amr_tape_record(amr_ctx* ctx, const amr_criterion* criterion, uint64_t steps, uint64_t regrid_every,
+                    uint64_t checkpoint_every, amr_tape** out);
+int amr_tape_destroy(amr_tape* tape);
+int amr_tape_get_info(const amr_tape* tape, amr_tape_info* info);
+/* The dt of every recorded step (count >= steps). */
+int amr_tape_get_dts(const amr_tape* tape, double* dts, uint64_t count);
+/* Regrid `index` (0-based, in step order): the step it preceded, the leaf
+ * count before it, whether it changed the tree, and its refine flags
+ * (flags_count >= *leaves_before; flags may be NULL). */
+int amr_tape_get_regrid(const amr_tape* tape, uint64_t index, uint64_t* step, uint64_t* leaves_before, int* changed,
+                        int8_t* flags, uint64_t flags_count);
+/* Keys of topology `index` (0 = initial tree); *leaves gets its leaf count. */
+int amr_tape_get_keys(const amr_tape* tape, uint64_t index, uint64_t* keys, uint64_t count, uint64_t* leaves);
+/* u(T) from initial interiors u0 (initial_leaves * cells_per_leaf doubles)
+ * and velocity v, over the recorded topologies and dts (dt does not follow
+ * v).  Device pointers; the call waits for all prior device work before
+ * reading and for its own work before returning.  Refreshes the checkpoints:
+ * the next amr_tape_adjoint differentiates this pass. */
+int amr_tape_forward(amr_tape* tape, const double* u0_device, const double velocity[3], double* uT_device);
+/* grad_u0 = (du(T)/du0)^T lambda_T and grad_velocity = (du(T)/dv)^T lambda_T
+ * of the most recent forward pass (the recording or amr_tape_forward), with
+ * the states between checkpoints recomputed.  lambda_T and grad_u0 are
+ * device pointers (final_leaves and initial_leaves patches); grad_velocity
+ * is a host array of 3 (unused dimensions 0). */
+int amr_tape_adjoint(amr_tape* tape, const double* lambda_T_device, double* grad_u0_device, double grad_velocity[3]);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -4,6 +4,7 @@
 #include "device_common.hpp"
 #include "kernels_step.hpp"
 #include "kernels_tree.hpp"
+#include "kernels_adjoint.hpp"
 
 #include <cstring>
 #include <vector>
@@ -88,7 +89,7 @@ void flags(int kind, const uint64_t* keys, uint32_t n, const face_link* links, c
 template <int R, int N>
 int regrid(int depth, const uint64_t* keys, uint32_t n, const int8_t* fl, const face_link* links,
            const uint32_t* metas, const double* u, int max_level, uint64_t* keys_out, double* u_out, uint32_t cap,
-           uint32_t* summary)
+           uint32_t* summary, uint32_t* maps = nullptr)
 {
     using S = patch_shape<R, N>;
     std::vector<uint8_t>  mark(n), head(n);
@@ -116,6 +117,14 @@ int regrid(int depth, const uint64_t* keys, uint32_t n, const int8_t* fl, const 
     summary[1] = nr;
     summary[2] = nc;
     summary[3] = err;
+    if (maps) // rows of n: mark, head, counts, pos
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            maps[i]         = mark[i];
+            maps[n + i]     = head[i];
+            maps[2 * n + i] = counts[i];
+            maps[3 * n + i] = pos[i];
+        }
     if (err || (nr == 0 && nc == 0) || total > cap) return 0;
     for (uint32_t i = 0; i < n; ++i)
     {
@@ -135,6 +144,69 @@ int regrid(int depth, const uint64_t* keys, uint32_t n, const int8_t* fl, const 
         }
     }
     return 1;
+}
+
+// k_adj_step, serialised: lam = transpose(step)(lam_next), vpart per leaf.
+template <int R, int N>
+void adj_step(const uint64_t* keys, uint32_t n, const face_link* links, const uint32_t* metas, const double* lam_next,
+              const double* u, const adjoint_params& prm, double* lam, double* vpart)
+{
+    using S = patch_shape<R, N>;
+    std::vector<double> tile(tile_shape<R, N>::TS, 0.0), h(S::NDIR * S::FACE);
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+    {
+        for (int i = 0; i < S::NI; ++i) stage_interior<R, N>(tile.data(), u + static_cast<size_t>(leaf) * S::NI, i);
+        for (int g = 0; g < S::NDIR * S::FACE; ++g) stage_ghost<R, N>(tile.data(), u, leaf, links, metas, g);
+        const face_link* L  = links + static_cast<size_t>(leaf) * S::NDIR;
+        const uint32_t*  M  = metas + static_cast<size_t>(leaf) * S::NDIR;
+        const int        lv = key_level(keys[leaf]);
+        for (int g = 0; g < S::NDIR * S::FACE; ++g)
+        {
+            const int dir = g / S::FACE;
+            int       c[R];
+            face_coords<R, N>(g % S::FACE, dir, c);
+            h[g] = adj_face_term<R, N>(lam_next, L[dir], M[dir], dir, c, lv, prm);
+        }
+        double dtdx[R], vp[R], vm[R], sgn[R], acc[R];
+        for (int d = 0; d < R; ++d)
+        {
+            dtdx[d] = prm.dtdx[lv][d];
+            vp[d]   = prm.vp[d];
+            vm[d]   = prm.vm[d];
+            sgn[d]  = prm.sgn[d];
+            acc[d]  = 0.0;
+        }
+        const double* lt = lam_next + static_cast<size_t>(leaf) * S::NI;
+        for (int i = 0; i < S::NI; ++i)
+        {
+            lam[static_cast<size_t>(leaf) * S::NI + i] = adj_cell<R, N>(lt, h.data(), M, i, dtdx, vp, vm);
+            adj_velocity_cell<R, N>(tile.data(), i, lt[i], dtdx, sgn, acc);
+        }
+        for (int d = 0; d < 3; ++d) vpart[static_cast<size_t>(leaf) * 3 + d] = d < R ? acc[d] : 0.0;
+    }
+}
+
+// k_adj_regrid, serialised.
+template <int R, int N>
+void adj_regrid(const double* lam_new, const uint8_t* mark, const uint8_t* head, const uint32_t* counts,
+                const uint32_t* pos, double* lam_old, uint32_t n_old)
+{
+    using S = patch_shape<R, N>;
+    for (uint32_t i = 0; i < n_old; ++i)
+    {
+        if (counts[i] == 0) continue;
+        for (int t = 0; t < S::NI; ++t)
+        {
+            if (mark[i])
+                lam_old[static_cast<size_t>(i) * S::NI + t] = adj_refine_value<R, N>(lam_new, pos[i], t);
+            else if (head[i])
+                for (int kid = 0; kid < S::F; ++kid)
+                    lam_old[static_cast<size_t>(i + kid) * S::NI + t] =
+                        adj_restrict_value<R, N>(lam_new + static_cast<size_t>(pos[i]) * S::NI, kid, t);
+            else
+                lam_old[static_cast<size_t>(i) * S::NI + t] = lam_new[static_cast<size_t>(pos[i]) * S::NI + t];
+        }
+    }
 }
 
 } // namespace
@@ -197,12 +269,53 @@ int emu_flags(int rank, int nn, int h, int kind, const uint64_t* keys, uint32_t 
 
 int emu_regrid(int rank, int nn, int depth, const uint64_t* keys, uint32_t n, const int8_t* fl, const void* links,
                const uint32_t* metas, const double* u, int max_level, uint64_t* keys_out, double* u_out, uint32_t cap,
-               uint32_t* summary)
+               uint32_t* summary, uint32_t* maps)
 {
     auto const* L = static_cast<const face_link*>(links);
-#define X(R_, N_, H_)                                                                                 \
-    if (rank == R_ && nn == N_)                                                                       \
-        return regrid<R_, N_>(depth, keys, n, fl, L, metas, u, max_level, keys_out, u_out, cap, summary);
+#define X(R_, N_, H_)                                                                                       \
+    if (rank == R_ && nn == N_)                                                                             \
+        return regrid<R_, N_>(depth, keys, n, fl, L, metas, u, max_level, keys_out, u_out, cap, summary, maps);
+    EMU_SHAPES(X)
+#undef X
+    return -1;
+}
+
+// dtdx: [17][3]; vel: 3 values (vp, vm and sgn are derived as on the device).
+int emu_adj_step(int rank, int nn, const uint64_t* keys, uint32_t n, const void* links, const uint32_t* metas,
+                 const double* lam_next, const double* u, const double* dtdx, const double* vel, double* lam,
+                 double* vpart)
+{
+    auto const*    L = static_cast<const face_link*>(links);
+    adjoint_params prm{};
+    std::memcpy(prm.dtdx, dtdx, sizeof(prm.dtdx));
+    for (int d = 0; d < 3; ++d)
+    {
+        prm.vp[d]  = 0.5 * (vel[d] + std::abs(vel[d]));
+        prm.vm[d]  = 0.5 * (vel[d] - std::abs(vel[d]));
+        prm.sgn[d] = vel[d] > 0.0 ? 1.0 : (vel[d] < 0.0 ? -1.0 : 0.0);
+    }
+#define X(R_, N_, H_)                                                              \
+    if (rank == R_ && nn == N_)                                                    \
+    {                                                                              \
+        adj_step<R_, N_>(keys, n, L, metas, lam_next, u, prm, lam, vpart);         \
+        return 0;                                                                  \
+    }
+    EMU_SHAPES(X)
+#undef X
+    return -1;
+}
+
+int emu_adj_regrid(int rank, int nn, const double* lam_new, const uint32_t* maps, uint32_t n_old, double* lam_old)
+{
+    std::vector<uint8_t> mark(maps, maps + n_old), head(maps + n_old, maps + 2 * n_old);
+    const uint32_t*      counts = maps + 2 * n_old;
+    const uint32_t*      pos    = maps + 3 * n_old;
+#define X(R_, N_, H_)                                                          \
+    if (rank == R_ && nn == N_)                                                \
+    {                                                                          \
+        adj_regrid<R_, N_>(lam_new, mark.data(), head.data(), counts, pos, lam_old, n_old); \
+        return 0;                                                              \
+    }
     EMU_SHAPES(X)
 #undef X
     return -1;

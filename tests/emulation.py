@@ -25,7 +25,8 @@ def load():
         if not LIB.exists() or LIB.stat().st_mtime < max(p.stat().st_mtime for p in (ROOT / "gpu-amr_amd" / "csrc").glob("*.hpp")):
             build()
         _lib = ctypes.CDLL(str(LIB))
-        for name in ("emu_build_table", "emu_step", "emu_padded", "emu_flags", "emu_regrid"):
+        for name in ("emu_build_table", "emu_step", "emu_padded", "emu_flags", "emu_regrid", "emu_adj_step",
+                     "emu_adj_regrid"):
             getattr(_lib, name).restype = ctypes.c_int
     return _lib
 
@@ -102,14 +103,42 @@ def flags(rank, n, h, kind, keys, links, metas, u, rthr, cthr, mn, mx):
     return out
 
 
-def regrid(rank, n, depth, keys, fl, links, metas, u, max_level, cap):
+def regrid(rank, n, depth, keys, fl, links, metas, u, max_level, cap, with_maps=False):
+    """The regrid kernels in sequence; with_maps also returns the (4, n_old)
+    uint32 rows mark, head, counts, pos of the old->new leaf map."""
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
     u = np.ascontiguousarray(u, dtype=np.float64)
     fl = np.ascontiguousarray(fl, dtype=np.int8)
     keys_out = np.zeros(cap, dtype=np.uint64)
     u_out = np.zeros(cap * n**rank)
     summary = np.zeros(4, dtype=np.uint32)
+    maps = np.zeros((4, keys.size), dtype=np.uint32)
     changed = load().emu_regrid(rank, n, depth, _p(keys), ctypes.c_uint32(keys.size), _p(fl), _p(links), _p(metas),
-                                _p(u), max_level, _p(keys_out), _p(u_out), ctypes.c_uint32(cap), _p(summary))
+                                _p(u), max_level, _p(keys_out), _p(u_out), ctypes.c_uint32(cap), _p(summary), _p(maps))
     total = int(summary[0])
-    return bool(changed), keys_out[:total], u_out[: total * n**rank], summary
+    out = (bool(changed), keys_out[:total], u_out[: total * n**rank], summary)
+    return out + (maps,) if with_maps else out
+
+
+def adj_step(rank, n, keys, links, metas, lam_next, u, dtdx, vel):
+    """Transposed step (lam) and per-leaf velocity partials (n_leaves, 3)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    lam_next = np.ascontiguousarray(lam_next, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.zeros(3)
+    v[: len(vel)] = vel
+    lam = np.zeros_like(lam_next)
+    vpart = np.zeros((keys.size, 3))
+    rc = load().emu_adj_step(rank, n, _p(keys), ctypes.c_uint32(keys.size), _p(links), _p(metas), _p(lam_next), _p(u),
+                             _p(np.ascontiguousarray(dtdx)), _p(v), _p(lam), _p(vpart))
+    assert rc == 0
+    return lam, vpart
+
+
+def adj_regrid(rank, n, lam_new, maps, n_old):
+    lam_new = np.ascontiguousarray(lam_new, dtype=np.float64)
+    maps = np.ascontiguousarray(maps, dtype=np.uint32)
+    lam_old = np.zeros(n_old * n**rank)
+    rc = load().emu_adj_regrid(rank, n, _p(lam_new), _p(maps), ctypes.c_uint32(n_old), _p(lam_old))
+    assert rc == 0
+    return lam_old
