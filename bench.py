@@ -10,8 +10,9 @@ all leaves with the state resident in HBM.
 Prints one JSON line with the throughput, the roofline of the step kernel
 (16 algorithmic bytes per cell update: one FP64 read of u^n, one FP64 write of
 u^(n+1)) measured with HIP events on the context stream, the CPU oracle timed
-on this host as a baseline, and the batched form (64 independent 2D
-instances, one graph launch) against 64 sequential plain solves.
+on this host as a baseline, the batched form (64 independent 2D instances,
+one graph launch) against 64 sequential plain solves, and the discrete
+adjoint of the 2D active-AMR solve against its forward pass.
 """
 from __future__ import annotations
 
@@ -102,19 +103,49 @@ def cpu_baseline(steps, threads):
     return steps * o.size() * 512 / dt, dt
 
 
-def copy_calibration(mib, reps):
-    """Plain device-to-device copies of a known size: the reference point
-    that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+def calib_lib():
+    lib = ctypes.CDLL(str(ROOT / "gpu-amr_amd" / "libamr_calib.so"))
+    lib.amr_calib_copy.restype = ctypes.c_int
+    lib.amr_calib_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    return lib
+
+
+def copy_kernel(n_doubles, reps, ev=None):
+    """A plain copy of n_doubles FP64 values (8 bytes read and 8 written per
+    value) with the step kernels' access pattern, `reps` times on its own
+    stream; returns the per-copy time in ms when `ev` (HipEvents) is given.
+    Its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the HBM counters."""
     hip = ctypes.CDLL("libamdhip64.so")
-    nbytes = mib << 20
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    lib = calib_lib()
+    nbytes = n_doubles * 8
+    a, b, st = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
     assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
     assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
-    for _ in range(reps):
-        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
     assert hip.hipDeviceSynchronize() == 0
-    print(json.dumps({"copy_calibration_bytes_per_dispatch": nbytes, "reps": reps}), flush=True)
+    assert lib.amr_calib_copy(a, b, n_doubles, st) == 0  # warm
+    ms = None
+    if ev is not None:
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, st.value)
+    for _ in range(reps):
+        assert lib.amr_calib_copy(a, b, n_doubles, st) == 0
+    if ev is not None:
+        ev.record(e1, st.value)
+        ms = ev.elapsed_ms(e0, e1) / reps
+    assert hip.hipStreamSynchronize(st) == 0
+    hip.hipStreamDestroy(st)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return ms
+
+
+def copy_calibration(mib, reps):
+    n = (mib << 20) // 8
+    copy_kernel(n, reps)
+    print(json.dumps({"copy_calibration_bytes_read_per_dispatch": n * 8, "bytes_written_per_dispatch": n * 8,
+                      "reps": reps}), flush=True)
 
 
 def batched_section(amr, instances, steps):
@@ -165,14 +196,60 @@ def batched_section(amr, instances, steps):
     }
 
 
+def adjoint_section(amr, steps=300, checkpoint_every=20, reps=3):
+    """C5: the recorded 2D active-AMR solve (levels 3..6, gradient criterion
+    every 10 steps); time of one full forward replay vs one full adjoint pass
+    (with the recompute between checkpoints), best of `reps`."""
+    import torch
+
+    from gpu_amr_amd import workloads
+    from gpu_amr_amd.adjoint import advection_solve
+
+    tree, solver, crit, _, _ = workloads.c5_setup(amr)
+    u0 = torch.from_numpy(tree.get_interior().reshape(-1).copy()).cuda()
+    t0 = time.perf_counter()
+    tape = amr.Tape(solver, steps, crit, regrid_every=10, checkpoint_every=checkpoint_every)
+    rec = time.perf_counter() - t0
+    info = tape.info()
+    vel = torch.tensor((1.0, 0.5), dtype=torch.float64)
+    fwd, adj = [], []
+    lam = torch.randn(info["final_leaves"] * info["cells_per_leaf"], dtype=torch.float64, device="cuda")
+    g = torch.empty_like(u0)
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            advection_solve(u0, vel, tape)
+        fwd.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        tape.adjoint(lam.data_ptr(), g.data_ptr())
+        adj.append(time.perf_counter() - t0)
+    f, a = min(fwd), min(adj)
+    log(f"adjoint: {info['steps']} steps, leaves {info['initial_leaves']}..{info['max_leaves']}, "
+        f"forward {f * 1e3:.1f} ms, adjoint {a * 1e3:.1f} ms, ratio {a / f:.2f}")
+    tape.close()
+    tree.close()
+    return {
+        "workload": "C5: 2D active AMR, 16^2 FP64 patches, levels 3..6, gradient criterion every 10 steps, 300 steps",
+        "steps": info["steps"],
+        "checkpoint_every": info["checkpoint_every"],
+        "leaves_initial": info["initial_leaves"],
+        "leaves_max": info["max_leaves"],
+        "record_ms": rec * 1e3,
+        "forward_ms": f * 1e3,
+        "adjoint_ms": a * 1e3,
+        "adjoint_over_forward": a / f,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-instances", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
@@ -210,6 +287,11 @@ def main():
     best, best_us = s.autotune(20)
     log(f"step variants (us/step): {per_variant}; autotuner -> {names[best]} ({best_us:.1f} us)")
 
+    # the read+write streaming rate of a plain copy of the same state size
+    copy_ms = copy_kernel(cells, 20, ev)
+    copy_gbs = 2 * 8 * cells / (copy_ms * 1e-3) / 1e9
+    log(f"plain copy of {8 * cells / 1e6:.0f} MB: {copy_ms * 1e3:.1f} us, {copy_gbs:.0f} GB/s")
+
     g.synchronize()
     t0 = time.perf_counter()
     ev.record(e0, stream)
@@ -227,6 +309,10 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_section(amr, args.batched_instances, 200)
+
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_section(amr)
 
     cpu = None
     if not args.no_cpu:
@@ -270,9 +356,12 @@ def main():
             "kernel": f"step kernel variant '{names[best]}' (autotuned)",
             "kernel_us": kernel_ms * 1e3,
             "variants_us": per_variant,
+            "copy_GBps": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs,
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

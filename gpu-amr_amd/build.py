@@ -45,16 +45,27 @@ def flags():
     ]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *flags(), "-o", str(tmp), *map(str, SOURCES)]
+CALIB_LIB = HERE / "libamr_calib.so"
+CALIB_SOURCES = [HERE / "csrc" / "calibration.hip"]
+
+
+def _compile(sources, out, verbose):
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [hipcc(), *flags(), "-o", str(tmp), *map(str, sources)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if verbose or r.returncode != 0:
         print(" ".join(cmd))
         print(r.stdout[-4000:], r.stderr[-8000:])
     if r.returncode != 0:
-        raise RuntimeError("libamr_hip.so build failed")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"{out.name} build failed")
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    """libamr_hip.so (the solver) and libamr_calib.so (the copy kernel that
+    calibrates the HBM byte counters)."""
+    _compile(SOURCES, LIB, verbose)
+    _compile(CALIB_SOURCES, CALIB_LIB, verbose)
     return str(LIB)
 
 

@@ -57,12 +57,22 @@ constexpr int block_threads(int cells)
     return t;
 }
 
+// The +x run lists of one leaf table (for the run-streaming step kernels),
+// runs of at most kRunK[k] leaves; an empty view makes the run variants fall
+// back to the wave-per-leaf kernel.
+constexpr uint32_t kRunK[2] = { 4, 8 };
+struct run_view
+{
+    const leaf_run* list[2]{};
+    uint32_t        count[2]{};
+};
+
 // Kernel table of one (rank, patch size, halo width) instance.
 struct ops
 {
     int rank, n, h;
     void (*step)(int variant, const double*, double*, const uint64_t*, const face_link*, const uint32_t*,
-                 step_params const&, const step_params*, const uint16_t*, uint32_t, hipStream_t);
+                 step_params const&, const step_params*, const uint16_t*, uint32_t, run_view const&, hipStream_t);
     int (*variant_count)();
     char const* (*variant_name)(int);
     void (*padded)(const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
@@ -86,14 +96,15 @@ struct instance
     // Step-kernel variants (all bit-identical; they differ in how leaves map
     // to workgroups and in how many workgroups share a CU, set through extra
     // dynamic LDS).  Variant 0 is the default until the autotuner picks.
-    static constexpr char const* kVariants3d8[]  = { "wave",          "wave-nt",     "group8", "group8-nt",
-                                                     "group8-lds56k", "wave-lds10k", "tile256" };
-    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256" };
+    static constexpr char const* kVariants3d8[]  = { "wave",          "wave-nt",     "group8",  "group8-nt",
+                                                     "group8-lds56k", "wave-lds10k", "tile256", "run4",
+                                                     "run8",          "run8-nt" };
+    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256", "run4", "run8", "run8-nt" };
     static constexpr char const* kVariantsAny[]  = { "tile" };
     static int variant_count()
     {
-        if constexpr (R == 3 && N == 8) return 7;
-        if constexpr (R == 2 && N == 16) return 4;
+        if constexpr (R == 3 && N == 8) return 10;
+        if constexpr (R == 2 && N == 16) return 7;
         return 1;
     }
     static char const* variant_name(int v)
@@ -104,10 +115,43 @@ struct instance
     }
     static void step(int variant, const double* u, double* un, const uint64_t* keys, const face_link* links,
                      const uint32_t* metas, step_params const& prm, const step_params* table,
-                     const uint16_t* inst, uint32_t n, hipStream_t s)
+                     const uint16_t* inst, uint32_t n, run_view const& runs, hipStream_t s)
     {
         if (n == 0) return;
         const dim3 per_leaf(xcd_grid(n));
+        // run variants: (list index, non-temporal stores)
+        int rk = -1;
+        bool rnt = false;
+        if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
+        {
+            const int first = (R == 3) ? 7 : 4;
+            if (variant >= first && variant < first + 3)
+            {
+                rk  = (variant == first) ? 0 : 1;
+                rnt = variant == first + 2;
+                if (!runs.list[rk]) variant = 0; // no run list: wave-per-leaf kernel
+            }
+        }
+        if (rk >= 0 && variant != 0)
+        {
+            const dim3 per_run(xcd_grid(runs.count[rk]));
+            if constexpr (R == 3 && N == 8)
+            {
+                if (rnt)
+                    hipLaunchKernelGGL(k_step_3d8_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+                else
+                    hipLaunchKernelGGL(k_step_3d8_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+            }
+            else if constexpr (R == 2 && N == 16)
+            {
+                if (rnt)
+                    hipLaunchKernelGGL(k_step_2d16_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+                else
+                    hipLaunchKernelGGL(k_step_2d16_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+            }
+            check_launch("k_step (runs)");
+            return;
+        }
         if constexpr (R == 3 && N == 8)
         {
             const dim3 per_group(xcd_grid((n + kGroup3d - 1) / kGroup3d));
@@ -271,6 +315,94 @@ int pick_fastest(int nvar, Launch&& launch, hipStream_t stream, uint32_t trials,
     return static_cast<int>(it - best.begin());
 }
 
+// Exclusive scan of n uint32 values into out; the total lands in
+// summary->total.
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* block_sums, regrid_summary* summary,
+                    hipStream_t s)
+{
+    const unsigned nb = std::max(grid1d(n, kScanBlock), 1u);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, s, in, out, n, block_sums);
+    check_launch("k_scan_blocks");
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, block_sums, nb, summary);
+    check_launch("k_scan_sums");
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_scan_add, dim3(grid1d(n, 256)), dim3(256), 0, s, out, n, block_sums);
+    check_launch("k_scan_add");
+}
+
+// The +x run lists (runs of at most 4 and 8 leaves) of one leaf table, for
+// the shapes that have run-streaming step kernels.
+struct run_lists
+{
+    leaf_run*       list[2]{};
+    uint32_t        count[2]{};
+    uint32_t*       flag       = nullptr;
+    uint32_t*       pos        = nullptr;
+    uint32_t*       block_sums = nullptr;
+    regrid_summary* summary    = nullptr;
+    regrid_summary* h_summary  = nullptr; // pinned, one per list
+
+    run_lists() = default;
+    run_lists(run_lists const&) = delete;
+    run_lists& operator=(run_lists const&) = delete;
+    ~run_lists()
+    {
+        for (void* p : { (void*)list[0], (void*)list[1], (void*)flag, (void*)pos, (void*)block_sums, (void*)summary })
+            if (p) (void)hipFree(p);
+        if (h_summary) (void)hipHostFree(h_summary);
+    }
+
+    static bool wanted(ops const* op) { return (op->rank == 3 && op->n == 8) || (op->rank == 2 && op->n == 16); }
+
+    void allocate(uint64_t cap)
+    {
+        for (int k = 0; k < 2; ++k) list[k] = dev_alloc<leaf_run>(cap);
+        flag       = dev_alloc<uint32_t>(cap);
+        pos        = dev_alloc<uint32_t>(cap);
+        block_sums = dev_alloc<uint32_t>(cap / kScanBlock + 2);
+        summary    = dev_alloc<regrid_summary>(2);
+        AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_summary), 2 * sizeof(regrid_summary), 0));
+    }
+
+    run_view view() const
+    {
+        run_view v;
+        for (int k = 0; k < 2; ++k)
+        {
+            v.list[k]  = list[k];
+            v.count[k] = count[k];
+        }
+        return v;
+    }
+
+    // Builds both lists for a valid table of n leaves and waits for them.
+    void build(int rank, const uint64_t* keys, uint32_t n, int depth, const face_link* links, const uint32_t* metas,
+               hipStream_t s)
+    {
+        if (!flag) return;
+        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, 2 * sizeof(regrid_summary), s));
+        for (int k = 0; k < 2; ++k)
+        {
+            if (rank == 2)
+                hipLaunchKernelGGL(k_run_heads<2>, dim3(grid1d(n, 256)), dim3(256), 0, s, keys, n, depth, metas, kRunK[k], flag);
+            else
+                hipLaunchKernelGGL(k_run_heads<3>, dim3(grid1d(n, 256)), dim3(256), 0, s, keys, n, depth, metas, kRunK[k], flag);
+            check_launch("k_run_heads");
+            exclusive_scan(flag, pos, n, block_sums, summary + k, s);
+            if (rank == 2)
+                hipLaunchKernelGGL(k_run_emit<2>, dim3(grid1d(n, 256)), dim3(256), 0, s, keys, n, depth, links, metas,
+                                   kRunK[k], flag, pos, list[k]);
+            else
+                hipLaunchKernelGGL(k_run_emit<3>, dim3(grid1d(n, 256)), dim3(256), 0, s, keys, n, depth, links, metas,
+                                   kRunK[k], flag, pos, list[k]);
+            check_launch("k_run_emit");
+        }
+        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, 2 * sizeof(regrid_summary), hipMemcpyDeviceToHost, s));
+        AMR_HIP_CHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < 2; ++k) count[k] = h_summary[k].total;
+    }
+};
+
 // Batch packing: shift an instance's neighbour slots by its leaf offset, and
 // tag its leaves with the instance index.
 __global__ void k_offset_links(face_link* __restrict__ links, uint64_t count, int32_t offset)
@@ -325,6 +457,7 @@ struct amr_ctx
     uint64_t        pending_steps = 0;
     bool            broken        = false;
     int             variant       = 0; // step-kernel variant (autotuner)
+    run_lists       runs;              // +x runs of the current table
 
     ~amr_ctx()
     {
@@ -380,6 +513,7 @@ struct amr_ctx
             broken = true;
             throw amr_error(AMR_ERR_TOPOLOGY, "leaves do not tile the domain or violate 2:1 face balance");
         }
+        runs.build(rank, keys[kc], static_cast<uint32_t>(n), depth, links, metas, stream);
     }
 
     void ensure_usable() const
@@ -404,7 +538,8 @@ struct amr_ctx
     {
         ensure_usable();
         const step_params prm = make_params(dt);
-        op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), stream);
+        op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
+                 runs.view(), stream);
         cur = 1 - cur;
     }
 
@@ -420,23 +555,13 @@ struct amr_ctx
             [&](int v)
             {
                 op->step(v, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr,
-                         static_cast<uint32_t>(n), stream);
+                         static_cast<uint32_t>(n), runs.view(), stream);
             },
             stream, trials, best_us);
         return variant;
     }
 
-    void scan(uint32_t count)
-    {
-        const unsigned nb = grid1d(count, kScanBlock);
-        hipLaunchKernelGGL(k_scan_blocks, dim3(std::max(nb, 1u)), dim3(kScanThreads), 0, stream, counts, pos, count,
-                           block_sums);
-        check_launch("k_scan_blocks");
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, stream, block_sums, std::max(nb, 1u), summary);
-        check_launch("k_scan_sums");
-        hipLaunchKernelGGL(k_scan_add, dim3(grid1d(count, 256)), dim3(256), 0, stream, pos, count, block_sums);
-        check_launch("k_scan_add");
-    }
+    void scan(uint32_t count) { exclusive_scan(counts, pos, count, block_sums, summary, stream); }
 
     // ndtree::reconstruct_tree (ndtree.hpp:1249-1271) with the flags already
     // in device memory.
@@ -511,6 +636,7 @@ struct amr_batch
     uint32_t*                metas   = nullptr;
     uint16_t*                inst    = nullptr;
     step_params*             table   = nullptr;
+    run_lists                runs;
     struct graph_entry
     {
         uint64_t        steps;
@@ -532,7 +658,7 @@ struct amr_batch
     void launch(int v)
     {
         op->step(v, u[cur], u[1 - cur], keys, links, metas, host_table[0], table, inst, static_cast<uint32_t>(total),
-                 stream);
+                 runs.view(), stream);
         cur = 1 - cur;
     }
 
@@ -783,7 +909,7 @@ struct amr_tape
     {
         topology const& t = topo(k);
         op->step(variant, src, dst, t.keys, t.links, t.metas, step_consts(dts[k]), nullptr, nullptr,
-                 static_cast<uint32_t>(t.n), stream);
+                 static_cast<uint32_t>(t.n), run_view{}, stream);
     }
 
     void forward(const double* u0, const double* v, double* uT)
@@ -916,6 +1042,7 @@ int amr_tree_create(const amr_tree_config* cfg, amr_ctx** out)
             AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
             c->n         = 1;
             c->max_level = 0;
+            if (run_lists::wanted(op)) c->runs.allocate(cap);
             c->build_table();
             *out = c.release();
         }
@@ -1307,6 +1434,12 @@ int amr_batch_create(amr_ctx* const* trees, uint32_t count, amr_batch** out)
             }
             AMR_HIP_CHECK(hipMemcpyAsync(b->table, b->host_table.data(), count * sizeof(step_params),
                                          hipMemcpyHostToDevice, b->stream));
+            if (run_lists::wanted(b->op))
+            {
+                b->runs.allocate(b->total);
+                b->runs.build(b->op->rank, b->keys, static_cast<uint32_t>(b->total), trees[0]->depth, b->links,
+                              b->metas, b->stream);
+            }
             AMR_HIP_CHECK(hipStreamSynchronize(b->stream));
             *out = b.release();
         }
@@ -1402,7 +1535,7 @@ int amr_batch_autotune(amr_batch* b, uint32_t trial_steps, int32_t* chosen, floa
                 [&](int v)
                 {
                     b->op->step(v, b->u[b->cur], b->u[1 - b->cur], b->keys, b->links, b->metas, b->host_table[0],
-                                b->table, b->inst, static_cast<uint32_t>(b->total), b->stream);
+                                b->table, b->inst, static_cast<uint32_t>(b->total), b->runs.view(), b->stream);
                 },
                 b->stream, trial_steps, best_us);
             if (chosen) *chosen = b->variant;

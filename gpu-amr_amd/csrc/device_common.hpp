@@ -58,6 +58,15 @@ __host__ __device__ inline void deinterleave(uint64_t m, uint32_t* c)
 
 __host__ __device__ inline int key_level(uint64_t k) { return static_cast<int>(k & 0x3Fu); }
 
+// A run: `len` same-level leaves that follow each other along +x, starting
+// at leaf `head` (see run_head / run_length).  The run-streaming step kernels
+// give each run to one wave, so the x faces inside a run move through
+// registers instead of memory.
+struct leaf_run
+{
+    uint32_t head, len;
+};
+
 template <int R>
 __host__ __device__ inline uint64_t make_key(const uint32_t* c, int level)
 {

@@ -364,6 +364,110 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
     }
 }
 
+// 3D 8^3 patches, one wave per run of same-level leaves along +x (leaf_run).
+// The lane layout is k_step_3d8's.  Walking the run front to back, the -x
+// ghosts of a leaf are the previous leaf's x = 7 column (left in LDS) and
+// its +x ghosts the next leaf's x = 0 column (that leaf's interior is
+// loaded before the current leaf is updated, so its loads overlap the
+// update); only the run's two end faces are gathered from memory.  An x face
+// touches every 64 B row of the neighbour patch, so this removes most of the
+// ghost traffic that does not hit in L2.
+template <bool NT>
+__global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ u, double* __restrict__ un,
+                                                     const uint64_t* __restrict__ keys,
+                                                     const face_link* __restrict__ links,
+                                                     const uint32_t* __restrict__ metas, step_params prm,
+                                                     const step_params* __restrict__ table,
+                                                     const uint16_t* __restrict__ inst,
+                                                     const leaf_run* __restrict__ runs, uint32_t nruns)
+{
+    constexpr int NI = 512;
+    __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
+    const uint32_t r = xcd_item(blockIdx.x, gridDim.x);
+    if (r >= nruns) return;
+    const int      lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
+    const int      y = hi3, x = lo3;
+    const leaf_run run  = runs[r];
+    uint32_t       leaf = run.head;
+    double         cur[8], nxt[8];
+    {
+        const double* up = u + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+        for (int z = 0; z < 8; ++z)
+        {
+            cur[z] = up[z * 64 + lane];
+            nxt[z] = 0.0;
+        }
+    }
+    for (uint32_t i = 0; i < run.len; ++i)
+    {
+        const face_link* L    = links + static_cast<size_t>(leaf) * 6;
+        const uint32_t*  M    = metas + static_cast<size_t>(leaf) * 6;
+        const bool       more = i + 1 < run.len;
+        // ghosts of this leaf first, then the next leaf's interior, so that
+        // the update below waits only for the ghosts
+        int c[3];
+        c[0] = -1, c[1] = hi3, c[2] = lo3;
+        const double gzm = ghost_value<3, 8>(u, L[0], M[0], 0, c);
+        c[0]             = 8;
+        const double gzp = ghost_value<3, 8>(u, L[1], M[1], 1, c);
+        c[0] = hi3, c[1] = -1, c[2] = lo3;
+        gface[0][lane] = ghost_value<3, 8>(u, L[2], M[2], 2, c);
+        c[1]           = 8;
+        gface[1][lane] = ghost_value<3, 8>(u, L[3], M[3], 3, c);
+        c[0] = hi3, c[1] = lo3, c[2] = -1;
+        if (i == 0) gface[2][lane] = ghost_value<3, 8>(u, L[4], M[4], 4, c);
+        c[2] = 8;
+        if (!more) gface[3][lane] = ghost_value<3, 8>(u, L[5], M[5], 5, c);
+        const uint32_t next = more ? static_cast<uint32_t>(L[5].ids[0]) : leaf;
+        if (more)
+        {
+            const double* np = u + static_cast<size_t>(next) * NI;
+#pragma unroll
+            for (int z = 0; z < 8; ++z) nxt[z] = np[z * 64 + lane];
+        }
+        __syncthreads();
+        const step_params& P       = params_for(prm, table, inst, leaf);
+        const int          lv      = key_level(keys[leaf]);
+        const double       dtdx[3] = { P.dtdx[lv][0], P.dtdx[lv][1], P.dtdx[lv][2] };
+        double*            out     = un + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+        for (int z = 0; z < 8; ++z)
+        {
+            const double uc = cur[z];
+            const double xl = __shfl_up(uc, 1, 64);
+            const double xr = __shfl_down(uc, 1, 64);
+            const double yl = __shfl_up(uc, 8, 64);
+            const double yr = __shfl_down(uc, 8, 64);
+            const double xn = __shfl(nxt[z], lane & ~7, 64); // next leaf, x = 0, same (z, y)
+            double       lo[3], hi[3];
+            lo[0] = (x == 0) ? gface[2][z * 8 + y] : xl;
+            hi[0] = (x == 7) ? (more ? xn : gface[3][z * 8 + y]) : xr;
+            lo[1] = (y == 0) ? gface[0][z * 8 + x] : yl;
+            hi[1] = (y == 7) ? gface[1][z * 8 + x] : yr;
+            if (z == 0)
+                lo[2] = gzm;
+            else
+                lo[2] = cur[z > 0 ? z - 1 : 0];
+            if (z == 7)
+                hi[2] = gzp;
+            else
+                hi[2] = cur[z < 7 ? z + 1 : 7];
+            store_cell<NT>(out + z * 64 + lane, upwind_update<3>(uc, lo, hi, dtdx, P.vel, P.hs));
+        }
+        __syncthreads();
+        // this leaf's x = 7 column is the next leaf's -x ghost face
+#pragma unroll
+        for (int z = 0; z < 8; ++z)
+        {
+            if (x == 7) gface[2][z * 8 + y] = cur[z];
+            cur[z] = nxt[z];
+        }
+        leaf = next;
+        __syncthreads();
+    }
+}
+
 // 3D 8^3 patches, one workgroup of 8 waves per group of 8 consecutive
 // leaves (a sibling octet wherever the tree is locally uniform), one wave per
 // leaf as in k_step_3d8.  Every wave publishes its six boundary faces in LDS;
@@ -551,6 +655,99 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
         else
             hi[1] = (q == 3) ? gface[1][x] : below;
         store_cell<NT>(out + y * 16 + x, upwind_update<2>(uc, lo, hi, dtdx, P.vel, P.hs));
+    }
+}
+
+// 2D 16^2 patches, one wave per run of same-level leaves along +x, the lane
+// layout of k_step_2d16 and the hand-over of k_step_3d8_run: the -x ghosts
+// of a leaf are the previous leaf's x = 15 column (left in LDS), its +x
+// ghosts the next leaf's x = 0 column (registers of the prefetched leaf).
+template <bool NT>
+__global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__ u, double* __restrict__ un,
+                                                      const uint64_t* __restrict__ keys,
+                                                      const face_link* __restrict__ links,
+                                                      const uint32_t* __restrict__ metas, step_params prm,
+                                                      const step_params* __restrict__ table,
+                                                      const uint16_t* __restrict__ inst,
+                                                      const leaf_run* __restrict__ runs, uint32_t nruns)
+{
+    constexpr int NI = 256;
+    __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
+    const uint32_t r = xcd_item(blockIdx.x, gridDim.x);
+    if (r >= nruns) return;
+    const int      lane = threadIdx.x, x = lane & 15, q = lane >> 4;
+    const leaf_run run  = runs[r];
+    uint32_t       leaf = run.head;
+    double         cur[4], nxt[4];
+    {
+        const double* up = u + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            cur[j] = up[(4 * q + j) * 16 + x];
+            nxt[j] = 0.0;
+        }
+    }
+    for (uint32_t i = 0; i < run.len; ++i)
+    {
+        const bool   more = i + 1 < run.len;
+        const size_t e0   = static_cast<size_t>(leaf) * 4;
+        {
+            // lanes of quarter q gather face q (y faces always, the run's end x faces)
+            const int f = q, t = x;
+            if (f < 2 || (f == 2 && i == 0) || (f == 3 && !more))
+            {
+                int c[2];
+                c[0]        = (f == 0) ? -1 : (f == 1) ? 16 : t;
+                c[1]        = (f == 2) ? -1 : (f == 3) ? 16 : t;
+                gface[f][t] = ghost_value<2, 16>(u, links[e0 + f], metas[e0 + f], f, c);
+            }
+        }
+        const uint32_t next = more ? static_cast<uint32_t>(links[e0 + 3].ids[0]) : leaf;
+        if (more)
+        {
+            const double* np = u + static_cast<size_t>(next) * NI;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) nxt[j] = np[(4 * q + j) * 16 + x];
+        }
+        __syncthreads();
+        const step_params& P       = params_for(prm, table, inst, leaf);
+        const int          lv      = key_level(keys[leaf]);
+        const double       dtdx[2] = { P.dtdx[lv][0], P.dtdx[lv][1] };
+        const double       above   = __shfl_up(cur[3], 16, 64);  // row 4q - 1
+        const double       below   = __shfl_down(cur[0], 16, 64); // row 4q + 4
+        double*            out     = un + static_cast<size_t>(leaf) * NI;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int    y  = 4 * q + j;
+            const double uc = cur[j];
+            const double xl = __shfl_up(uc, 1, 64);
+            const double xr = __shfl_down(uc, 1, 64);
+            const double xn = __shfl(nxt[j], lane & ~15, 64); // next leaf, x = 0, same row
+            double       lo[2], hi[2];
+            lo[0] = (x == 0) ? gface[2][y] : xl;
+            hi[0] = (x == 15) ? (more ? xn : gface[3][y]) : xr;
+            if (j > 0)
+                lo[1] = cur[j > 0 ? j - 1 : 0];
+            else
+                lo[1] = (q == 0) ? gface[0][x] : above;
+            if (j < 3)
+                hi[1] = cur[j < 3 ? j + 1 : 3];
+            else
+                hi[1] = (q == 3) ? gface[1][x] : below;
+            store_cell<NT>(out + y * 16 + x, upwind_update<2>(uc, lo, hi, dtdx, P.vel, P.hs));
+        }
+        __syncthreads();
+        // this leaf's x = 15 column is the next leaf's -x ghost face
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            if (x == 15) gface[2][4 * q + j] = cur[j];
+            cur[j] = nxt[j];
+        }
+        leaf = next;
+        __syncthreads();
     }
 }
 

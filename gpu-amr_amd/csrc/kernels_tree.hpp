@@ -347,6 +347,56 @@ __global__ void k_emit_counts(const uint64_t* __restrict__ keys, uint32_t n, int
     if (mark[i]) atomicAdd(&summary->n_refine, 1u);
 }
 
+// Runs along +x (layout dim R-1; directions 2(R-1) = -x, 2(R-1)+1 = +x).  A
+// leaf heads a run when its x index at its own level is a multiple of K or
+// when its -x neighbour is not a same-level leaf; the run continues through
+// same-level +x neighbours that are not heads, so it holds at most K leaves
+// and every leaf belongs to exactly one run (a periodic same-level cycle
+// always contains x index 0).
+template <int R>
+__host__ __device__ inline bool run_head(const uint64_t* keys, int depth, const uint32_t* metas, uint32_t i, uint32_t K)
+{
+    uint32_t c[R];
+    deinterleave<R>(keys[i] >> 6, c);
+    const uint32_t xl = c[0] >> (depth - key_level(keys[i]));
+    return (xl % K) == 0 || rel_of(metas[static_cast<size_t>(i) * 2 * R + 2 * (R - 1)]) != REL_SAME;
+}
+
+template <int R>
+__host__ __device__ inline uint32_t run_length(const uint64_t* keys, int depth, const face_link* links,
+                                               const uint32_t* metas, uint32_t head, uint32_t K)
+{
+    uint32_t j = head, len = 1;
+    while (len < K)
+    {
+        const size_t e = static_cast<size_t>(j) * 2 * R + 2 * (R - 1) + 1;
+        if (rel_of(metas[e]) != REL_SAME) break;
+        const uint32_t nb = static_cast<uint32_t>(links[e].ids[0]);
+        if (run_head<R>(keys, depth, metas, nb, K)) break;
+        j = nb;
+        ++len;
+    }
+    return len;
+}
+
+template <int R>
+__global__ void k_run_heads(const uint64_t* __restrict__ keys, uint32_t n, int depth,
+                            const uint32_t* __restrict__ metas, uint32_t K, uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = run_head<R>(keys, depth, metas, i, K) ? 1u : 0u;
+}
+
+template <int R>
+__global__ void k_run_emit(const uint64_t* __restrict__ keys, uint32_t n, int depth,
+                           const face_link* __restrict__ links, const uint32_t* __restrict__ metas, uint32_t K,
+                           const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                           leaf_run* __restrict__ runs)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) runs[pos[i]] = leaf_run{ i, run_length<R>(keys, depth, links, metas, i, K) };
+}
+
 // Exclusive scan of uint32 counts: per-block scan, scan of block sums in one
 // workgroup, then add.  1024 items per block (256 threads x 4).
 constexpr int kScanThreads = 256;

@@ -280,6 +280,23 @@ int emu_regrid(int rank, int nn, int depth, const uint64_t* keys, uint32_t n, co
     return -1;
 }
 
+// Run list (head, len pairs in leaf order) of a valid table; returns the count.
+int emu_runs(int rank, int depth, const uint64_t* keys, uint32_t n, const void* links, const uint32_t* metas,
+             uint32_t K, uint32_t* out)
+{
+    auto const* L     = static_cast<const face_link*>(links);
+    int         count = 0;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const bool h = rank == 2 ? run_head<2>(keys, depth, metas, i, K) : run_head<3>(keys, depth, metas, i, K);
+        if (!h) continue;
+        out[2 * count]     = i;
+        out[2 * count + 1] = rank == 2 ? run_length<2>(keys, depth, L, metas, i, K) : run_length<3>(keys, depth, L, metas, i, K);
+        ++count;
+    }
+    return count;
+}
+
 // dtdx: [17][3]; vel: 3 values (vp, vm and sgn are derived as on the device).
 int emu_adj_step(int rank, int nn, const uint64_t* keys, uint32_t n, const void* links, const uint32_t* metas,
                  const double* lam_next, const double* u, const double* dtdx, const double* vel, double* lam,

@@ -26,7 +26,7 @@ def load():
             build()
         _lib = ctypes.CDLL(str(LIB))
         for name in ("emu_build_table", "emu_step", "emu_padded", "emu_flags", "emu_regrid", "emu_adj_step",
-                     "emu_adj_regrid"):
+                     "emu_adj_regrid", "emu_runs"):
             getattr(_lib, name).restype = ctypes.c_int
     return _lib
 
@@ -142,3 +142,12 @@ def adj_regrid(rank, n, lam_new, maps, n_old):
     rc = load().emu_adj_regrid(rank, n, _p(lam_new), _p(maps), ctypes.c_uint32(n_old), _p(lam_old))
     assert rc == 0
     return lam_old
+
+
+def runs(rank, depth, keys, links, metas, K):
+    """(head, len) rows of the +x run list of a table."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    out = np.zeros((keys.size, 2), dtype=np.uint32)
+    count = load().emu_runs(rank, depth, _p(keys), ctypes.c_uint32(keys.size), _p(links), _p(metas),
+                            ctypes.c_uint32(K), _p(out))
+    return out[:count]

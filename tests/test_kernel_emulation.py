@@ -87,3 +87,38 @@ def test_emulated_table_detects_unbalanced_tree():
     keys = np.array(sorted(keys), dtype=np.uint64)
     _, _, err = emu.build_table(2, 5, keys)
     assert err == 1
+
+
+@pytest.mark.parametrize("rank,n,depth,start,rounds,seed", [(2, 16, 5, 2, 5, 21), (3, 8, 4, 1, 4, 22), (2, 4, 6, 3, 6, 23)])
+def test_run_lists_partition_the_leaves(rank, n, depth, start, rounds, seed):
+    """The +x run lists used by the run-streaming step kernels: every leaf in
+    exactly one run, consecutive leaves of a run are same-level +x
+    neighbours, runs hold at most K leaves and start where the x index at the
+    leaf's level is a multiple of K or the -x neighbour is not same-level."""
+    rng = np.random.default_rng(seed)
+    o = OracleTree(rank, n, 1, depth, 2 * ((1 << (rank * depth)) + 64))
+    for _ in range(start):
+        o.refine_all()
+    for rd in range(rounds + 1):
+        keys = o.keys()
+        links, metas, err = emu.build_table(rank, depth, keys)
+        assert err == 0
+        mx, px = 2 * (rank - 1), 2 * (rank - 1) + 1
+        for K in (1, 4, 8):
+            seen = np.zeros(keys.size, dtype=np.int64)
+            for head, ln in emu.runs(rank, depth, keys, links, metas, K):
+                assert 1 <= ln <= K
+                j = int(head)
+                key = int(keys[j])
+                level = key & 63
+                x = brute.deinterleave(key >> 6, rank)[0] >> (depth - level)
+                assert x % K == 0 or (metas[j * 2 * rank + mx] & 3) != 1
+                for t in range(ln):
+                    seen[j] += 1
+                    if t + 1 < ln:
+                        e = j * 2 * rank + px
+                        assert (metas[e] & 3) == 1, "run continues through a same-level +x neighbour"
+                        j = int(links["ids"][e, 0])
+            assert np.all(seen == 1), f"round {rd} K {K}: leaves not covered exactly once"
+        if rd < rounds:
+            o.reconstruct(random_flags(rng, keys, rank, depth))
