@@ -252,6 +252,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-instances", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
+    ap.add_argument("--variant", default="", help="step-kernel variant to run instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
@@ -273,19 +274,26 @@ def main():
     ev = HipEvents()
     e0, e1 = ev.create(), ev.create()
     stream = g.stream()
-    # per-variant timing (diagnostic), then the autotuner's choice
     names = s.step_variants()
     per_variant = {}
-    for v, name in enumerate(names):
-        s.set_step_variant(v)
-        g.synchronize()
-        ev.record(e0, stream)
-        s.advance_batch_async(20)
-        ev.record(e1, stream)
-        s.finish_advance_batch()
-        per_variant[name] = round(ev.elapsed_ms(e0, e1) / 20 * 1e3, 2)
-    best, best_us = s.autotune(20)
-    log(f"step variants (us/step): {per_variant}; autotuner -> {names[best]} ({best_us:.1f} us)")
+    if args.variant:
+        # one named variant only (profiling runs)
+        best = names.index(args.variant)
+        s.set_step_variant(best)
+        how = "fixed"
+    else:
+        # per-variant timing (diagnostic), then the autotuner's choice
+        for v, name in enumerate(names):
+            s.set_step_variant(v)
+            g.synchronize()
+            ev.record(e0, stream)
+            s.advance_batch_async(20)
+            ev.record(e1, stream)
+            s.finish_advance_batch()
+            per_variant[name] = round(ev.elapsed_ms(e0, e1) / 20 * 1e3, 2)
+        best, best_us = s.autotune(20)
+        how = "autotuned"
+        log(f"step variants (us/step): {per_variant}; autotuner -> {names[best]} ({best_us:.1f} us)")
 
     # the read+write streaming rate of a plain copy of the same state size
     copy_ms = copy_kernel(cells, 20, ev)
@@ -353,7 +361,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(),
-            "kernel": f"step kernel variant '{names[best]}' (autotuned)",
+            "kernel": f"step kernel variant '{names[best]}' ({how})",
             "kernel_us": kernel_ms * 1e3,
             "variants_us": per_variant,
             "copy_GBps": copy_gbs,

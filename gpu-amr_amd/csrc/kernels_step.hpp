@@ -162,19 +162,33 @@ __host__ __device__ inline void stage_ghost(double* tile, const double* __restri
 // AdvectionPhysics::rusanovFlux (AdvectionPhysics.hpp:45-66):
 //   total = 0; total -= dt/dx_d * (F(u, u_R) - F(u_L, u)) for d = x, y(, z);
 //   F(a, b) = 0.5 * (a v + b v) - (0.5 |v|) (b - a); out = u + total.
+__host__ __device__ inline double upwind_flux(double a, double b, double v, double hs)
+{
+    return 0.5 * (a * v + b * v) - hs * (b - a);
+}
+
+// The update from the cell's face fluxes, fL[d] = F(u_L, u), fR[d] = F(u, u_R).
+// A face flux is the same bits whether computed for the cell on its left or
+// on its right, so kernels may evaluate each face once.
+template <int R>
+__host__ __device__ inline double update_from_fluxes(double uc, const double* fL, const double* fR, const double* dtdx)
+{
+    double total = 0.0;
+    for (int d = 0; d < R; ++d) total -= dtdx[d] * (fR[d] - fL[d]);
+    return uc + total;
+}
+
 template <int R>
 __host__ __device__ inline double upwind_update(double uc, const double* lo, const double* hi, const double* dtdx,
                                                 const double* vel, const double* hs)
 {
-    double total = 0.0;
+    double fL[R], fR[R];
     for (int d = 0; d < R; ++d)
     {
-        const double v  = vel[d];
-        const double fL = 0.5 * (lo[d] * v + uc * v) - hs[d] * (uc - lo[d]);
-        const double fR = 0.5 * (uc * v + hi[d] * v) - hs[d] * (hi[d] - uc);
-        total -= dtdx[d] * (fR - fL);
+        fL[d] = upwind_flux(lo[d], uc, vel[d], hs[d]);
+        fR[d] = upwind_flux(uc, hi[d], vel[d], hs[d]);
     }
-    return uc + total;
+    return update_from_fluxes<R>(uc, fL, fR, dtdx);
 }
 
 // Upwind update of interior cell i from the staged tile.
@@ -339,6 +353,8 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
     const double       dtdx[3] = { P.dtdx[lv][0], P.dtdx[lv][1], P.dtdx[lv][2] };
     const int    y = hi3, x = lo3;
     double*      out = un + static_cast<size_t>(leaf) * NI;
+    // z faces are evaluated once: the +z flux of cell z is the -z flux of z+1
+    double fz = upwind_flux(gzm, r[0], P.vel[2], P.hs[2]);
 #pragma unroll
     for (int z = 0; z < 8; ++z)
     {
@@ -347,20 +363,16 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
         const double xr = __shfl_down(uc, 1, 64);
         const double yl = __shfl_up(uc, 8, 64);
         const double yr = __shfl_down(uc, 8, 64);
-        double       lo[3], hi[3];
-        lo[0] = (x == 0) ? gface[2][z * 8 + y] : xl;
-        hi[0] = (x == 7) ? gface[3][z * 8 + y] : xr;
-        lo[1] = (y == 0) ? gface[0][z * 8 + x] : yl;
-        hi[1] = (y == 7) ? gface[1][z * 8 + x] : yr;
-        if (z == 0)
-            lo[2] = gzm;
-        else
-            lo[2] = r[z > 0 ? z - 1 : 0];
-        if (z == 7)
-            hi[2] = gzp;
-        else
-            hi[2] = r[z < 7 ? z + 1 : 7];
-        store_cell<NT>(out + z * 64 + lane, upwind_update<3>(uc, lo, hi, dtdx, P.vel, P.hs));
+        const double lx = (x == 0) ? gface[2][z * 8 + y] : xl;
+        const double hx = (x == 7) ? gface[3][z * 8 + y] : xr;
+        const double ly = (y == 0) ? gface[0][z * 8 + x] : yl;
+        const double hy = (y == 7) ? gface[1][z * 8 + x] : yr;
+        const double hz = (z == 7) ? gzp : r[z < 7 ? z + 1 : 7];
+        const double fzr = upwind_flux(uc, hz, P.vel[2], P.hs[2]);
+        const double fL[3] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), upwind_flux(ly, uc, P.vel[1], P.hs[1]), fz };
+        const double fR[3] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), upwind_flux(uc, hy, P.vel[1], P.hs[1]), fzr };
+        store_cell<NT>(out + z * 64 + lane, update_from_fluxes<3>(uc, fL, fR, dtdx));
+        fz = fzr;
     }
 }
 
@@ -431,6 +443,7 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
         const int          lv      = key_level(keys[leaf]);
         const double       dtdx[3] = { P.dtdx[lv][0], P.dtdx[lv][1], P.dtdx[lv][2] };
         double*            out     = un + static_cast<size_t>(leaf) * NI;
+        double             fz      = upwind_flux(gzm, cur[0], P.vel[2], P.hs[2]); // z faces once
 #pragma unroll
         for (int z = 0; z < 8; ++z)
         {
@@ -440,20 +453,16 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
             const double yl = __shfl_up(uc, 8, 64);
             const double yr = __shfl_down(uc, 8, 64);
             const double xn = __shfl(nxt[z], lane & ~7, 64); // next leaf, x = 0, same (z, y)
-            double       lo[3], hi[3];
-            lo[0] = (x == 0) ? gface[2][z * 8 + y] : xl;
-            hi[0] = (x == 7) ? (more ? xn : gface[3][z * 8 + y]) : xr;
-            lo[1] = (y == 0) ? gface[0][z * 8 + x] : yl;
-            hi[1] = (y == 7) ? gface[1][z * 8 + x] : yr;
-            if (z == 0)
-                lo[2] = gzm;
-            else
-                lo[2] = cur[z > 0 ? z - 1 : 0];
-            if (z == 7)
-                hi[2] = gzp;
-            else
-                hi[2] = cur[z < 7 ? z + 1 : 7];
-            store_cell<NT>(out + z * 64 + lane, upwind_update<3>(uc, lo, hi, dtdx, P.vel, P.hs));
+            const double lx = (x == 0) ? gface[2][z * 8 + y] : xl;
+            const double hx = (x == 7) ? (more ? xn : gface[3][z * 8 + y]) : xr;
+            const double ly = (y == 0) ? gface[0][z * 8 + x] : yl;
+            const double hy = (y == 7) ? gface[1][z * 8 + x] : yr;
+            const double hz = (z == 7) ? gzp : cur[z < 7 ? z + 1 : 7];
+            const double fzr = upwind_flux(uc, hz, P.vel[2], P.hs[2]);
+            const double fL[3] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), upwind_flux(ly, uc, P.vel[1], P.hs[1]), fz };
+            const double fR[3] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), upwind_flux(uc, hy, P.vel[1], P.hs[1]), fzr };
+            store_cell<NT>(out + z * 64 + lane, update_from_fluxes<3>(uc, fL, fR, dtdx));
+            fz = fzr;
         }
         __syncthreads();
         // this leaf's x = 7 column is the next leaf's -x ghost face
@@ -636,6 +645,8 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
     const double above   = __shfl_up(r[3], 16, 64);  // row 4q - 1
     const double below   = __shfl_down(r[0], 16, 64); // row 4q + 4
     double*      out     = un + static_cast<size_t>(leaf) * NI;
+    // y faces are evaluated once: the +y flux of row j is the -y flux of j+1
+    double fy = upwind_flux((q == 0) ? gface[0][x] : above, r[0], P.vel[1], P.hs[1]);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
     {
@@ -643,18 +654,14 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
         const double uc = r[j];
         const double xl = __shfl_up(uc, 1, 64);
         const double xr = __shfl_down(uc, 1, 64);
-        double       lo[2], hi[2];
-        lo[0] = (x == 0) ? gface[2][y] : xl;
-        hi[0] = (x == 15) ? gface[3][y] : xr;
-        if (j > 0)
-            lo[1] = r[j > 0 ? j - 1 : 0];
-        else
-            lo[1] = (q == 0) ? gface[0][x] : above;
-        if (j < 3)
-            hi[1] = r[j < 3 ? j + 1 : 3];
-        else
-            hi[1] = (q == 3) ? gface[1][x] : below;
-        store_cell<NT>(out + y * 16 + x, upwind_update<2>(uc, lo, hi, dtdx, P.vel, P.hs));
+        const double lx = (x == 0) ? gface[2][y] : xl;
+        const double hx = (x == 15) ? gface[3][y] : xr;
+        const double hy = (j < 3) ? r[j < 3 ? j + 1 : 3] : ((q == 3) ? gface[1][x] : below);
+        const double fyr   = upwind_flux(uc, hy, P.vel[1], P.hs[1]);
+        const double fL[2] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), fy };
+        const double fR[2] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), fyr };
+        store_cell<NT>(out + y * 16 + x, update_from_fluxes<2>(uc, fL, fR, dtdx));
+        fy = fyr;
     }
 }
 
@@ -717,6 +724,7 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
         const double       above   = __shfl_up(cur[3], 16, 64);  // row 4q - 1
         const double       below   = __shfl_down(cur[0], 16, 64); // row 4q + 4
         double*            out     = un + static_cast<size_t>(leaf) * NI;
+        double             fy      = upwind_flux((q == 0) ? gface[0][x] : above, cur[0], P.vel[1], P.hs[1]);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
@@ -725,18 +733,14 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
             const double xl = __shfl_up(uc, 1, 64);
             const double xr = __shfl_down(uc, 1, 64);
             const double xn = __shfl(nxt[j], lane & ~15, 64); // next leaf, x = 0, same row
-            double       lo[2], hi[2];
-            lo[0] = (x == 0) ? gface[2][y] : xl;
-            hi[0] = (x == 15) ? (more ? xn : gface[3][y]) : xr;
-            if (j > 0)
-                lo[1] = cur[j > 0 ? j - 1 : 0];
-            else
-                lo[1] = (q == 0) ? gface[0][x] : above;
-            if (j < 3)
-                hi[1] = cur[j < 3 ? j + 1 : 3];
-            else
-                hi[1] = (q == 3) ? gface[1][x] : below;
-            store_cell<NT>(out + y * 16 + x, upwind_update<2>(uc, lo, hi, dtdx, P.vel, P.hs));
+            const double lx = (x == 0) ? gface[2][y] : xl;
+            const double hx = (x == 15) ? (more ? xn : gface[3][y]) : xr;
+            const double hy = (j < 3) ? cur[j < 3 ? j + 1 : 3] : ((q == 3) ? gface[1][x] : below);
+            const double fyr   = upwind_flux(uc, hy, P.vel[1], P.hs[1]);
+            const double fL[2] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), fy };
+            const double fR[2] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), fyr };
+            store_cell<NT>(out + y * 16 + x, update_from_fluxes<2>(uc, fL, fR, dtdx));
+            fy = fyr;
         }
         __syncthreads();
         // this leaf's x = 15 column is the next leaf's -x ghost face
