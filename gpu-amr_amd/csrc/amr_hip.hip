@@ -57,14 +57,24 @@ constexpr int block_threads(int cells)
     return t;
 }
 
-// The +x run lists of one leaf table (for the run-streaming step kernels),
-// runs of at most kRunK[k] leaves; an empty view makes the run variants fall
-// back to the wave-per-leaf kernel.
+// Per-launch extras of the step kernels: the +x run lists of the table
+// (runs of at most kRunK[k] leaves) and the x-face side buffers (xf_in: of
+// the state read, xf_out: of the state written).  A variant whose extras are
+// missing falls back to the wave-per-leaf kernel.
 constexpr uint32_t kRunK[2] = { 4, 8 };
-struct run_view
+struct step_aux
 {
     const leaf_run* list[2]{};
     uint32_t        count[2]{};
+    const double*   xf_in  = nullptr;
+    double*         xf_out = nullptr;
+};
+
+// What a step-kernel variant needs besides the table.
+enum : int
+{
+    NEEDS_RUNS   = 1,
+    NEEDS_XFACES = 2,
 };
 
 // Kernel table of one (rank, patch size, halo width) instance.
@@ -72,9 +82,11 @@ struct ops
 {
     int rank, n, h;
     void (*step)(int variant, const double*, double*, const uint64_t*, const face_link*, const uint32_t*,
-                 step_params const&, const step_params*, const uint16_t*, uint32_t, run_view const&, hipStream_t);
+                 step_params const&, const step_params*, const uint16_t*, uint32_t, step_aux const&, hipStream_t);
     int (*variant_count)();
     char const* (*variant_name)(int);
+    int (*variant_needs)(int);
+    void (*xfaces)(const double*, double*, uint32_t, hipStream_t);
     void (*padded)(const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
     void (*criterion)(int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double, double,
                       int, int, int8_t*, uint32_t, hipStream_t);
@@ -98,14 +110,31 @@ struct instance
     // dynamic LDS).  Variant 0 is the default until the autotuner picks.
     static constexpr char const* kVariants3d8[]  = { "wave",          "wave-nt",     "group8",  "group8-nt",
                                                      "group8-lds56k", "wave-lds10k", "tile256", "run4",
-                                                     "run8",          "run8-nt" };
-    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256", "run4", "run8", "run8-nt" };
+                                                     "run8",          "run8-nt",     "wave-xf" };
+    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256",
+                                                     "run4", "run8",    "run8-nt",     "wave-xf" };
     static constexpr char const* kVariantsAny[]  = { "tile" };
     static int variant_count()
     {
-        if constexpr (R == 3 && N == 8) return 10;
-        if constexpr (R == 2 && N == 16) return 7;
+        if constexpr (R == 3 && N == 8) return 11;
+        if constexpr (R == 2 && N == 16) return 8;
         return 1;
+    }
+    static int variant_needs(int v)
+    {
+        if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
+        {
+            const int run0 = (R == 3) ? 7 : 4;
+            if (v >= run0 && v < run0 + 3) return NEEDS_RUNS;
+            if (v == run0 + 3) return NEEDS_XFACES;
+        }
+        return 0;
+    }
+    static void xfaces(const double* u, double* xf, uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_xfaces<R, N, 64>), dim3(n), dim3(64), 0, s, u, xf, n);
+        check_launch("k_xfaces");
     }
     static char const* variant_name(int v)
     {
@@ -115,7 +144,7 @@ struct instance
     }
     static void step(int variant, const double* u, double* un, const uint64_t* keys, const face_link* links,
                      const uint32_t* metas, step_params const& prm, const step_params* table,
-                     const uint16_t* inst, uint32_t n, run_view const& runs, hipStream_t s)
+                     const uint16_t* inst, uint32_t n, step_aux const& aux, hipStream_t s)
     {
         if (n == 0) return;
         const dim3 per_leaf(xcd_grid(n));
@@ -129,25 +158,41 @@ struct instance
             {
                 rk  = (variant == first) ? 0 : 1;
                 rnt = variant == first + 2;
-                if (!runs.list[rk]) variant = 0; // no run list: wave-per-leaf kernel
+                if (!aux.list[rk]) variant = 0; // no run list: wave-per-leaf kernel
+            }
+        }
+        if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
+        {
+            if (variant_needs(variant) & NEEDS_XFACES)
+            {
+                if (aux.xf_in && aux.xf_out)
+                {
+                    if constexpr (R == 3)
+                        hipLaunchKernelGGL((k_step_3d8<false, true>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, aux.xf_in, aux.xf_out);
+                    else
+                        hipLaunchKernelGGL((k_step_2d16<false, true>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, aux.xf_in, aux.xf_out);
+                    check_launch("k_step (x-face buffers)");
+                    return;
+                }
+                variant = 0; // no side buffers: wave-per-leaf kernel
             }
         }
         if (rk >= 0 && variant != 0)
         {
-            const dim3 per_run(xcd_grid(runs.count[rk]));
+            const dim3 per_run(xcd_grid(aux.count[rk]));
             if constexpr (R == 3 && N == 8)
             {
                 if (rnt)
-                    hipLaunchKernelGGL(k_step_3d8_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+                    hipLaunchKernelGGL(k_step_3d8_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
                 else
-                    hipLaunchKernelGGL(k_step_3d8_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+                    hipLaunchKernelGGL(k_step_3d8_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
             }
             else if constexpr (R == 2 && N == 16)
             {
                 if (rnt)
-                    hipLaunchKernelGGL(k_step_2d16_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+                    hipLaunchKernelGGL(k_step_2d16_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
                 else
-                    hipLaunchKernelGGL(k_step_2d16_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, runs.list[rk], runs.count[rk]);
+                    hipLaunchKernelGGL(k_step_2d16_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
             }
             check_launch("k_step (runs)");
             return;
@@ -228,7 +273,8 @@ struct instance
     }
     static constexpr ops table()
     {
-        return ops{ R, N, H, &step, &variant_count, &variant_name, &padded, &criterion, &emit, &adj_step, &adj_regrid };
+        return ops{ R,       N,          H,       &step,     &variant_count, &variant_name, &variant_needs,
+                    &xfaces, &padded,    &criterion, &emit, &adj_step,      &adj_regrid };
     }
 };
 
@@ -364,9 +410,9 @@ struct run_lists
         AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_summary), 2 * sizeof(regrid_summary), 0));
     }
 
-    run_view view() const
+    step_aux view() const
     {
-        run_view v;
+        step_aux v;
         for (int k = 0; k < 2; ++k)
         {
             v.list[k]  = list[k];
@@ -402,6 +448,87 @@ struct run_lists
         for (int k = 0; k < 2; ++k) count[k] = h_summary[k].total;
     }
 };
+
+// The x-face side buffers of a ping-pong pair of interior buffers (index =
+// interior buffer index) and whether those of the current buffer match it.
+struct xface_pair
+{
+    double* xf[2]{};
+    bool    valid = false;
+
+    xface_pair() = default;
+    xface_pair(xface_pair const&) = delete;
+    xface_pair& operator=(xface_pair const&) = delete;
+    ~xface_pair()
+    {
+        for (double* p : xf)
+            if (p) (void)hipFree(p);
+    }
+
+    void allocate(uint64_t leaves, int face)
+    {
+        for (int k = 0; k < 2; ++k) xf[k] = dev_alloc<double>(leaves * 2 * static_cast<uint64_t>(face));
+    }
+    bool wanted(ops const* op, int v) const { return xf[0] && (op->variant_needs(v) & NEEDS_XFACES); }
+
+    // Extras of a step of variant v reading buffer `cur`: the side buffers
+    // of the state read (built first if stale) and of the state written.
+    step_aux prepare(ops const* op, int v, step_aux aux, const double* u_cur, int cur, uint64_t n, hipStream_t s)
+    {
+        if (!wanted(op, v)) return aux;
+        if (!valid)
+        {
+            op->xfaces(u_cur, xf[cur], static_cast<uint32_t>(n), s);
+            valid = true;
+        }
+        aux.xf_in  = xf[cur];
+        aux.xf_out = xf[1 - cur];
+        return aux;
+    }
+    // After that step: the new state's side buffers exist only if it wrote them.
+    void stepped(ops const* op, int v) { valid = wanted(op, v); }
+};
+
+// Times every step-kernel variant on trial steps that evolve a copy of the
+// state u_cur, ping-ponging between u_other and a temporary buffer, so that
+// every launch reads what the previous one wrote (as in a solve); u_cur is
+// not modified.  launch(v, src, dst, aux) queues one step.  Returns the
+// fastest variant.
+template <typename Launch>
+int autotune_steps(ops const* op, const double* u_cur, double* u_other, uint64_t leaves, int ni, int face,
+                   bool with_xf, step_aux const& runs, Launch&& launch, hipStream_t stream, uint32_t trials,
+                   float* best_us)
+{
+    double*    tmp = dev_alloc<double>(leaves * ni);
+    xface_pair txf;
+    int        best = 0;
+    try
+    {
+        if (with_xf) txf.allocate(leaves, face);
+        double* bufs[2] = { tmp, u_other };
+        int     p       = 0;
+        AMR_HIP_CHECK(hipMemcpyAsync(tmp, u_cur, leaves * ni * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        best = pick_fastest(
+            op->variant_count(),
+            [&](int v)
+            {
+                const step_aux aux = txf.prepare(op, v, runs, bufs[p], p, leaves, stream);
+                launch(v, bufs[p], bufs[1 - p], aux);
+                txf.stepped(op, v);
+                p = 1 - p;
+            },
+            stream, trials, best_us);
+    }
+    catch (...)
+    {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(tmp);
+        throw;
+    }
+    AMR_HIP_CHECK(hipStreamSynchronize(stream));
+    AMR_HIP_CHECK(hipFree(tmp));
+    return best;
+}
 
 // Batch packing: shift an instance's neighbour slots by its leaf offset, and
 // tag its leaves with the instance index.
@@ -458,6 +585,8 @@ struct amr_ctx
     bool            broken        = false;
     int             variant       = 0; // step-kernel variant (autotuner)
     run_lists       runs;              // +x runs of the current table
+    xface_pair      xfp;               // x-face side buffers of u[0] / u[1]
+    uint64_t        sweeps        = 0; // steps launched (sets the sweep direction)
 
     ~amr_ctx()
     {
@@ -537,25 +666,28 @@ struct amr_ctx
     void step(double dt)
     {
         ensure_usable();
-        const step_params prm = make_params(dt);
+        step_params prm = make_params(dt);
+        prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
+        const step_aux aux = xfp.prepare(op, variant, runs.view(), u[cur], cur, n, stream);
         op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
-                 runs.view(), stream);
+                 aux, stream);
+        xfp.stepped(op, variant);
         cur = 1 - cur;
     }
 
-    // Times every step-kernel variant on the current tree and keeps the
-    // fastest.  Trial launches read the current interiors and write only the
-    // other buffer, so the state is unchanged.
+    // Times every step-kernel variant on the current tree (trial steps on a
+    // copy of the state, see autotune_steps) and keeps the fastest.
     int autotune(uint32_t trials, float* best_us)
     {
         ensure_usable();
-        const step_params prm = make_params(compute_dt());
-        variant               = pick_fastest(
-            op->variant_count(),
-            [&](int v)
+        step_params prm = make_params(compute_dt());
+        variant         = autotune_steps(
+            op, u[cur], u[1 - cur], n, ni, ni / cfg.patch_size, xfp.xf[0] != nullptr, runs.view(),
+            [&](int v, const double* src, double* dst, step_aux const& aux)
             {
-                op->step(v, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr,
-                         static_cast<uint32_t>(n), runs.view(), stream);
+                prm.reverse ^= 1u;
+                op->step(v, src, dst, keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), aux,
+                         stream);
             },
             stream, trials, best_us);
         return variant;
@@ -605,8 +737,9 @@ struct amr_ctx
         if (s.n_refine == 0 && s.n_coarsen == 0) return false;
         if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
         op->emit(keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, stream);
-        cur = 1 - cur;
-        kc  = 1 - kc;
+        cur       = 1 - cur;
+        kc        = 1 - kc;
+        xfp.valid = false;
         n   = s.total;
         AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
         sync();
@@ -623,7 +756,7 @@ struct amr_batch
 {
     ops const*               op = nullptr;
     hipStream_t              stream{};
-    int                      ni = 0, ndir = 0;
+    int                      ni = 0, ndir = 0, face = 0;
     uint64_t                 total = 0;
     std::vector<amr_ctx*>    src;
     std::vector<uint64_t>    off;
@@ -631,12 +764,14 @@ struct amr_batch
     double*                  u[2]{};
     int                      cur     = 0;
     int                      variant = 0;
+    uint64_t                 sweeps  = 0;
     uint64_t*                keys    = nullptr;
     face_link*               links   = nullptr;
     uint32_t*                metas   = nullptr;
     uint16_t*                inst    = nullptr;
     step_params*             table   = nullptr;
     run_lists                runs;
+    xface_pair               xfp;
     struct graph_entry
     {
         uint64_t        steps;
@@ -657,8 +792,12 @@ struct amr_batch
 
     void launch(int v)
     {
-        op->step(v, u[cur], u[1 - cur], keys, links, metas, host_table[0], table, inst, static_cast<uint32_t>(total),
-                 runs.view(), stream);
+        const step_aux aux = xfp.prepare(op, v, runs.view(), u[cur], cur, total, stream);
+        step_params    prm = host_table[0];
+        prm.reverse        = static_cast<uint32_t>(sweeps++ & 1);
+        op->step(v, u[cur], u[1 - cur], keys, links, metas, prm, table, inst, static_cast<uint32_t>(total), aux,
+                 stream);
+        xfp.stepped(op, v);
         cur = 1 - cur;
     }
 
@@ -671,6 +810,12 @@ struct amr_batch
         {
             for (uint64_t s = 0; s < steps; ++s) launch(variant);
             return;
+        }
+        // side buffers are built outside the graph, which then keeps them current
+        if (xfp.wanted(op, variant) && !xfp.valid)
+        {
+            op->xfaces(u[cur], xfp.xf[cur], static_cast<uint32_t>(total), stream);
+            xfp.valid = true;
         }
         hipGraphExec_t exec = nullptr;
         for (auto const& g : graphs)
@@ -700,6 +845,7 @@ struct amr_batch
         }
         AMR_HIP_CHECK(hipGraphLaunch(exec, stream));
         cur = (steps % 2) ? 1 - start : start;
+        if (steps > 0) xfp.stepped(op, variant);
     }
 };
 
@@ -907,9 +1053,11 @@ struct amr_tape
 
     void forward_step(uint64_t k, const double* src, double* dst)
     {
-        topology const& t = topo(k);
-        op->step(variant, src, dst, t.keys, t.links, t.metas, step_consts(dts[k]), nullptr, nullptr,
-                 static_cast<uint32_t>(t.n), run_view{}, stream);
+        topology const& t   = topo(k);
+        step_params     prm = step_consts(dts[k]);
+        prm.reverse         = static_cast<uint32_t>(k & 1);
+        op->step(variant, src, dst, t.keys, t.links, t.metas, prm, nullptr, nullptr, static_cast<uint32_t>(t.n),
+                 step_aux{}, stream);
     }
 
     void forward(const double* u0, const double* v, double* uT)
@@ -1042,7 +1190,11 @@ int amr_tree_create(const amr_tree_config* cfg, amr_ctx** out)
             AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
             c->n         = 1;
             c->max_level = 0;
-            if (run_lists::wanted(op)) c->runs.allocate(cap);
+            if (run_lists::wanted(op))
+            {
+                c->runs.allocate(cap);
+                c->xfp.allocate(cap, c->ni / cfg->patch_size);
+            }
             c->build_table();
             *out = c.release();
         }
@@ -1166,6 +1318,7 @@ int amr_tree_set_interior(amr_ctx* ctx, const double* host, uint64_t count)
             if (count != ctx->n * ctx->ni) throw amr_error(AMR_ERR_ARGUMENT, "count must be size * n^rank");
             AMR_HIP_CHECK(hipMemcpyAsync(ctx->u[ctx->cur], host, count * sizeof(double), hipMemcpyHostToDevice,
                                          ctx->stream));
+            ctx->xfp.valid = false;
             ctx->sync();
         }
     );
@@ -1224,7 +1377,8 @@ int amr_tree_swap_buffers(amr_ctx* ctx)
         [&]
         {
             if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            ctx->cur = 1 - ctx->cur;
+            ctx->cur       = 1 - ctx->cur;
+            ctx->xfp.valid = false;
         }
     );
 }
@@ -1235,7 +1389,9 @@ int amr_tree_device_interior(amr_ctx* ctx, double** dev_ptr)
         [&]
         {
             if (!ctx || !dev_ptr) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            *dev_ptr = ctx->u[ctx->cur];
+            *dev_ptr       = ctx->u[ctx->cur];
+            ctx->xfp.valid = false; // the caller may write the interiors
+            ctx->sync();
         }
     );
 }
@@ -1437,6 +1593,8 @@ int amr_batch_create(amr_ctx* const* trees, uint32_t count, amr_batch** out)
             if (run_lists::wanted(b->op))
             {
                 b->runs.allocate(b->total);
+                b->face = b->ni / trees[0]->cfg.patch_size;
+                b->xfp.allocate(b->total, b->face);
                 b->runs.build(b->op->rank, b->keys, static_cast<uint32_t>(b->total), trees[0]->depth, b->links,
                               b->metas, b->stream);
             }
@@ -1495,6 +1653,7 @@ int amr_batch_scatter(amr_batch* b)
                 amr_ctx* t = b->src[i];
                 AMR_HIP_CHECK(hipMemcpyAsync(t->u[t->cur], b->u[b->cur] + b->off[i] * b->ni, t->n * b->ni * sizeof(double),
                                              hipMemcpyDeviceToDevice, b->stream));
+                t->xfp.valid = false;
             }
             AMR_HIP_CHECK(hipStreamSynchronize(b->stream));
         }
@@ -1530,12 +1689,15 @@ int amr_batch_autotune(amr_batch* b, uint32_t trial_steps, int32_t* chosen, floa
         [&]
         {
             if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            b->variant = pick_fastest(
-                b->op->variant_count(),
-                [&](int v)
+            step_params prm = b->host_table[0];
+            b->variant      = autotune_steps(
+                b->op, b->u[b->cur], b->u[1 - b->cur], b->total, b->ni, b->face, b->xfp.xf[0] != nullptr,
+                b->runs.view(),
+                [&](int v, const double* src, double* dst, step_aux const& aux)
                 {
-                    b->op->step(v, b->u[b->cur], b->u[1 - b->cur], b->keys, b->links, b->metas, b->host_table[0],
-                                b->table, b->inst, static_cast<uint32_t>(b->total), b->runs.view(), b->stream);
+                    prm.reverse ^= 1u;
+                    b->op->step(v, src, dst, b->keys, b->links, b->metas, prm, b->table, b->inst,
+                                static_cast<uint32_t>(b->total), aux, b->stream);
                 },
                 b->stream, trial_steps, best_us);
             if (chosen) *chosen = b->variant;

@@ -24,10 +24,22 @@ namespace amrhip
 // AdvectionPhysics.hpp:61-65).
 struct step_params
 {
-    double vel[3];
-    double hs[3];
-    double dtdx[kMaxLevels][3];
+    double   vel[3];
+    double   hs[3];
+    double   dtdx[kMaxLevels][3];
+    uint32_t reverse; // walk this launch's leaves backwards (sweep_item)
 };
+
+// Leaf (or run) of workgroup b in a launch over `count` items: XCD-contiguous
+// ranges (xcd_item), walked backwards when `reverse` is set.  Consecutive
+// steps alternate the direction, so that a step starts on the leaves the
+// previous one wrote last -- the lines most likely still held by the
+// memory-side cache.  Items >= count are skipped by the caller.
+__device__ inline uint32_t sweep_item(uint32_t b, uint32_t nblocks, uint32_t count, uint32_t reverse)
+{
+    const uint32_t i = xcd_item(b, nblocks);
+    return (reverse && i < count) ? count - 1 - i : i;
+}
 
 // Output store of a step kernel.  The non-temporal form streams the new
 // state past L2 so that L2 keeps the input lines that neighbouring leaves'
@@ -154,6 +166,189 @@ __host__ __device__ inline void stage_ghost(double* tile, const double* __restri
     c[dim]                          = (dir & 1) ? N : -1;
     const size_t e                  = static_cast<size_t>(leaf) * S::NDIR + dir;
     tile[tile_shape<R, N>::idx(c)] = ghost_value<R, N>(u, links[e], metas[e], dir, c);
+}
+
+// x-face side buffers: per leaf slot, the interior layers x = 0 (side 0) and
+// x = N-1 (side 1), N^(R-1) doubles each, row-major over the other layout
+// dims.  An x face of a row-major patch touches every row of it (every
+// 64-128 B line); the side buffer holds the same values contiguously, so a
+// kernel that gathers its x ghosts from there reads 2 x N^(R-1) doubles
+// instead of touching two whole neighbour patches.  A step kernel that reads
+// them writes those of the state it produces.
+template <int R, int N>
+__host__ __device__ inline int xface_index(const int* c)
+{
+    int f = 0;
+    for (int d = 0; d < R - 1; ++d) f = f * N + c[d];
+    return f;
+}
+
+// The x ghost of ghost_value read from the side buffer where the neighbour's
+// touching layer holds it (same level: copy; coarser: injection); a finer
+// neighbour's mean needs two layers and is read from the interiors.
+template <int R, int N>
+__host__ __device__ inline double xface_ghost(const double* __restrict__ u, const double* __restrict__ xf,
+                                              const face_link& L, uint32_t meta, int dir, const int* c)
+{
+    using S        = patch_shape<R, N>;
+    const int side = (dir & 1) ? 0 : 1; // -x reads the neighbour's x = N-1 layer
+    switch (rel_of(meta))
+    {
+        case REL_SAME:
+            return xf[(static_cast<size_t>(L.ids[0]) * 2 + side) * S::FACE + xface_index<R, N>(c)];
+        case REL_COARSER:
+        {
+            int cc[R];
+            for (int d = 0; d < R - 1; ++d) cc[d] = static_cast<int>(quad_of(meta, d)) * S::HALF + c[d] / 2;
+            return xf[(static_cast<size_t>(L.ids[0]) * 2 + side) * S::FACE + xface_index<R, N>(cc)];
+        }
+        default: return ghost_value<R, N>(u, L, meta, dir, c);
+    }
+}
+
+// Builds the x-face side buffers of a state from its interiors.
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_xfaces(const double* __restrict__ u, double* __restrict__ xf, uint32_t nleaves)
+{
+    using S             = patch_shape<R, N>;
+    const uint32_t leaf = blockIdx.x;
+    if (leaf >= nleaves) return;
+    for (int t = threadIdx.x; t < 2 * S::FACE; t += NT)
+    {
+        const int side = t / S::FACE;
+        int       c[R], f = t % S::FACE;
+        for (int d = R - 2; d >= 0; --d)
+        {
+            c[d] = f % N;
+            f /= N;
+        }
+        c[R - 1] = side ? N - 1 : 0;
+        xf[static_cast<size_t>(leaf) * 2 * S::FACE + t] = u[static_cast<size_t>(leaf) * S::NI + S::lin(c)];
+    }
+}
+
+// Index into u of the one interior cell that gives the ghost value of cell c
+// when the direction-`dir` neighbour (slot nb) is same-level (copy) or
+// coarser (injection): ghost_value's single-cell rules.
+template <int R, int N>
+__host__ __device__ inline size_t ghost_source(int32_t nb, uint32_t meta, int dir, const int* c)
+{
+    using S           = patch_shape<R, N>;
+    const int  dim    = dir >> 1;
+    const int  nrm    = (dir & 1) ? c[dim] - N : c[dim] + N;
+    const bool coarse = rel_of(meta) == REL_COARSER;
+    int        cc[R];
+    for (int d = 0; d < R; ++d)
+    {
+        const int v = (d == dim) ? nrm : c[d];
+        cc[d]       = coarse ? static_cast<int>(quad_of(meta, d)) * S::HALF + v / 2 : v;
+    }
+    return static_cast<size_t>(nb) * S::NI + S::lin(cc);
+}
+
+// The same cell in the x-face side buffer (dir = -x or +x).
+template <int R, int N>
+__host__ __device__ inline size_t xface_source(int32_t nb, uint32_t meta, int dir, const int* c)
+{
+    using S           = patch_shape<R, N>;
+    const int  side   = (dir & 1) ? 0 : 1;
+    const bool coarse = rel_of(meta) == REL_COARSER;
+    int        cc[R];
+    for (int d = 0; d < R - 1; ++d) cc[d] = coarse ? static_cast<int>(quad_of(meta, d)) * S::HALF + c[d] / 2 : c[d];
+    return (static_cast<size_t>(nb) * 2 + side) * S::FACE + xface_index<R, N>(cc);
+}
+
+// Whether none of a leaf's faces has finer neighbours (every ghost is then a
+// single cell and the gathers below issue all their loads back to back).
+template <int R>
+__device__ inline bool single_cell_ghosts(const uint32_t* m)
+{
+    uint32_t finer = 0;
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k) finer |= (rel_of(m[k]) == REL_FINER) ? 1u : 0u;
+    return finer == 0;
+}
+
+// The six ghost values of a lane of the 3D wave layout (lane = 8 hi + lo):
+// g[0], g[1] = -z / +z of the column (y, x) = (hi, lo); g[2], g[3] = the
+// (z, x) = (hi, lo) cell of the -y / +y face; g[4], g[5] = the (z, y) =
+// (hi, lo) cell of the -x / +x face (from the side buffer with XF).  The
+// leaf's table entries are read first and, unless a face has finer
+// neighbours, all six loads are in flight at once.
+template <bool XF>
+__device__ inline void gather_ghosts_3d8(const double* __restrict__ u, const double* __restrict__ xf,
+                                         const face_link* __restrict__ L, const uint32_t* __restrict__ M, int hi,
+                                         int lo, bool want_xm, bool want_xp, double* g)
+{
+    const int c[6][3] = { { -1, hi, lo }, { 8, hi, lo }, { hi, -1, lo }, { hi, 8, lo }, { hi, lo, -1 }, { hi, lo, 8 } };
+    uint32_t  m[6];
+    int32_t   nb[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+    {
+        m[k]  = M[k];
+        nb[k] = L[k].ids[0];
+    }
+    if (single_cell_ghosts<3>(m))
+    {
+        size_t src[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            src[k] = (XF && k >= 4) ? xface_source<3, 8>(nb[k], m[k], k, c[k]) : ghost_source<3, 8>(nb[k], m[k], k, c[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) g[k] = u[src[k]];
+        const double* xb = XF ? xf : u;
+        g[4]             = want_xm ? xb[src[4]] : 0.0;
+        g[5]             = want_xp ? xb[src[5]] : 0.0;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+    {
+        if ((k == 4 && !want_xm) || (k == 5 && !want_xp))
+            g[k] = 0.0;
+        else if (XF && k >= 4)
+            g[k] = xface_ghost<3, 8>(u, xf, L[k], M[k], k, c[k]);
+        else
+            g[k] = ghost_value<3, 8>(u, L[k], M[k], k, c[k]);
+    }
+}
+
+// The ghost value a lane of the 2D wave layout gathers: cell t of face q
+// (lane = 16 q + t), from the side buffer for the x faces with XF.
+template <bool XF>
+__device__ inline double gather_ghost_2d16(const double* __restrict__ u, const double* __restrict__ xf,
+                                           const face_link* __restrict__ L, const uint32_t* __restrict__ M, int q,
+                                           int t)
+{
+    int c[2];
+    c[0] = (q == 0) ? -1 : (q == 1) ? 16 : t;
+    c[1] = (q == 2) ? -1 : (q == 3) ? 16 : t;
+    uint32_t ms[4];
+    int32_t  nbs[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        ms[k]  = M[k];
+        nbs[k] = L[k].ids[0];
+    }
+    if (single_cell_ghosts<2>(ms))
+    {
+        // the lane's own face record, selected from the four uniform ones
+        uint32_t m  = ms[0];
+        int32_t  nb = nbs[0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            if (q == k)
+            {
+                m  = ms[k];
+                nb = nbs[k];
+            }
+        if (XF && q >= 2) return xf[xface_source<2, 16>(nb, m, q, c)];
+        return u[ghost_source<2, 16>(nb, m, q, c)];
+    }
+    if (XF && q >= 2) return xface_ghost<2, 16>(u, xf, L[q], M[q], q, c);
+    return ghost_value<2, 16>(u, L[q], M[q], q, c);
 }
 
 // Upwind update of one cell from its neighbours along each physical axis d
@@ -292,7 +487,7 @@ __global__ __launch_bounds__(NT) void k_step(const double* __restrict__ u, doubl
 {
     using S = patch_shape<R, N>;
     __shared__ double tile[tile_shape<R, N>::TS];
-    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
     if (leaf >= nleaves) return;
     stage_tile<R, N, NT>(tile, u, leaf, links, metas);
     __syncthreads();
@@ -314,39 +509,35 @@ __global__ __launch_bounds__(NT) void k_step(const double* __restrict__ u, doubl
 // l -+ 1 and l -+ 8, z-neighbours from the lane's own registers.  The six
 // ghost faces are gathered one face cell per lane; the -+z faces land in the
 // lane that needs them, the -+y / -+x faces go through 2 KiB of LDS.  With
-// no interior tile in LDS, 32 leaves are in flight per CU.
-template <bool NT>
+// no interior tile in LDS, 32 leaves are in flight per CU.  With XF the x
+// ghosts come from the x-face side buffers (xf_in) and the kernel writes
+// those of the new state (xf_out).
+template <bool NT, bool XF = false>
 __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, double* __restrict__ un,
                                                  const uint64_t* __restrict__ keys,
                                                  const face_link* __restrict__ links,
                                                  const uint32_t* __restrict__ metas, step_params prm,
                                                  const step_params* __restrict__ table,
-                                                 const uint16_t* __restrict__ inst, uint32_t nleaves)
+                                                 const uint16_t* __restrict__ inst, uint32_t nleaves,
+                                                 const double* __restrict__ xf_in = nullptr,
+                                                 double* __restrict__ xf_out = nullptr)
 {
     constexpr int NI = 512;
     __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
-    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    __shared__ double xs[XF ? 2 : 1][64];
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
     if (leaf >= nleaves) return;
     const int     lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
     const double* up   = u + static_cast<size_t>(leaf) * NI;
     double        r[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) r[z] = up[z * 64 + lane];
-    const face_link* L = links + static_cast<size_t>(leaf) * 6;
-    const uint32_t*  M = metas + static_cast<size_t>(leaf) * 6;
-    int              c[3];
-    c[0] = -1, c[1] = hi3, c[2] = lo3; // (z, y, x) of the lane's -z ghost
-    const double gzm = ghost_value<3, 8>(u, L[0], M[0], 0, c);
-    c[0]             = 8;
-    const double gzp = ghost_value<3, 8>(u, L[1], M[1], 1, c);
-    c[0] = hi3, c[1] = -1, c[2] = lo3; // y faces: face cell (z, x)
-    gface[0][lane] = ghost_value<3, 8>(u, L[2], M[2], 2, c);
-    c[1]           = 8;
-    gface[1][lane] = ghost_value<3, 8>(u, L[3], M[3], 3, c);
-    c[0] = hi3, c[1] = lo3, c[2] = -1; // x faces: face cell (z, y)
-    gface[2][lane] = ghost_value<3, 8>(u, L[4], M[4], 4, c);
-    c[2]           = 8;
-    gface[3][lane] = ghost_value<3, 8>(u, L[5], M[5], 5, c);
+    double g[6];
+    gather_ghosts_3d8<XF>(u, xf_in, links + static_cast<size_t>(leaf) * 6, metas + static_cast<size_t>(leaf) * 6, hi3,
+                          lo3, true, true, g);
+    const double gzm = g[0], gzp = g[1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gface[k][lane] = g[2 + k];
     __syncthreads();
     const step_params& P       = params_for(prm, table, inst, leaf);
     const int          lv      = key_level(keys[leaf]);
@@ -371,8 +562,20 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
         const double fzr = upwind_flux(uc, hz, P.vel[2], P.hs[2]);
         const double fL[3] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), upwind_flux(ly, uc, P.vel[1], P.hs[1]), fz };
         const double fR[3] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), upwind_flux(uc, hy, P.vel[1], P.hs[1]), fzr };
-        store_cell<NT>(out + z * 64 + lane, update_from_fluxes<3>(uc, fL, fR, dtdx));
+        const double o     = update_from_fluxes<3>(uc, fL, fR, dtdx);
+        store_cell<NT>(out + z * 64 + lane, o);
+        if constexpr (XF)
+        {
+            if (x == 0) xs[0][z * 8 + y] = o;
+            if (x == 7) xs[1][z * 8 + y] = o;
+        }
         fz = fzr;
+    }
+    if constexpr (XF)
+    {
+        __syncthreads();
+        xf_out[static_cast<size_t>(leaf) * 128 + lane]      = xs[0][lane];
+        xf_out[static_cast<size_t>(leaf) * 128 + 64 + lane] = xs[1][lane];
     }
 }
 
@@ -395,7 +598,7 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
 {
     constexpr int NI = 512;
     __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
-    const uint32_t r = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, nruns, prm.reverse);
     if (r >= nruns) return;
     const int      lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
     const int      y = hi3, x = lo3;
@@ -418,19 +621,13 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
         const bool       more = i + 1 < run.len;
         // ghosts of this leaf first, then the next leaf's interior, so that
         // the update below waits only for the ghosts
-        int c[3];
-        c[0] = -1, c[1] = hi3, c[2] = lo3;
-        const double gzm = ghost_value<3, 8>(u, L[0], M[0], 0, c);
-        c[0]             = 8;
-        const double gzp = ghost_value<3, 8>(u, L[1], M[1], 1, c);
-        c[0] = hi3, c[1] = -1, c[2] = lo3;
-        gface[0][lane] = ghost_value<3, 8>(u, L[2], M[2], 2, c);
-        c[1]           = 8;
-        gface[1][lane] = ghost_value<3, 8>(u, L[3], M[3], 3, c);
-        c[0] = hi3, c[1] = lo3, c[2] = -1;
-        if (i == 0) gface[2][lane] = ghost_value<3, 8>(u, L[4], M[4], 4, c);
-        c[2] = 8;
-        if (!more) gface[3][lane] = ghost_value<3, 8>(u, L[5], M[5], 5, c);
+        double g[6];
+        gather_ghosts_3d8<false>(u, nullptr, L, M, hi3, lo3, i == 0, !more, g);
+        const double gzm = g[0], gzp = g[1];
+        gface[0][lane]   = g[2];
+        gface[1][lane]   = g[3];
+        if (i == 0) gface[2][lane] = g[4];
+        if (!more) gface[3][lane] = g[5];
         const uint32_t next = more ? static_cast<uint32_t>(L[5].ids[0]) : leaf;
         if (more)
         {
@@ -613,31 +810,29 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // y = 4 (l / 16) .. 4 (l / 16) + 3; x-neighbours from lanes l -+ 1, the row
 // above / below a lane's four from lanes l -+ 16; the four ghost faces are
 // gathered one face cell per lane into 512 B of LDS.
-template <bool NT>
+template <bool NT, bool XF = false>
 __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, double* __restrict__ un,
                                                   const uint64_t* __restrict__ keys,
                                                   const face_link* __restrict__ links,
                                                   const uint32_t* __restrict__ metas, step_params prm,
                                                   const step_params* __restrict__ table,
-                                                  const uint16_t* __restrict__ inst, uint32_t nleaves)
+                                                  const uint16_t* __restrict__ inst, uint32_t nleaves,
+                                                  const double* __restrict__ xf_in = nullptr,
+                                                  double* __restrict__ xf_out = nullptr)
 {
     constexpr int NI = 256;
     __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
-    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    __shared__ double xs[XF ? 32 : 1];  // new x = 0 / x = 15 columns
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
     if (leaf >= nleaves) return;
     const int     lane = threadIdx.x, x = lane & 15, q = lane >> 4;
     const double* up   = u + static_cast<size_t>(leaf) * NI;
     double        r[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) r[j] = up[(4 * q + j) * 16 + x];
-    {
-        const int    f = q, t = x; // face f = direction f, face cell t
-        int          c[2];
-        const size_t e = static_cast<size_t>(leaf) * 4 + f;
-        c[0]           = (f == 0) ? -1 : (f == 1) ? 16 : t;
-        c[1]           = (f == 2) ? -1 : (f == 3) ? 16 : t;
-        gface[f][t]    = ghost_value<2, 16>(u, links[e], metas[e], f, c);
-    }
+    // lane (q, x) gathers cell x of face q
+    gface[q][x] = gather_ghost_2d16<XF>(u, xf_in, links + static_cast<size_t>(leaf) * 4,
+                                        metas + static_cast<size_t>(leaf) * 4, q, x);
     __syncthreads();
     const step_params& P       = params_for(prm, table, inst, leaf);
     const int          lv      = key_level(keys[leaf]);
@@ -660,8 +855,19 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
         const double fyr   = upwind_flux(uc, hy, P.vel[1], P.hs[1]);
         const double fL[2] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), fy };
         const double fR[2] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), fyr };
-        store_cell<NT>(out + y * 16 + x, update_from_fluxes<2>(uc, fL, fR, dtdx));
+        const double o     = update_from_fluxes<2>(uc, fL, fR, dtdx);
+        store_cell<NT>(out + y * 16 + x, o);
+        if constexpr (XF)
+        {
+            if (x == 0) xs[y] = o;
+            if (x == 15) xs[16 + y] = o;
+        }
         fy = fyr;
+    }
+    if constexpr (XF)
+    {
+        __syncthreads();
+        if (lane < 32) xf_out[static_cast<size_t>(leaf) * 32 + lane] = xs[lane];
     }
 }
 
@@ -680,7 +886,7 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
 {
     constexpr int NI = 256;
     __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
-    const uint32_t r = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, nruns, prm.reverse);
     if (r >= nruns) return;
     const int      lane = threadIdx.x, x = lane & 15, q = lane >> 4;
     const leaf_run run  = runs[r];
@@ -699,17 +905,9 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
     {
         const bool   more = i + 1 < run.len;
         const size_t e0   = static_cast<size_t>(leaf) * 4;
-        {
-            // lanes of quarter q gather face q (y faces always, the run's end x faces)
-            const int f = q, t = x;
-            if (f < 2 || (f == 2 && i == 0) || (f == 3 && !more))
-            {
-                int c[2];
-                c[0]        = (f == 0) ? -1 : (f == 1) ? 16 : t;
-                c[1]        = (f == 2) ? -1 : (f == 3) ? 16 : t;
-                gface[f][t] = ghost_value<2, 16>(u, links[e0 + f], metas[e0 + f], f, c);
-            }
-        }
+        // lanes of quarter q gather face q (y faces always, the run's end x faces)
+        if (q < 2 || (q == 2 && i == 0) || (q == 3 && !more))
+            gface[q][x] = gather_ghost_2d16<false>(u, nullptr, links + e0, metas + e0, q, x);
         const uint32_t next = more ? static_cast<uint32_t>(links[e0 + 3].ids[0]) : leaf;
         if (more)
         {

@@ -106,3 +106,25 @@ def test_batch_variants_and_autotune(amr):
             assert np.array_equal(t.get_interior().view(np.uint64), exp[i].view(np.uint64)), f"variant {v} instance {i}"
     best, us = bt.autotune(3)
     assert 0 <= best < nvar and us > 0
+
+
+def test_batch_side_buffer_variant_with_graphs_and_switches(amr):
+    say("case: batch wave-xf graph replays with variant switches")
+    built = build(amr, 4)
+    trees = [b[0] for b in built]
+    names = built[0][1].step_variants()
+    xf = names.index("wave-xf")
+    ref = amr.Batch(trees)
+    bt = amr.Batch(trees)
+    plan = [(xf, True, 4), (0, True, 3), (xf, True, 4), (xf, False, 1), (1, False, 2), (xf, True, 4), (xf, True, 3)]
+    for i, (v, graph, steps) in enumerate(plan):
+        bt.set_step_variant(v)
+        bt.advance(steps, use_graph=graph)
+        bt.synchronize()
+        ref.advance(steps, use_graph=False)
+        ref.synchronize()
+        ref.scatter()
+        exp = [t.get_interior().copy() for t in trees]
+        bt.scatter()
+        for k, t in enumerate(trees):
+            assert np.array_equal(t.get_interior().view(np.uint64), exp[k].view(np.uint64)), f"plan step {i} instance {k}"

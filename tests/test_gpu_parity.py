@@ -366,3 +366,56 @@ def test_step_variants_bit_identical(amr, rank, n, levels, mixed):
     s.advance_batch_async(3)
     s.finish_advance_batch()
     assert_fields(g.get_interior(), exp, "autotuned variant")
+
+
+@pytest.mark.parametrize("rank,n,start", [(2, 16, 3), (3, 8, 2)])
+def test_side_buffer_variant_across_regrids_and_calls(amr, rank, n, start):
+    """The x-face side-buffer variant keeps its buffers current across single
+    steps, batches of steps, regrids, set_interior and variant switches:
+    every state matches the oracle bit for bit."""
+    say(f"case: wave-xf across regrids rank={rank}")
+    depth = 7 if rank == 2 else 4
+    vel = (0.8, -0.45, 0.3)[:rank]
+    g, s, o = pair(amr, rank, n, 1, depth, 16384, vel, 0.35)
+    for _ in range(start):
+        g.refine_all()
+        o.refine_all()
+    names = s.step_variants()
+    xf = names.index("wave-xf")
+    s.set_step_variant(xf)
+    rng = np.random.default_rng(31 + rank)
+    u0 = rng.random(g.size() * n**rank)
+    g.set_interior(u0)
+    o.set_interior(u0)
+    o.halo_exchange()
+    for rd in range(4):
+        flags = random_flags(rng, g.keys(), rank, depth)
+        assert g.reconstruct_tree(flags) == o.reconstruct(flags)
+        o.halo_exchange()
+        assert_topology(g, o, f"round {rd}")
+        # single steps, then a batch of steps, then a switch away and back
+        for _ in range(2):
+            s.advance_batch_async(1)
+            s.finish_advance_batch()
+            o.advance_batch(1)
+        s.advance_batch_async(3)
+        s.finish_advance_batch()
+        o.advance_batch(3)
+        s.set_step_variant(0)
+        s.advance_batch_async(1)
+        s.finish_advance_batch()
+        o.advance_batch(1)
+        s.set_step_variant(xf)
+        s.advance_batch_async(2)
+        s.finish_advance_batch()
+        o.advance_batch(2)
+        assert_fields(g.get_interior(), o.get_interior(), f"round {rd}")
+    # an interior written from the host replaces the side buffers too
+    u1 = rng.random(g.size() * n**rank)
+    g.set_interior(u1)
+    o.set_interior(u1)
+    o.halo_exchange()
+    s.advance_batch_async(2)
+    s.finish_advance_batch()
+    o.advance_batch(2)
+    assert_fields(g.get_interior(), o.get_interior(), "after set_interior")
