@@ -68,6 +68,7 @@ struct step_aux
     uint32_t        count[2]{};
     const double*   xf_in  = nullptr;
     double*         xf_out = nullptr;
+    int             uniform_level = -1; // >= 0: the tree is uniform at this level
 };
 
 // What a step-kernel variant needs besides the table.
@@ -110,14 +111,14 @@ struct instance
     // dynamic LDS).  Variant 0 is the default until the autotuner picks.
     static constexpr char const* kVariants3d8[]  = { "wave",          "wave-nt",     "group8",  "group8-nt",
                                                      "group8-lds56k", "wave-lds10k", "tile256", "run4",
-                                                     "run8",          "run8-nt",     "wave-xf" };
+                                                     "run8",          "run8-nt",     "wave-xf",  "wave-uniform" };
     static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256",
-                                                     "run4", "run8",    "run8-nt",     "wave-xf" };
+                                                     "run4", "run8",    "run8-nt",     "wave-xf", "wave-uniform" };
     static constexpr char const* kVariantsAny[]  = { "tile" };
     static int variant_count()
     {
-        if constexpr (R == 3 && N == 8) return 11;
-        if constexpr (R == 2 && N == 16) return 8;
+        if constexpr (R == 3 && N == 8) return 12;
+        if constexpr (R == 2 && N == 16) return 9;
         return 1;
     }
     static int variant_needs(int v)
@@ -163,6 +164,17 @@ struct instance
         }
         if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
         {
+            if (variant == ((R == 3) ? 11 : 8))
+            {
+                // wave kernel with neighbour slots computed from the leaf index when
+                // the whole tree is uniform (no table reads); the table otherwise
+                if constexpr (R == 3)
+                    hipLaunchKernelGGL((k_step_3d8<false, false>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, nullptr, nullptr, aux.uniform_level);
+                else
+                    hipLaunchKernelGGL((k_step_2d16<false, false>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, nullptr, nullptr, aux.uniform_level);
+                check_launch("k_step (uniform)");
+                return;
+            }
             if (variant_needs(variant) & NEEDS_XFACES)
             {
                 if (aux.xf_in && aux.xf_out)
@@ -650,6 +662,19 @@ struct amr_ctx
         if (broken) throw amr_error(AMR_ERR_TOPOLOGY, "context unusable after a topology error");
     }
 
+    // Level of a uniform tree (all 2^(rank L) leaves at level L), else -1.
+    int uniform_level() const
+    {
+        return (rank * max_level < 32 && n == (1ull << (rank * max_level))) ? max_level : -1;
+    }
+
+    step_aux aux_base() const
+    {
+        step_aux a      = runs.view();
+        a.uniform_level = uniform_level();
+        return a;
+    }
+
     step_params make_params(double dt) const
     {
         step_params prm{};
@@ -668,7 +693,7 @@ struct amr_ctx
         ensure_usable();
         step_params prm = make_params(dt);
         prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
-        const step_aux aux = xfp.prepare(op, variant, runs.view(), u[cur], cur, n, stream);
+        const step_aux aux = xfp.prepare(op, variant, aux_base(), u[cur], cur, n, stream);
         op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
                  aux, stream);
         xfp.stepped(op, variant);
@@ -682,7 +707,7 @@ struct amr_ctx
         ensure_usable();
         step_params prm = make_params(compute_dt());
         variant         = autotune_steps(
-            op, u[cur], u[1 - cur], n, ni, ni / cfg.patch_size, xfp.xf[0] != nullptr, runs.view(),
+            op, u[cur], u[1 - cur], n, ni, ni / cfg.patch_size, xfp.xf[0] != nullptr, aux_base(),
             [&](int v, const double* src, double* dst, step_aux const& aux)
             {
                 prm.reverse ^= 1u;

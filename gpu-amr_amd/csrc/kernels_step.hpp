@@ -258,6 +258,42 @@ __host__ __device__ inline size_t xface_source(int32_t nb, uint32_t meta, int di
     return (static_cast<size_t>(nb) * 2 + side) * S::FACE + xface_index<R, N>(cc);
 }
 
+// A leaf's face records: relation/quadrant word and first neighbour slot per
+// direction, read with one batch of scalar loads.
+template <int R>
+__device__ inline void load_faces(const face_link* __restrict__ L, const uint32_t* __restrict__ M, uint32_t* m,
+                                  int32_t* nb)
+{
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k)
+    {
+        m[k]  = M[k];
+        nb[k] = L[k].ids[0];
+    }
+}
+
+// The face records of leaf i of a uniform periodic tree whose leaves are all
+// at level ul (so the leaves are the 2^(R ul) Morton codes in order and slot
+// = Morton code): every neighbour is same-level, its slot is the Morton code
+// of the neighbouring coordinates.  No table is read.
+template <int R>
+__host__ __device__ inline void uniform_faces(uint32_t i, int ul, uint32_t* m, int32_t* nb)
+{
+    const uint32_t mask = (1u << ul) - 1u;
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k)
+    {
+        const int a = R - 1 - (k >> 1); // physical axis of layout dim k / 2 (Morton bit a of each group)
+        uint32_t  c = 0;
+        for (int b = 0; b < ul; ++b) c |= ((i >> (R * b + a)) & 1u) << b;
+        c          = (c + ((k & 1) ? 1u : mask)) & mask;
+        uint32_t j = i;
+        for (int b = 0; b < ul; ++b) j = (j & ~(1u << (R * b + a))) | (((c >> b) & 1u) << (R * b + a));
+        m[k]  = REL_SAME;
+        nb[k] = static_cast<int32_t>(j);
+    }
+}
+
 // Whether none of a leaf's faces has finer neighbours (every ghost is then a
 // single cell and the gathers below issue all their loads back to back).
 template <int R>
@@ -269,7 +305,8 @@ __device__ inline bool single_cell_ghosts(const uint32_t* m)
     return finer == 0;
 }
 
-// The six ghost values of a lane of the 3D wave layout (lane = 8 hi + lo):
+// The six ghost values of a lane of the 3D wave layout (lane = 8 hi + lo),
+// from the leaf's face records (m, nb; L, M for the finer-neighbour path):
 // g[0], g[1] = -z / +z of the column (y, x) = (hi, lo); g[2], g[3] = the
 // (z, x) = (hi, lo) cell of the -y / +y face; g[4], g[5] = the (z, y) =
 // (hi, lo) cell of the -x / +x face (from the side buffer with XF).  The
@@ -277,18 +314,11 @@ __device__ inline bool single_cell_ghosts(const uint32_t* m)
 // neighbours, all six loads are in flight at once.
 template <bool XF>
 __device__ inline void gather_ghosts_3d8(const double* __restrict__ u, const double* __restrict__ xf,
-                                         const face_link* __restrict__ L, const uint32_t* __restrict__ M, int hi,
-                                         int lo, bool want_xm, bool want_xp, double* g)
+                                         const uint32_t* m, const int32_t* nb, const face_link* __restrict__ L,
+                                         const uint32_t* __restrict__ M, int hi, int lo, bool want_xm, bool want_xp,
+                                         double* g)
 {
     const int c[6][3] = { { -1, hi, lo }, { 8, hi, lo }, { hi, -1, lo }, { hi, 8, lo }, { hi, lo, -1 }, { hi, lo, 8 } };
-    uint32_t  m[6];
-    int32_t   nb[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-    {
-        m[k]  = M[k];
-        nb[k] = L[k].ids[0];
-    }
     if (single_cell_ghosts<3>(m))
     {
         size_t src[6];
@@ -318,20 +348,12 @@ __device__ inline void gather_ghosts_3d8(const double* __restrict__ u, const dou
 // (lane = 16 q + t), from the side buffer for the x faces with XF.
 template <bool XF>
 __device__ inline double gather_ghost_2d16(const double* __restrict__ u, const double* __restrict__ xf,
-                                           const face_link* __restrict__ L, const uint32_t* __restrict__ M, int q,
-                                           int t)
+                                           const uint32_t* ms, const int32_t* nbs, const face_link* __restrict__ L,
+                                           const uint32_t* __restrict__ M, int q, int t)
 {
     int c[2];
     c[0] = (q == 0) ? -1 : (q == 1) ? 16 : t;
     c[1] = (q == 2) ? -1 : (q == 3) ? 16 : t;
-    uint32_t ms[4];
-    int32_t  nbs[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-    {
-        ms[k]  = M[k];
-        nbs[k] = L[k].ids[0];
-    }
     if (single_cell_ghosts<2>(ms))
     {
         // the lane's own face record, selected from the four uniform ones
@@ -520,7 +542,7 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
                                                  const step_params* __restrict__ table,
                                                  const uint16_t* __restrict__ inst, uint32_t nleaves,
                                                  const double* __restrict__ xf_in = nullptr,
-                                                 double* __restrict__ xf_out = nullptr)
+                                                 double* __restrict__ xf_out = nullptr, int ulevel = -1)
 {
     constexpr int NI = 512;
     __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
@@ -532,15 +554,22 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
     double        r[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) r[z] = up[z * 64 + lane];
+    const face_link* L = links + static_cast<size_t>(leaf) * 6;
+    const uint32_t*  M = metas + static_cast<size_t>(leaf) * 6;
+    uint32_t         m[6];
+    int32_t          nb[6];
+    if (ulevel >= 0)
+        uniform_faces<3>(leaf, ulevel, m, nb);
+    else
+        load_faces<3>(L, M, m, nb);
     double g[6];
-    gather_ghosts_3d8<XF>(u, xf_in, links + static_cast<size_t>(leaf) * 6, metas + static_cast<size_t>(leaf) * 6, hi3,
-                          lo3, true, true, g);
+    gather_ghosts_3d8<XF>(u, xf_in, m, nb, L, M, hi3, lo3, true, true, g);
     const double gzm = g[0], gzp = g[1];
 #pragma unroll
     for (int k = 0; k < 4; ++k) gface[k][lane] = g[2 + k];
     __syncthreads();
     const step_params& P       = params_for(prm, table, inst, leaf);
-    const int          lv      = key_level(keys[leaf]);
+    const int          lv      = ulevel >= 0 ? ulevel : key_level(keys[leaf]);
     const double       dtdx[3] = { P.dtdx[lv][0], P.dtdx[lv][1], P.dtdx[lv][2] };
     const int    y = hi3, x = lo3;
     double*      out = un + static_cast<size_t>(leaf) * NI;
@@ -621,8 +650,11 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
         const bool       more = i + 1 < run.len;
         // ghosts of this leaf first, then the next leaf's interior, so that
         // the update below waits only for the ghosts
+        uint32_t m[6];
+        int32_t  nb[6];
+        load_faces<3>(L, M, m, nb);
         double g[6];
-        gather_ghosts_3d8<false>(u, nullptr, L, M, hi3, lo3, i == 0, !more, g);
+        gather_ghosts_3d8<false>(u, nullptr, m, nb, L, M, hi3, lo3, i == 0, !more, g);
         const double gzm = g[0], gzp = g[1];
         gface[0][lane]   = g[2];
         gface[1][lane]   = g[3];
@@ -818,7 +850,7 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
                                                   const step_params* __restrict__ table,
                                                   const uint16_t* __restrict__ inst, uint32_t nleaves,
                                                   const double* __restrict__ xf_in = nullptr,
-                                                  double* __restrict__ xf_out = nullptr)
+                                                  double* __restrict__ xf_out = nullptr, int ulevel = -1)
 {
     constexpr int NI = 256;
     __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
@@ -830,12 +862,19 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
     double        r[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) r[j] = up[(4 * q + j) * 16 + x];
+    const face_link* L = links + static_cast<size_t>(leaf) * 4;
+    const uint32_t*  M = metas + static_cast<size_t>(leaf) * 4;
+    uint32_t         m[4];
+    int32_t          nb[4];
+    if (ulevel >= 0)
+        uniform_faces<2>(leaf, ulevel, m, nb);
+    else
+        load_faces<2>(L, M, m, nb);
     // lane (q, x) gathers cell x of face q
-    gface[q][x] = gather_ghost_2d16<XF>(u, xf_in, links + static_cast<size_t>(leaf) * 4,
-                                        metas + static_cast<size_t>(leaf) * 4, q, x);
+    gface[q][x] = gather_ghost_2d16<XF>(u, xf_in, m, nb, L, M, q, x);
     __syncthreads();
     const step_params& P       = params_for(prm, table, inst, leaf);
-    const int          lv      = key_level(keys[leaf]);
+    const int          lv      = ulevel >= 0 ? ulevel : key_level(keys[leaf]);
     const double       dtdx[2] = { P.dtdx[lv][0], P.dtdx[lv][1] };
     const double above   = __shfl_up(r[3], 16, 64);  // row 4q - 1
     const double below   = __shfl_down(r[0], 16, 64); // row 4q + 4
@@ -906,8 +945,11 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
         const bool   more = i + 1 < run.len;
         const size_t e0   = static_cast<size_t>(leaf) * 4;
         // lanes of quarter q gather face q (y faces always, the run's end x faces)
+        uint32_t m[4];
+        int32_t  nb[4];
+        load_faces<2>(links + e0, metas + e0, m, nb);
         if (q < 2 || (q == 2 && i == 0) || (q == 3 && !more))
-            gface[q][x] = gather_ghost_2d16<false>(u, nullptr, links + e0, metas + e0, q, x);
+            gface[q][x] = gather_ghost_2d16<false>(u, nullptr, m, nb, links + e0, metas + e0, q, x);
         const uint32_t next = more ? static_cast<uint32_t>(links[e0 + 3].ids[0]) : leaf;
         if (more)
         {
