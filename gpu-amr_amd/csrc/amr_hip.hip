@@ -111,14 +111,16 @@ struct instance
     // dynamic LDS).  Variant 0 is the default until the autotuner picks.
     static constexpr char const* kVariants3d8[]  = { "wave",          "wave-nt",     "group8",  "group8-nt",
                                                      "group8-lds56k", "wave-lds10k", "tile256", "run4",
-                                                     "run8",          "run8-nt",     "wave-xf",  "wave-uniform" };
-    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256",
-                                                     "run4", "run8",    "run8-nt",     "wave-xf", "wave-uniform" };
+                                                     "run8",          "run8-nt",     "wave-xf",  "wave-uniform",
+                                                     "wave-pf2",      "wave-pf4" };
+    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256",      "run4",
+                                                     "run8", "run8-nt", "wave-xf",     "wave-uniform", "wave-pf2",
+                                                     "wave-pf4" };
     static constexpr char const* kVariantsAny[]  = { "tile" };
     static int variant_count()
     {
-        if constexpr (R == 3 && N == 8) return 12;
-        if constexpr (R == 2 && N == 16) return 9;
+        if constexpr (R == 3 && N == 8) return 14;
+        if constexpr (R == 2 && N == 16) return 11;
         return 1;
     }
     static int variant_needs(int v)
@@ -164,6 +166,29 @@ struct instance
         }
         if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
         {
+            const int pf0 = (R == 3) ? 12 : 9;
+            if (variant == pf0 || variant == pf0 + 1)
+            {
+                // K consecutive leaves per wave, next leaf's records prefetched
+                const uint32_t k = (variant == pf0) ? 2 : 4;
+                const dim3     per_wave(xcd_grid((n + k - 1) / k));
+                if constexpr (R == 3)
+                {
+                    if (k == 2)
+                        hipLaunchKernelGGL(k_step_3d8_pf<2>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+                    else
+                        hipLaunchKernelGGL(k_step_3d8_pf<4>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+                }
+                else
+                {
+                    if (k == 2)
+                        hipLaunchKernelGGL(k_step_2d16_pf<2>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+                    else
+                        hipLaunchKernelGGL(k_step_2d16_pf<4>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+                }
+                check_launch("k_step (prefetch)");
+                return;
+            }
             if (variant == ((R == 3) ? 11 : 8))
             {
                 // wave kernel with neighbour slots computed from the leaf index when

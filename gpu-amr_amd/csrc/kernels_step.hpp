@@ -294,6 +294,32 @@ __host__ __device__ inline void uniform_faces(uint32_t i, int ul, uint32_t* m, i
     }
 }
 
+// A leaf's face records and level fetched with one vector load per lane
+// (lane k < 2R: relation word k; 2R <= k < 4R: first slot of face k - 2R;
+// lane 4R: low word of the key), so that the load counts in the in-order
+// vector-memory counter: a record fetched behind the current leaf's loads
+// does not hold up the current leaf.  unpack_faces reads them back per lane.
+template <int R>
+__device__ inline uint32_t fetch_faces(const face_link* __restrict__ L, const uint32_t* __restrict__ M,
+                                       const uint64_t* __restrict__ key, int lane)
+{
+    if (lane < 2 * R) return M[lane];
+    if (lane < 4 * R) return static_cast<uint32_t>(L[lane - 2 * R].ids[0]);
+    return static_cast<uint32_t>(*key);
+}
+
+template <int R>
+__device__ inline int unpack_faces(uint32_t rec, uint32_t* m, int32_t* nb)
+{
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k)
+    {
+        m[k]  = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rec), k));
+        nb[k] = __builtin_amdgcn_readlane(static_cast<int>(rec), 2 * R + k);
+    }
+    return __builtin_amdgcn_readlane(static_cast<int>(rec), 4 * R) & 0x3F;
+}
+
 // Whether none of a leaf's faces has finer neighbours (every ghost is then a
 // single cell and the gathers below issue all their loads back to back).
 template <int R>
@@ -608,6 +634,76 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
     }
 }
 
+// 3D 8^3 patches, one wave per K consecutive leaves, k_step_3d8's lane
+// layout.  The table records of the next leaf are fetched behind the current
+// leaf's interior and ghost loads, so after the first leaf a leaf's ghost
+// loads are issued together with its interior loads instead of one table
+// round trip later.
+template <int K>
+__global__ __launch_bounds__(64) void k_step_3d8_pf(const double* __restrict__ u, double* __restrict__ un,
+                                                    const uint64_t* __restrict__ keys,
+                                                    const face_link* __restrict__ links,
+                                                    const uint32_t* __restrict__ metas, step_params prm,
+                                                    const step_params* __restrict__ table,
+                                                    const uint16_t* __restrict__ inst, uint32_t nleaves)
+{
+    constexpr int NI = 512;
+    __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
+    const uint32_t nw = (nleaves + K - 1) / K;
+    const uint32_t w  = sweep_item(blockIdx.x, gridDim.x, nw, prm.reverse);
+    if (w >= nw) return;
+    const int      lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
+    const int      y = hi3, x = lo3;
+    const uint32_t first = w * K, end = (first + K < nleaves) ? first + K : nleaves;
+    uint32_t       rec   = fetch_faces<3>(links + static_cast<size_t>(first) * 6, metas + static_cast<size_t>(first) * 6,
+                                          keys + first, lane);
+#pragma unroll 1
+    for (uint32_t leaf = first; leaf < end; ++leaf)
+    {
+        uint32_t  m[6];
+        int32_t   nb[6];
+        const int lv = unpack_faces<3>(rec, m, nb);
+        const double* up = u + static_cast<size_t>(leaf) * NI;
+        double        r[8];
+#pragma unroll
+        for (int z = 0; z < 8; ++z) r[z] = up[z * 64 + lane];
+        double g[6];
+        gather_ghosts_3d8<false>(u, nullptr, m, nb, links + static_cast<size_t>(leaf) * 6,
+                                 metas + static_cast<size_t>(leaf) * 6, hi3, lo3, true, true, g);
+        if (leaf + 1 < end)
+            rec = fetch_faces<3>(links + static_cast<size_t>(leaf + 1) * 6, metas + static_cast<size_t>(leaf + 1) * 6,
+                                 keys + leaf + 1, lane);
+        const double gzm = g[0], gzp = g[1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gface[k][lane] = g[2 + k];
+        __syncthreads();
+        const step_params& P       = params_for(prm, table, inst, leaf);
+        const double       dtdx[3] = { P.dtdx[lv][0], P.dtdx[lv][1], P.dtdx[lv][2] };
+        double*            out     = un + static_cast<size_t>(leaf) * NI;
+        double             fz      = upwind_flux(gzm, r[0], P.vel[2], P.hs[2]);
+#pragma unroll
+        for (int z = 0; z < 8; ++z)
+        {
+            const double uc = r[z];
+            const double xl = __shfl_up(uc, 1, 64);
+            const double xr = __shfl_down(uc, 1, 64);
+            const double yl = __shfl_up(uc, 8, 64);
+            const double yr = __shfl_down(uc, 8, 64);
+            const double lx = (x == 0) ? gface[2][z * 8 + y] : xl;
+            const double hx = (x == 7) ? gface[3][z * 8 + y] : xr;
+            const double ly = (y == 0) ? gface[0][z * 8 + x] : yl;
+            const double hy = (y == 7) ? gface[1][z * 8 + x] : yr;
+            const double hz = (z == 7) ? gzp : r[z < 7 ? z + 1 : 7];
+            const double fzr = upwind_flux(uc, hz, P.vel[2], P.hs[2]);
+            const double fL[3] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), upwind_flux(ly, uc, P.vel[1], P.hs[1]), fz };
+            const double fR[3] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), upwind_flux(uc, hy, P.vel[1], P.hs[1]), fzr };
+            out[z * 64 + lane] = update_from_fluxes<3>(uc, fL, fR, dtdx);
+            fz                 = fzr;
+        }
+        __syncthreads(); // gface is rewritten by the next leaf
+    }
+}
+
 // 3D 8^3 patches, one wave per run of same-level leaves along +x (leaf_run).
 // The lane layout is k_step_3d8's.  Walking the run front to back, the -x
 // ghosts of a leaf are the previous leaf's x = 7 column (left in LDS) and
@@ -907,6 +1003,69 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
     {
         __syncthreads();
         if (lane < 32) xf_out[static_cast<size_t>(leaf) * 32 + lane] = xs[lane];
+    }
+}
+
+// 2D 16^2 patches, one wave per K consecutive leaves, k_step_2d16's lane
+// layout, the next leaf's table records fetched behind the current leaf's
+// loads (as k_step_3d8_pf).
+template <int K>
+__global__ __launch_bounds__(64) void k_step_2d16_pf(const double* __restrict__ u, double* __restrict__ un,
+                                                     const uint64_t* __restrict__ keys,
+                                                     const face_link* __restrict__ links,
+                                                     const uint32_t* __restrict__ metas, step_params prm,
+                                                     const step_params* __restrict__ table,
+                                                     const uint16_t* __restrict__ inst, uint32_t nleaves)
+{
+    constexpr int NI = 256;
+    __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
+    const uint32_t nw = (nleaves + K - 1) / K;
+    const uint32_t w  = sweep_item(blockIdx.x, gridDim.x, nw, prm.reverse);
+    if (w >= nw) return;
+    const int      lane = threadIdx.x, x = lane & 15, q = lane >> 4;
+    const uint32_t first = w * K, end = (first + K < nleaves) ? first + K : nleaves;
+    uint32_t       rec   = fetch_faces<2>(links + static_cast<size_t>(first) * 4, metas + static_cast<size_t>(first) * 4,
+                                          keys + first, lane);
+#pragma unroll 1
+    for (uint32_t leaf = first; leaf < end; ++leaf)
+    {
+        uint32_t  m[4];
+        int32_t   nb[4];
+        const int lv = unpack_faces<2>(rec, m, nb);
+        const double* up = u + static_cast<size_t>(leaf) * NI;
+        double        r[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = up[(4 * q + j) * 16 + x];
+        const double gq = gather_ghost_2d16<false>(u, nullptr, m, nb, links + static_cast<size_t>(leaf) * 4,
+                                                   metas + static_cast<size_t>(leaf) * 4, q, x);
+        if (leaf + 1 < end)
+            rec = fetch_faces<2>(links + static_cast<size_t>(leaf + 1) * 4, metas + static_cast<size_t>(leaf + 1) * 4,
+                                 keys + leaf + 1, lane);
+        gface[q][x] = gq;
+        __syncthreads();
+        const step_params& P       = params_for(prm, table, inst, leaf);
+        const double       dtdx[2] = { P.dtdx[lv][0], P.dtdx[lv][1] };
+        const double       above   = __shfl_up(r[3], 16, 64);  // row 4q - 1
+        const double       below   = __shfl_down(r[0], 16, 64); // row 4q + 4
+        double*            out     = un + static_cast<size_t>(leaf) * NI;
+        double             fy      = upwind_flux((q == 0) ? gface[0][x] : above, r[0], P.vel[1], P.hs[1]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int    y  = 4 * q + j;
+            const double uc = r[j];
+            const double xl = __shfl_up(uc, 1, 64);
+            const double xr = __shfl_down(uc, 1, 64);
+            const double lx = (x == 0) ? gface[2][y] : xl;
+            const double hx = (x == 15) ? gface[3][y] : xr;
+            const double hy = (j < 3) ? r[j < 3 ? j + 1 : 3] : ((q == 3) ? gface[1][x] : below);
+            const double fyr   = upwind_flux(uc, hy, P.vel[1], P.hs[1]);
+            const double fL[2] = { upwind_flux(lx, uc, P.vel[0], P.hs[0]), fy };
+            const double fR[2] = { upwind_flux(uc, hx, P.vel[0], P.hs[0]), fyr };
+            out[y * 16 + x]    = update_from_fluxes<2>(uc, fL, fR, dtdx);
+            fy                 = fyr;
+        }
+        __syncthreads();
     }
 }
 
