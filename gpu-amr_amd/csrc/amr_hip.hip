@@ -58,24 +58,20 @@ constexpr int block_threads(int cells)
 }
 
 // Per-launch extras of the step kernels: the +x run lists of the table
-// (runs of at most kRunK[k] leaves) and the x-face side buffers (xf_in: of
-// the state read, xf_out: of the state written).  A variant whose extras are
-// missing falls back to the wave-per-leaf kernel.
-constexpr uint32_t kRunK[2] = { 4, 8 };
+// (runs of at most kRunK[k] leaves).  A run variant without its list falls
+// back to the wave-per-leaf kernel.
+constexpr int      kRunLists        = 2;
+constexpr uint32_t kRunK[kRunLists] = { 4, 8 };
 struct step_aux
 {
-    const leaf_run* list[2]{};
-    uint32_t        count[2]{};
-    const double*   xf_in  = nullptr;
-    double*         xf_out = nullptr;
-    int             uniform_level = -1; // >= 0: the tree is uniform at this level
+    const leaf_run* list[kRunLists]{};
+    uint32_t        count[kRunLists]{};
 };
 
 // What a step-kernel variant needs besides the table.
 enum : int
 {
-    NEEDS_RUNS   = 1,
-    NEEDS_XFACES = 2,
+    NEEDS_RUNS = 1,
 };
 
 // Kernel table of one (rank, patch size, halo width) instance.
@@ -87,7 +83,6 @@ struct ops
     int (*variant_count)();
     char const* (*variant_name)(int);
     int (*variant_needs)(int);
-    void (*xfaces)(const double*, double*, uint32_t, hipStream_t);
     void (*padded)(const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
     void (*criterion)(int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double, double,
                       int, int, int8_t*, uint32_t, hipStream_t);
@@ -106,160 +101,65 @@ struct instance
     static constexpr int P   = N + 2 * H;
     static constexpr int NTP = block_threads((R == 2) ? P * P : P * P * P);
 
-    // Step-kernel variants (all bit-identical; they differ in how leaves map
-    // to workgroups and in how many workgroups share a CU, set through extra
-    // dynamic LDS).  Variant 0 is the default until the autotuner picks.
-    static constexpr char const* kVariants3d8[]  = { "wave",          "wave-nt",     "group8",  "group8-nt",
-                                                     "group8-lds56k", "wave-lds10k", "tile256", "run4",
-                                                     "run8",          "run8-nt",     "wave-xf",  "wave-uniform",
-                                                     "wave-pf2",      "wave-pf4" };
-    static constexpr char const* kVariants2d16[] = { "wave", "wave-nt", "wave-lds10k", "tile256",      "run4",
-                                                     "run8", "run8-nt", "wave-xf",     "wave-uniform", "wave-pf2",
-                                                     "wave-pf4" };
+    // Step-kernel variants (all bit-identical).  wave: one wave per leaf (the
+    // default); -nt: non-temporal stores; -lds10k: 10 KiB of extra dynamic
+    // LDS per workgroup, i.e. fewer leaves in flight per CU; tile256: LDS tile
+    // per leaf (the generic kernel); run4 / run8: one wave walks a run of up
+    // to 4 / 8 same-level leaves along +x.  The autotuner picks one per tree.
+    static constexpr char const* kVariantsWave[] = { "wave", "wave-nt", "wave-lds10k", "tile256", "run4", "run8" };
     static constexpr char const* kVariantsAny[]  = { "tile" };
-    static int variant_count()
+    static constexpr bool        kWave           = (R == 3 && N == 8) || (R == 2 && N == 16);
+    static int                   variant_count() { return kWave ? 6 : 1; }
+    static char const*           variant_name(int v) { return kWave ? kVariantsWave[v] : kVariantsAny[v]; }
+    static int                   variant_needs(int v) { return (kWave && v >= 4) ? NEEDS_RUNS : 0; }
+
+    template <bool NTS>
+    static void launch_wave(dim3 grid, unsigned lds, const double* u, double* un, const uint64_t* keys,
+                            const face_link* links, const uint32_t* metas, step_params const& prm,
+                            const step_params* table, const uint16_t* inst, uint32_t n, hipStream_t s)
     {
-        if constexpr (R == 3 && N == 8) return 14;
-        if constexpr (R == 2 && N == 16) return 11;
-        return 1;
+        if constexpr (R == 3 && N == 8)
+            hipLaunchKernelGGL(k_step_3d8<NTS>, grid, dim3(64), lds, s, u, un, keys, links, metas, prm, table, inst, n);
+        else if constexpr (R == 2 && N == 16)
+            hipLaunchKernelGGL(k_step_2d16<NTS>, grid, dim3(64), lds, s, u, un, keys, links, metas, prm, table, inst, n);
     }
-    static int variant_needs(int v)
-    {
-        if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
-        {
-            const int run0 = (R == 3) ? 7 : 4;
-            if (v >= run0 && v < run0 + 3) return NEEDS_RUNS;
-            if (v == run0 + 3) return NEEDS_XFACES;
-        }
-        return 0;
-    }
-    static void xfaces(const double* u, double* xf, uint32_t n, hipStream_t s)
-    {
-        if (n == 0) return;
-        hipLaunchKernelGGL((k_xfaces<R, N, 64>), dim3(n), dim3(64), 0, s, u, xf, n);
-        check_launch("k_xfaces");
-    }
-    static char const* variant_name(int v)
-    {
-        if constexpr (R == 3 && N == 8) return kVariants3d8[v];
-        if constexpr (R == 2 && N == 16) return kVariants2d16[v];
-        return kVariantsAny[v];
-    }
+
     static void step(int variant, const double* u, double* un, const uint64_t* keys, const face_link* links,
                      const uint32_t* metas, step_params const& prm, const step_params* table,
                      const uint16_t* inst, uint32_t n, step_aux const& aux, hipStream_t s)
     {
         if (n == 0) return;
         const dim3 per_leaf(xcd_grid(n));
-        // run variants: (list index, non-temporal stores)
-        int rk = -1;
-        bool rnt = false;
-        if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
+        if constexpr (kWave)
         {
-            const int first = (R == 3) ? 7 : 4;
-            if (variant >= first && variant < first + 3)
+            const int rk = variant - 4; // run list index of the run variants
+            if (rk >= 0 && !aux.list[rk]) variant = 0; // no run list: wave-per-leaf kernel
+            switch (variant)
             {
-                rk  = (variant == first) ? 0 : 1;
-                rnt = variant == first + 2;
-                if (!aux.list[rk]) variant = 0; // no run list: wave-per-leaf kernel
-            }
-        }
-        if constexpr ((R == 3 && N == 8) || (R == 2 && N == 16))
-        {
-            const int pf0 = (R == 3) ? 12 : 9;
-            if (variant == pf0 || variant == pf0 + 1)
-            {
-                // K consecutive leaves per wave, next leaf's records prefetched
-                const uint32_t k = (variant == pf0) ? 2 : 4;
-                const dim3     per_wave(xcd_grid((n + k - 1) / k));
-                if constexpr (R == 3)
+                case 1: launch_wave<true>(per_leaf, 0, u, un, keys, links, metas, prm, table, inst, n, s); break;
+                case 2: launch_wave<false>(per_leaf, 10 << 10, u, un, keys, links, metas, prm, table, inst, n, s); break;
+                case 3:
+                    hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm,
+                                       table, inst, n);
+                    break;
+                case 4:
+                case 5:
                 {
-                    if (k == 2)
-                        hipLaunchKernelGGL(k_step_3d8_pf<2>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
-                    else
-                        hipLaunchKernelGGL(k_step_3d8_pf<4>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
-                }
-                else
-                {
-                    if (k == 2)
-                        hipLaunchKernelGGL(k_step_2d16_pf<2>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
-                    else
-                        hipLaunchKernelGGL(k_step_2d16_pf<4>, per_wave, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
-                }
-                check_launch("k_step (prefetch)");
-                return;
-            }
-            if (variant == ((R == 3) ? 11 : 8))
-            {
-                // wave kernel with neighbour slots computed from the leaf index when
-                // the whole tree is uniform (no table reads); the table otherwise
-                if constexpr (R == 3)
-                    hipLaunchKernelGGL((k_step_3d8<false, false>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, nullptr, nullptr, aux.uniform_level);
-                else
-                    hipLaunchKernelGGL((k_step_2d16<false, false>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, nullptr, nullptr, aux.uniform_level);
-                check_launch("k_step (uniform)");
-                return;
-            }
-            if (variant_needs(variant) & NEEDS_XFACES)
-            {
-                if (aux.xf_in && aux.xf_out)
-                {
+                    const dim3 per_run(xcd_grid(aux.count[rk]));
                     if constexpr (R == 3)
-                        hipLaunchKernelGGL((k_step_3d8<false, true>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, aux.xf_in, aux.xf_out);
+                        hipLaunchKernelGGL(k_step_3d8_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas,
+                                           prm, table, inst, aux.list[rk], aux.count[rk]);
                     else
-                        hipLaunchKernelGGL((k_step_2d16<false, true>), per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n, aux.xf_in, aux.xf_out);
-                    check_launch("k_step (x-face buffers)");
-                    return;
+                        hipLaunchKernelGGL(k_step_2d16_run<false>, per_run, dim3(64), 0, s, u, un, keys, links,
+                                           metas, prm, table, inst, aux.list[rk], aux.count[rk]);
+                    break;
                 }
-                variant = 0; // no side buffers: wave-per-leaf kernel
-            }
-        }
-        if (rk >= 0 && variant != 0)
-        {
-            const dim3 per_run(xcd_grid(aux.count[rk]));
-            if constexpr (R == 3 && N == 8)
-            {
-                if (rnt)
-                    hipLaunchKernelGGL(k_step_3d8_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
-                else
-                    hipLaunchKernelGGL(k_step_3d8_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
-            }
-            else if constexpr (R == 2 && N == 16)
-            {
-                if (rnt)
-                    hipLaunchKernelGGL(k_step_2d16_run<true>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
-                else
-                    hipLaunchKernelGGL(k_step_2d16_run<false>, per_run, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, aux.list[rk], aux.count[rk]);
-            }
-            check_launch("k_step (runs)");
-            return;
-        }
-        if constexpr (R == 3 && N == 8)
-        {
-            const dim3 per_group(xcd_grid((n + kGroup3d - 1) / kGroup3d));
-            switch (variant)
-            {
-                case 1: hipLaunchKernelGGL(k_step_3d8<true>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                case 2: hipLaunchKernelGGL(k_step_3d8_group<false>, per_group, dim3(512), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                case 3: hipLaunchKernelGGL(k_step_3d8_group<true>, per_group, dim3(512), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                case 4: hipLaunchKernelGGL(k_step_3d8_group<false>, per_group, dim3(512), 16 << 10, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                case 5: hipLaunchKernelGGL(k_step_3d8<false>, per_leaf, dim3(64), 8 << 10, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                case 6: hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                default: hipLaunchKernelGGL(k_step_3d8<false>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
-            }
-        }
-        else if constexpr (R == 2 && N == 16)
-        {
-            switch (variant)
-            {
-                case 1: hipLaunchKernelGGL(k_step_2d16<true>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                case 2: hipLaunchKernelGGL(k_step_2d16<false>, per_leaf, dim3(64), 10 << 10, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                case 3: hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm, table, inst, n); break;
-                default: hipLaunchKernelGGL(k_step_2d16<false>, per_leaf, dim3(64), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+                default: launch_wave<false>(per_leaf, 0, u, un, keys, links, metas, prm, table, inst, n, s);
             }
         }
         else
-            hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm, table, inst, n);
+            hipLaunchKernelGGL((k_step<R, N, NT>), per_leaf, dim3(NT), 0, s, u, un, keys, links, metas, prm, table,
+                               inst, n);
         check_launch("k_step");
     }
     static void padded(const double* u, double* pad, const face_link* links, const uint32_t* metas, uint32_t n,
@@ -311,7 +211,7 @@ struct instance
     static constexpr ops table()
     {
         return ops{ R,       N,          H,       &step,     &variant_count, &variant_name, &variant_needs,
-                    &xfaces, &padded,    &criterion, &emit, &adj_step,      &adj_regrid };
+                    &padded, &criterion, &emit, &adj_step, &adj_regrid };
     }
 };
 
@@ -414,11 +314,11 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* blo
 }
 
 // The +x run lists (runs of at most 4 and 8 leaves) of one leaf table, for
-// the shapes that have run-streaming step kernels.
+// the shapes that have run kernels.
 struct run_lists
 {
-    leaf_run*       list[2]{};
-    uint32_t        count[2]{};
+    leaf_run*       list[kRunLists]{};
+    uint32_t        count[kRunLists]{};
     uint32_t*       flag       = nullptr;
     uint32_t*       pos        = nullptr;
     uint32_t*       block_sums = nullptr;
@@ -430,7 +330,9 @@ struct run_lists
     run_lists& operator=(run_lists const&) = delete;
     ~run_lists()
     {
-        for (void* p : { (void*)list[0], (void*)list[1], (void*)flag, (void*)pos, (void*)block_sums, (void*)summary })
+        for (int k = 0; k < kRunLists; ++k)
+            if (list[k]) (void)hipFree(list[k]);
+        for (void* p : { (void*)flag, (void*)pos, (void*)block_sums, (void*)summary })
             if (p) (void)hipFree(p);
         if (h_summary) (void)hipHostFree(h_summary);
     }
@@ -439,18 +341,18 @@ struct run_lists
 
     void allocate(uint64_t cap)
     {
-        for (int k = 0; k < 2; ++k) list[k] = dev_alloc<leaf_run>(cap);
+        for (int k = 0; k < kRunLists; ++k) list[k] = dev_alloc<leaf_run>(cap);
         flag       = dev_alloc<uint32_t>(cap);
         pos        = dev_alloc<uint32_t>(cap);
         block_sums = dev_alloc<uint32_t>(cap / kScanBlock + 2);
-        summary    = dev_alloc<regrid_summary>(2);
-        AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_summary), 2 * sizeof(regrid_summary), 0));
+        summary    = dev_alloc<regrid_summary>(kRunLists);
+        AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_summary), kRunLists * sizeof(regrid_summary), 0));
     }
 
     step_aux view() const
     {
         step_aux v;
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < kRunLists; ++k)
         {
             v.list[k]  = list[k];
             v.count[k] = count[k];
@@ -463,8 +365,8 @@ struct run_lists
                hipStream_t s)
     {
         if (!flag) return;
-        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, 2 * sizeof(regrid_summary), s));
-        for (int k = 0; k < 2; ++k)
+        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, kRunLists * sizeof(regrid_summary), s));
+        for (int k = 0; k < kRunLists; ++k)
         {
             if (rank == 2)
                 hipLaunchKernelGGL(k_run_heads<2>, dim3(grid1d(n, 256)), dim3(256), 0, s, keys, n, depth, metas, kRunK[k], flag);
@@ -480,50 +382,10 @@ struct run_lists
                                    kRunK[k], flag, pos, list[k]);
             check_launch("k_run_emit");
         }
-        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, 2 * sizeof(regrid_summary), hipMemcpyDeviceToHost, s));
+        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, kRunLists * sizeof(regrid_summary), hipMemcpyDeviceToHost, s));
         AMR_HIP_CHECK(hipStreamSynchronize(s));
-        for (int k = 0; k < 2; ++k) count[k] = h_summary[k].total;
+        for (int k = 0; k < kRunLists; ++k) count[k] = h_summary[k].total;
     }
-};
-
-// The x-face side buffers of a ping-pong pair of interior buffers (index =
-// interior buffer index) and whether those of the current buffer match it.
-struct xface_pair
-{
-    double* xf[2]{};
-    bool    valid = false;
-
-    xface_pair() = default;
-    xface_pair(xface_pair const&) = delete;
-    xface_pair& operator=(xface_pair const&) = delete;
-    ~xface_pair()
-    {
-        for (double* p : xf)
-            if (p) (void)hipFree(p);
-    }
-
-    void allocate(uint64_t leaves, int face)
-    {
-        for (int k = 0; k < 2; ++k) xf[k] = dev_alloc<double>(leaves * 2 * static_cast<uint64_t>(face));
-    }
-    bool wanted(ops const* op, int v) const { return xf[0] && (op->variant_needs(v) & NEEDS_XFACES); }
-
-    // Extras of a step of variant v reading buffer `cur`: the side buffers
-    // of the state read (built first if stale) and of the state written.
-    step_aux prepare(ops const* op, int v, step_aux aux, const double* u_cur, int cur, uint64_t n, hipStream_t s)
-    {
-        if (!wanted(op, v)) return aux;
-        if (!valid)
-        {
-            op->xfaces(u_cur, xf[cur], static_cast<uint32_t>(n), s);
-            valid = true;
-        }
-        aux.xf_in  = xf[cur];
-        aux.xf_out = xf[1 - cur];
-        return aux;
-    }
-    // After that step: the new state's side buffers exist only if it wrote them.
-    void stepped(ops const* op, int v) { valid = wanted(op, v); }
 };
 
 // Times every step-kernel variant on trial steps that evolve a copy of the
@@ -532,16 +394,13 @@ struct xface_pair
 // not modified.  launch(v, src, dst, aux) queues one step.  Returns the
 // fastest variant.
 template <typename Launch>
-int autotune_steps(ops const* op, const double* u_cur, double* u_other, uint64_t leaves, int ni, int face,
-                   bool with_xf, step_aux const& runs, Launch&& launch, hipStream_t stream, uint32_t trials,
-                   float* best_us)
+int autotune_steps(ops const* op, const double* u_cur, double* u_other, uint64_t leaves, int ni, Launch&& launch,
+                   hipStream_t stream, uint32_t trials, float* best_us)
 {
-    double*    tmp = dev_alloc<double>(leaves * ni);
-    xface_pair txf;
-    int        best = 0;
+    double* tmp  = dev_alloc<double>(leaves * ni);
+    int     best = 0;
     try
     {
-        if (with_xf) txf.allocate(leaves, face);
         double* bufs[2] = { tmp, u_other };
         int     p       = 0;
         AMR_HIP_CHECK(hipMemcpyAsync(tmp, u_cur, leaves * ni * sizeof(double), hipMemcpyDeviceToDevice, stream));
@@ -549,9 +408,7 @@ int autotune_steps(ops const* op, const double* u_cur, double* u_other, uint64_t
             op->variant_count(),
             [&](int v)
             {
-                const step_aux aux = txf.prepare(op, v, runs, bufs[p], p, leaves, stream);
-                launch(v, bufs[p], bufs[1 - p], aux);
-                txf.stepped(op, v);
+                launch(v, bufs[p], bufs[1 - p]);
                 p = 1 - p;
             },
             stream, trials, best_us);
@@ -622,8 +479,6 @@ struct amr_ctx
     bool            broken        = false;
     int             variant       = 0; // step-kernel variant (autotuner)
     run_lists       runs;              // +x runs of the current table
-    xface_pair      xfp;               // x-face side buffers of u[0] / u[1]
-    uint64_t        sweeps        = 0; // steps launched (sets the sweep direction)
 
     ~amr_ctx()
     {
@@ -687,19 +542,6 @@ struct amr_ctx
         if (broken) throw amr_error(AMR_ERR_TOPOLOGY, "context unusable after a topology error");
     }
 
-    // Level of a uniform tree (all 2^(rank L) leaves at level L), else -1.
-    int uniform_level() const
-    {
-        return (rank * max_level < 32 && n == (1ull << (rank * max_level))) ? max_level : -1;
-    }
-
-    step_aux aux_base() const
-    {
-        step_aux a      = runs.view();
-        a.uniform_level = uniform_level();
-        return a;
-    }
-
     step_params make_params(double dt) const
     {
         step_params prm{};
@@ -716,12 +558,9 @@ struct amr_ctx
     void step(double dt)
     {
         ensure_usable();
-        step_params prm = make_params(dt);
-        prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
-        const step_aux aux = xfp.prepare(op, variant, aux_base(), u[cur], cur, n, stream);
+        const step_params prm = make_params(dt);
         op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
-                 aux, stream);
-        xfp.stepped(op, variant);
+                 runs.view(), stream);
         cur = 1 - cur;
     }
 
@@ -730,14 +569,13 @@ struct amr_ctx
     int autotune(uint32_t trials, float* best_us)
     {
         ensure_usable();
-        step_params prm = make_params(compute_dt());
-        variant         = autotune_steps(
-            op, u[cur], u[1 - cur], n, ni, ni / cfg.patch_size, xfp.xf[0] != nullptr, aux_base(),
-            [&](int v, const double* src, double* dst, step_aux const& aux)
+        const step_params prm = make_params(compute_dt());
+        variant               = autotune_steps(
+            op, u[cur], u[1 - cur], n, ni,
+            [&](int v, const double* src, double* dst)
             {
-                prm.reverse ^= 1u;
-                op->step(v, src, dst, keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n), aux,
-                         stream);
+                op->step(v, src, dst, keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
+                         runs.view(), stream);
             },
             stream, trials, best_us);
         return variant;
@@ -787,9 +625,8 @@ struct amr_ctx
         if (s.n_refine == 0 && s.n_coarsen == 0) return false;
         if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
         op->emit(keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, stream);
-        cur       = 1 - cur;
-        kc        = 1 - kc;
-        xfp.valid = false;
+        cur = 1 - cur;
+        kc  = 1 - kc;
         n   = s.total;
         AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
         sync();
@@ -806,7 +643,7 @@ struct amr_batch
 {
     ops const*               op = nullptr;
     hipStream_t              stream{};
-    int                      ni = 0, ndir = 0, face = 0;
+    int                      ni = 0, ndir = 0;
     uint64_t                 total = 0;
     std::vector<amr_ctx*>    src;
     std::vector<uint64_t>    off;
@@ -814,14 +651,12 @@ struct amr_batch
     double*                  u[2]{};
     int                      cur     = 0;
     int                      variant = 0;
-    uint64_t                 sweeps  = 0;
     uint64_t*                keys    = nullptr;
     face_link*               links   = nullptr;
     uint32_t*                metas   = nullptr;
     uint16_t*                inst    = nullptr;
     step_params*             table   = nullptr;
     run_lists                runs;
-    xface_pair               xfp;
     struct graph_entry
     {
         uint64_t        steps;
@@ -842,12 +677,8 @@ struct amr_batch
 
     void launch(int v)
     {
-        const step_aux aux = xfp.prepare(op, v, runs.view(), u[cur], cur, total, stream);
-        step_params    prm = host_table[0];
-        prm.reverse        = static_cast<uint32_t>(sweeps++ & 1);
-        op->step(v, u[cur], u[1 - cur], keys, links, metas, prm, table, inst, static_cast<uint32_t>(total), aux,
-                 stream);
-        xfp.stepped(op, v);
+        op->step(v, u[cur], u[1 - cur], keys, links, metas, host_table[0], table, inst, static_cast<uint32_t>(total),
+                 runs.view(), stream);
         cur = 1 - cur;
     }
 
@@ -860,12 +691,6 @@ struct amr_batch
         {
             for (uint64_t s = 0; s < steps; ++s) launch(variant);
             return;
-        }
-        // side buffers are built outside the graph, which then keeps them current
-        if (xfp.wanted(op, variant) && !xfp.valid)
-        {
-            op->xfaces(u[cur], xfp.xf[cur], static_cast<uint32_t>(total), stream);
-            xfp.valid = true;
         }
         hipGraphExec_t exec = nullptr;
         for (auto const& g : graphs)
@@ -895,7 +720,6 @@ struct amr_batch
         }
         AMR_HIP_CHECK(hipGraphLaunch(exec, stream));
         cur = (steps % 2) ? 1 - start : start;
-        if (steps > 0) xfp.stepped(op, variant);
     }
 };
 
@@ -1103,11 +927,9 @@ struct amr_tape
 
     void forward_step(uint64_t k, const double* src, double* dst)
     {
-        topology const& t   = topo(k);
-        step_params     prm = step_consts(dts[k]);
-        prm.reverse         = static_cast<uint32_t>(k & 1);
-        op->step(variant, src, dst, t.keys, t.links, t.metas, prm, nullptr, nullptr, static_cast<uint32_t>(t.n),
-                 step_aux{}, stream);
+        topology const& t = topo(k);
+        op->step(variant, src, dst, t.keys, t.links, t.metas, step_consts(dts[k]), nullptr, nullptr,
+                 static_cast<uint32_t>(t.n), step_aux{}, stream);
     }
 
     void forward(const double* u0, const double* v, double* uT)
@@ -1240,11 +1062,7 @@ int amr_tree_create(const amr_tree_config* cfg, amr_ctx** out)
             AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
             c->n         = 1;
             c->max_level = 0;
-            if (run_lists::wanted(op))
-            {
-                c->runs.allocate(cap);
-                c->xfp.allocate(cap, c->ni / cfg->patch_size);
-            }
+            if (run_lists::wanted(op)) c->runs.allocate(cap);
             c->build_table();
             *out = c.release();
         }
@@ -1368,7 +1186,6 @@ int amr_tree_set_interior(amr_ctx* ctx, const double* host, uint64_t count)
             if (count != ctx->n * ctx->ni) throw amr_error(AMR_ERR_ARGUMENT, "count must be size * n^rank");
             AMR_HIP_CHECK(hipMemcpyAsync(ctx->u[ctx->cur], host, count * sizeof(double), hipMemcpyHostToDevice,
                                          ctx->stream));
-            ctx->xfp.valid = false;
             ctx->sync();
         }
     );
@@ -1428,7 +1245,6 @@ int amr_tree_swap_buffers(amr_ctx* ctx)
         {
             if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             ctx->cur       = 1 - ctx->cur;
-            ctx->xfp.valid = false;
         }
     );
 }
@@ -1439,9 +1255,8 @@ int amr_tree_device_interior(amr_ctx* ctx, double** dev_ptr)
         [&]
         {
             if (!ctx || !dev_ptr) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            *dev_ptr       = ctx->u[ctx->cur];
-            ctx->xfp.valid = false; // the caller may write the interiors
-            ctx->sync();
+            *dev_ptr = ctx->u[ctx->cur];
+            ctx->sync(); // queued steps have written it
         }
     );
 }
@@ -1643,8 +1458,6 @@ int amr_batch_create(amr_ctx* const* trees, uint32_t count, amr_batch** out)
             if (run_lists::wanted(b->op))
             {
                 b->runs.allocate(b->total);
-                b->face = b->ni / trees[0]->cfg.patch_size;
-                b->xfp.allocate(b->total, b->face);
                 b->runs.build(b->op->rank, b->keys, static_cast<uint32_t>(b->total), trees[0]->depth, b->links,
                               b->metas, b->stream);
             }
@@ -1703,7 +1516,6 @@ int amr_batch_scatter(amr_batch* b)
                 amr_ctx* t = b->src[i];
                 AMR_HIP_CHECK(hipMemcpyAsync(t->u[t->cur], b->u[b->cur] + b->off[i] * b->ni, t->n * b->ni * sizeof(double),
                                              hipMemcpyDeviceToDevice, b->stream));
-                t->xfp.valid = false;
             }
             AMR_HIP_CHECK(hipStreamSynchronize(b->stream));
         }
@@ -1739,15 +1551,12 @@ int amr_batch_autotune(amr_batch* b, uint32_t trial_steps, int32_t* chosen, floa
         [&]
         {
             if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            step_params prm = b->host_table[0];
-            b->variant      = autotune_steps(
-                b->op, b->u[b->cur], b->u[1 - b->cur], b->total, b->ni, b->face, b->xfp.xf[0] != nullptr,
-                b->runs.view(),
-                [&](int v, const double* src, double* dst, step_aux const& aux)
+            b->variant = autotune_steps(
+                b->op, b->u[b->cur], b->u[1 - b->cur], b->total, b->ni,
+                [&](int v, const double* src, double* dst)
                 {
-                    prm.reverse ^= 1u;
-                    b->op->step(v, src, dst, b->keys, b->links, b->metas, prm, b->table, b->inst,
-                                static_cast<uint32_t>(b->total), aux, b->stream);
+                    b->op->step(v, src, dst, b->keys, b->links, b->metas, b->host_table[0], b->table, b->inst,
+                                static_cast<uint32_t>(b->total), b->runs.view(), b->stream);
                 },
                 b->stream, trial_steps, best_us);
             if (chosen) *chosen = b->variant;

@@ -297,20 +297,6 @@ int emu_runs(int rank, int depth, const uint64_t* keys, uint32_t n, const void* 
     return count;
 }
 
-// Face records computed from the leaf index of a uniform tree at level ul
-// (m: relation words, nb: neighbour slots, 2 rank entries per leaf).
-int emu_uniform_faces(int rank, int ul, uint32_t n, uint32_t* m, int32_t* nb)
-{
-    for (uint32_t i = 0; i < n; ++i)
-    {
-        if (rank == 2)
-            uniform_faces<2>(i, ul, m + 4 * i, nb + 4 * i);
-        else
-            uniform_faces<3>(i, ul, m + 6 * i, nb + 6 * i);
-    }
-    return 0;
-}
-
 // dtdx: [17][3]; vel: 3 values (vp, vm and sgn are derived as on the device).
 int emu_adj_step(int rank, int nn, const uint64_t* keys, uint32_t n, const void* links, const uint32_t* metas,
                  const double* lam_next, const double* u, const double* dtdx, const double* vel, double* lam,

@@ -26,7 +26,7 @@ def load():
             build()
         _lib = ctypes.CDLL(str(LIB))
         for name in ("emu_build_table", "emu_step", "emu_padded", "emu_flags", "emu_regrid", "emu_adj_step",
-                     "emu_adj_regrid", "emu_runs", "emu_uniform_faces"):
+                     "emu_adj_regrid", "emu_runs"):
             getattr(_lib, name).restype = ctypes.c_int
     return _lib
 
@@ -152,12 +152,3 @@ def runs(rank, depth, keys, links, metas, K):
                             ctypes.c_uint32(K), _p(out))
     return out[:count]
 
-
-def uniform_faces(rank, level):
-    """(relation words, neighbour slots) of every leaf of the uniform tree at
-    `level`, computed from the leaf index alone."""
-    n = 1 << (rank * level)
-    m = np.zeros(n * 2 * rank, dtype=np.uint32)
-    nb = np.zeros(n * 2 * rank, dtype=np.int32)
-    assert load().emu_uniform_faces(rank, level, ctypes.c_uint32(n), _p(m), _p(nb)) == 0
-    return m, nb

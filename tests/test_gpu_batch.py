@@ -108,15 +108,15 @@ def test_batch_variants_and_autotune(amr):
     assert 0 <= best < nvar and us > 0
 
 
-def test_batch_side_buffer_variant_with_graphs_and_switches(amr):
-    say("case: batch wave-xf graph replays with variant switches")
+def test_batch_run_variant_with_graphs_and_switches(amr):
+    say("case: batch run8 graph replays with variant switches")
     built = build(amr, 4)
     trees = [b[0] for b in built]
     names = built[0][1].step_variants()
-    xf = names.index("wave-xf")
+    rv = names.index("run8")
     ref = amr.Batch(trees)
     bt = amr.Batch(trees)
-    plan = [(xf, True, 4), (0, True, 3), (xf, True, 4), (xf, False, 1), (1, False, 2), (xf, True, 4), (xf, True, 3)]
+    plan = [(rv, True, 4), (0, True, 3), (rv, True, 4), (rv, False, 1), (1, False, 2), (rv, True, 4), (rv, True, 3)]
     for i, (v, graph, steps) in enumerate(plan):
         bt.set_step_variant(v)
         bt.advance(steps, use_graph=graph)

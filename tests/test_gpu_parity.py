@@ -369,11 +369,11 @@ def test_step_variants_bit_identical(amr, rank, n, levels, mixed):
 
 
 @pytest.mark.parametrize("rank,n,start", [(2, 16, 3), (3, 8, 2)])
-def test_side_buffer_variant_across_regrids_and_calls(amr, rank, n, start):
-    """The x-face side-buffer variant keeps its buffers current across single
+def test_run_variant_across_regrids_and_calls(amr, rank, n, start):
+    """The run-streaming variant keeps its run lists current across single
     steps, batches of steps, regrids, set_interior and variant switches:
     every state matches the oracle bit for bit."""
-    say(f"case: wave-xf across regrids rank={rank}")
+    say(f"case: run8 across regrids rank={rank}")
     depth = 7 if rank == 2 else 4
     vel = (0.8, -0.45, 0.3)[:rank]
     g, s, o = pair(amr, rank, n, 1, depth, 16384, vel, 0.35)
@@ -381,8 +381,8 @@ def test_side_buffer_variant_across_regrids_and_calls(amr, rank, n, start):
         g.refine_all()
         o.refine_all()
     names = s.step_variants()
-    xf = names.index("wave-xf")
-    s.set_step_variant(xf)
+    rv = names.index("run8")
+    s.set_step_variant(rv)
     rng = np.random.default_rng(31 + rank)
     u0 = rng.random(g.size() * n**rank)
     g.set_interior(u0)
@@ -405,12 +405,12 @@ def test_side_buffer_variant_across_regrids_and_calls(amr, rank, n, start):
         s.advance_batch_async(1)
         s.finish_advance_batch()
         o.advance_batch(1)
-        s.set_step_variant(xf)
+        s.set_step_variant(rv)
         s.advance_batch_async(2)
         s.finish_advance_batch()
         o.advance_batch(2)
         assert_fields(g.get_interior(), o.get_interior(), f"round {rd}")
-    # an interior written from the host replaces the side buffers too
+    # an interior written from the host
     u1 = rng.random(g.size() * n**rank)
     g.set_interior(u1)
     o.set_interior(u1)

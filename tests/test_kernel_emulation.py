@@ -123,15 +123,3 @@ def test_run_lists_partition_the_leaves(rank, n, depth, start, rounds, seed):
         if rd < rounds:
             o.reconstruct(random_flags(rng, keys, rank, depth))
 
-
-@pytest.mark.parametrize("rank,depth", [(2, 7), (3, 5)])
-def test_uniform_face_records_match_table(rank, depth):
-    """The implicit neighbour slots of a uniform tree (wave-uniform variant)
-    equal the neighbour table built from the keys, at every level."""
-    for level in range(0, 5 if rank == 2 else 4):
-        keys = np.array(brute.uniform_keys(rank, depth, level), dtype=np.uint64)
-        links, metas, err = emu.build_table(rank, depth, keys)
-        assert err == 0
-        m, nb = emu.uniform_faces(rank, level)
-        np.testing.assert_array_equal(metas & 3, m, err_msg=f"level {level}: relations")
-        np.testing.assert_array_equal(links["ids"][:, 0], nb, err_msg=f"level {level}: slots")
