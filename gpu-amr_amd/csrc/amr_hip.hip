@@ -479,6 +479,7 @@ struct amr_ctx
     bool            broken        = false;
     int             variant       = 0; // step-kernel variant (autotuner)
     run_lists       runs;              // +x runs of the current table
+    uint64_t        sweeps        = 0; // steps launched (sets the sweep direction)
 
     ~amr_ctx()
     {
@@ -558,7 +559,8 @@ struct amr_ctx
     void step(double dt)
     {
         ensure_usable();
-        const step_params prm = make_params(dt);
+        step_params prm = make_params(dt);
+        prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
         op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
                  runs.view(), stream);
         cur = 1 - cur;
@@ -569,11 +571,12 @@ struct amr_ctx
     int autotune(uint32_t trials, float* best_us)
     {
         ensure_usable();
-        const step_params prm = make_params(compute_dt());
-        variant               = autotune_steps(
+        step_params prm = make_params(compute_dt());
+        variant         = autotune_steps(
             op, u[cur], u[1 - cur], n, ni,
             [&](int v, const double* src, double* dst)
             {
+                prm.reverse ^= 1u; // trial steps alternate like a solve's
                 op->step(v, src, dst, keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
                          runs.view(), stream);
             },
@@ -651,6 +654,7 @@ struct amr_batch
     double*                  u[2]{};
     int                      cur     = 0;
     int                      variant = 0;
+    uint64_t                 sweeps  = 0;
     uint64_t*                keys    = nullptr;
     face_link*               links   = nullptr;
     uint32_t*                metas   = nullptr;
@@ -677,8 +681,10 @@ struct amr_batch
 
     void launch(int v)
     {
-        op->step(v, u[cur], u[1 - cur], keys, links, metas, host_table[0], table, inst, static_cast<uint32_t>(total),
-                 runs.view(), stream);
+        step_params prm = host_table[0];
+        prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
+        op->step(v, u[cur], u[1 - cur], keys, links, metas, prm, table, inst, static_cast<uint32_t>(total), runs.view(),
+                 stream);
         cur = 1 - cur;
     }
 
@@ -927,9 +933,11 @@ struct amr_tape
 
     void forward_step(uint64_t k, const double* src, double* dst)
     {
-        topology const& t = topo(k);
-        op->step(variant, src, dst, t.keys, t.links, t.metas, step_consts(dts[k]), nullptr, nullptr,
-                 static_cast<uint32_t>(t.n), step_aux{}, stream);
+        topology const& t   = topo(k);
+        step_params     prm = step_consts(dts[k]);
+        prm.reverse         = static_cast<uint32_t>(k & 1);
+        op->step(variant, src, dst, t.keys, t.links, t.metas, prm, nullptr, nullptr, static_cast<uint32_t>(t.n),
+                 step_aux{}, stream);
     }
 
     void forward(const double* u0, const double* v, double* uT)
@@ -1551,11 +1559,13 @@ int amr_batch_autotune(amr_batch* b, uint32_t trial_steps, int32_t* chosen, floa
         [&]
         {
             if (!b) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            b->variant = autotune_steps(
+            step_params prm = b->host_table[0];
+            b->variant      = autotune_steps(
                 b->op, b->u[b->cur], b->u[1 - b->cur], b->total, b->ni,
                 [&](int v, const double* src, double* dst)
                 {
-                    b->op->step(v, src, dst, b->keys, b->links, b->metas, b->host_table[0], b->table, b->inst,
+                    prm.reverse ^= 1u;
+                    b->op->step(v, src, dst, b->keys, b->links, b->metas, prm, b->table, b->inst,
                                 static_cast<uint32_t>(b->total), b->runs.view(), b->stream);
                 },
                 b->stream, trial_steps, best_us);

@@ -24,10 +24,23 @@ namespace amrhip
 // AdvectionPhysics.hpp:61-65).
 struct step_params
 {
-    double vel[3];
-    double hs[3];
-    double dtdx[kMaxLevels][3];
+    double   vel[3];
+    double   hs[3];
+    double   dtdx[kMaxLevels][3];
+    uint32_t reverse; // walk this launch's leaves backwards (sweep_item)
 };
+
+// Leaf (or run) of workgroup b in a launch over `count` items: XCD-contiguous
+// ranges (xcd_item), walked backwards when `reverse` is set.  Consecutive
+// steps alternate the direction, so that a step starts on the leaves the
+// previous one wrote last -- the lines most likely still held by the
+// memory-side cache (measured on the 8^3 step: 57 us alternating against
+// 65 us always forward).  Items >= count are skipped by the caller.
+__device__ inline uint32_t sweep_item(uint32_t b, uint32_t nblocks, uint32_t count, uint32_t reverse)
+{
+    const uint32_t i = xcd_item(b, nblocks);
+    return (reverse && i < count) ? count - 1 - i : i;
+}
 
 // Output store of a step kernel.  The non-temporal form streams the new
 // state past L2 so that L2 keeps the input lines that neighbouring leaves'
@@ -391,7 +404,7 @@ __global__ __launch_bounds__(NT) void k_step(const double* __restrict__ u, doubl
 {
     using S = patch_shape<R, N>;
     __shared__ double tile[tile_shape<R, N>::TS];
-    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
     if (leaf >= nleaves) return;
     stage_tile<R, N, NT>(tile, u, leaf, links, metas);
     __syncthreads();
@@ -424,7 +437,7 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
 {
     constexpr int NI = 512;
     __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
-    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
     if (leaf >= nleaves) return;
     const int     lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
     const double* up   = u + static_cast<size_t>(leaf) * NI;
@@ -489,7 +502,7 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
 {
     constexpr int NI = 512;
     __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
-    const uint32_t r = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, nruns, prm.reverse);
     if (r >= nruns) return;
     const int      lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
     const int      y = hi3, x = lo3;
@@ -582,7 +595,7 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
 {
     constexpr int NI = 256;
     __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
-    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
     if (leaf >= nleaves) return;
     const int     lane = threadIdx.x, x = lane & 15, q = lane >> 4;
     const double* up   = u + static_cast<size_t>(leaf) * NI;
@@ -638,7 +651,7 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
 {
     constexpr int NI = 256;
     __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
-    const uint32_t r = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, nruns, prm.reverse);
     if (r >= nruns) return;
     const int      lane = threadIdx.x, x = lane & 15, q = lane >> 4;
     const leaf_run run  = runs[r];
