@@ -30,16 +30,18 @@ struct step_params
     uint32_t reverse; // walk this launch's leaves backwards (sweep_item)
 };
 
-// Leaf (or run) of workgroup b in a launch over `count` items: XCD-contiguous
-// ranges (xcd_item), walked backwards when `reverse` is set.  Consecutive
-// steps alternate the direction, so that a step starts on the leaves the
-// previous one wrote last -- the lines most likely still held by the
-// memory-side cache (measured on the 8^3 step: 57 us alternating against
-// 65 us always forward).  Items >= count are skipped by the caller.
-__device__ inline uint32_t sweep_item(uint32_t b, uint32_t nblocks, uint32_t count, uint32_t reverse)
+// Leaf (or run) of workgroup b in a launch over nblocks items: each XCD owns
+// the same contiguous range in every launch (xcd_item), walked backwards
+// when `reverse` is set.  Consecutive steps alternate the direction, so that
+// a step starts on the leaves the previous one wrote last -- the lines most
+// likely still in the memory-side cache (8^3 step, 32768 leaves: 57 us
+// alternating against 65 us always forward) -- while a tree small enough for
+// the XCDs' L2s keeps finding its leaves in the L2 that wrote them.  Items
+// >= count are skipped by the caller.
+__device__ inline uint32_t sweep_item(uint32_t b, uint32_t nblocks, uint32_t reverse)
 {
-    const uint32_t i = xcd_item(b, nblocks);
-    return (reverse && i < count) ? count - 1 - i : i;
+    const uint32_t per = nblocks / kXcds, j = b / kXcds;
+    return (b % kXcds) * per + (reverse ? per - 1 - j : j);
 }
 
 // Output store of a step kernel.  The non-temporal form streams the new
@@ -243,29 +245,68 @@ __device__ inline void gather_ghosts_3d8(const double* __restrict__ u, const uin
 }
 
 // The ghost value a lane of the 2D wave layout gathers: cell t of face q
-// (lane = 16 q + t), single-cell fast path as gather_ghosts_3d8.
+// (lane = 16 q + t).  Branch-free over the relation, so that the wave's
+// loads are all in flight at once: every lane loads the four cells a finer
+// neighbour's mean needs -- a same-level or coarser neighbour's single cell
+// four times -- and keeps the mean (summed from 0.0 in row-major order, as
+// ghost_value) or the first value.
 __device__ inline double gather_ghost_2d16(const double* __restrict__ u, const uint32_t* ms, const int32_t* nbs,
-                                           const face_link* __restrict__ L, const uint32_t* __restrict__ M, int q,
-                                           int t)
+                                           const face_link* __restrict__ L, int q, int t)
 {
+    using S = patch_shape<2, 16>;
     int c[2];
     c[0] = (q == 0) ? -1 : (q == 1) ? 16 : t;
     c[1] = (q == 2) ? -1 : (q == 3) ? 16 : t;
+    // the lane's own face record, selected from the four uniform ones
+    uint32_t m  = ms[0];
+    int32_t  nb = nbs[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+        if (q == k)
+        {
+            m  = ms[k];
+            nb = nbs[k];
+        }
+    size_t src[4];
     if (single_cell_ghosts<2>(ms))
+        return u[ghost_source<2, 16>(nb, m, q, c)];
+    const bool finer = rel_of(m) == REL_FINER;
+    if (finer)
     {
-        // the lane's own face record, selected from the four uniform ones
-        uint32_t m  = ms[0];
-        int32_t  nb = nbs[0];
+        // ghost_value's finer rule: fine neighbour t / 8 of the face, cells
+        // (2 t) % 16 + {0, 1} along the face, the two layers next to it
+        const int dim = q >> 1, nrm = (q & 1) ? c[dim] - 16 : c[dim] + 16;
+        int       base[2];
+        for (int d = 0; d < 2; ++d) base[d] = (((d == dim) ? nrm : c[d]) * 2) % 16;
+        // second fine slot of each face, read as uniform values and selected
+        int32_t id1s[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) id1s[k] = L[k].ids[1];
+        int32_t id1 = id1s[0];
 #pragma unroll
         for (int k = 1; k < 4; ++k)
-            if (q == k)
-            {
-                m  = ms[k];
-                nb = nbs[k];
-            }
-        return u[ghost_source<2, 16>(nb, m, q, c)];
+            if (q == k) id1 = id1s[k];
+        const int32_t fine = (t / S::HALF) ? id1 : nb;
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+        {
+            const int cc[2] = { base[0] + ((o >> 1) & 1), base[1] + (o & 1) };
+            src[o]          = static_cast<size_t>(fine) * S::NI + S::lin(cc);
+        }
     }
-    return ghost_value<2, 16>(u, L[q], M[q], q, c);
+    else
+    {
+        const size_t one = ghost_source<2, 16>(nb, m, q, c);
+#pragma unroll
+        for (int o = 0; o < 4; ++o) src[o] = one;
+    }
+    double v[4];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) v[o] = u[src[o]];
+    double sum = 0.0;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) sum += v[o];
+    return finer ? sum / static_cast<double>(S::F) : v[0];
 }
 
 // Upwind update of one cell from its neighbours along each physical axis d
@@ -404,7 +445,7 @@ __global__ __launch_bounds__(NT) void k_step(const double* __restrict__ u, doubl
 {
     using S = patch_shape<R, N>;
     __shared__ double tile[tile_shape<R, N>::TS];
-    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
     if (leaf >= nleaves) return;
     stage_tile<R, N, NT>(tile, u, leaf, links, metas);
     __syncthreads();
@@ -437,7 +478,7 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
 {
     constexpr int NI = 512;
     __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
-    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
     if (leaf >= nleaves) return;
     const int     lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
     const double* up   = u + static_cast<size_t>(leaf) * NI;
@@ -502,7 +543,7 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
 {
     constexpr int NI = 512;
     __shared__ double gface[4][64]; // -y, +y (index z*8+x), -x, +x (index z*8+y)
-    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, nruns, prm.reverse);
+    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
     if (r >= nruns) return;
     const int      lane = threadIdx.x, hi3 = lane >> 3, lo3 = lane & 7;
     const int      y = hi3, x = lo3;
@@ -595,7 +636,7 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
 {
     constexpr int NI = 256;
     __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
-    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, nleaves, prm.reverse);
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
     if (leaf >= nleaves) return;
     const int     lane = threadIdx.x, x = lane & 15, q = lane >> 4;
     const double* up   = u + static_cast<size_t>(leaf) * NI;
@@ -608,7 +649,7 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
     int32_t          nb[4];
     load_faces<2>(L, M, m, nb);
     // lane (q, x) gathers cell x of face q
-    gface[q][x] = gather_ghost_2d16(u, m, nb, L, M, q, x);
+    gface[q][x] = gather_ghost_2d16(u, m, nb, L, q, x);
     __syncthreads();
     const step_params& P       = params_for(prm, table, inst, leaf);
     const int          lv      = key_level(keys[leaf]);
@@ -651,7 +692,7 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
 {
     constexpr int NI = 256;
     __shared__ double gface[4][16]; // -y, +y (index x), -x, +x (index y)
-    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, nruns, prm.reverse);
+    const uint32_t r = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
     if (r >= nruns) return;
     const int      lane = threadIdx.x, x = lane & 15, q = lane >> 4;
     const leaf_run run  = runs[r];
@@ -675,7 +716,7 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
         int32_t  nb[4];
         load_faces<2>(links + e0, metas + e0, m, nb);
         if (q < 2 || (q == 2 && i == 0) || (q == 3 && !more))
-            gface[q][x] = gather_ghost_2d16(u, m, nb, links + e0, metas + e0, q, x);
+            gface[q][x] = gather_ghost_2d16(u, m, nb, links + e0, q, x);
         const uint32_t next = more ? static_cast<uint32_t>(links[e0 + 3].ids[0]) : leaf;
         if (more)
         {
