@@ -309,6 +309,30 @@ __device__ inline double gather_ghost_2d16(const double* __restrict__ u, const u
     return finer ? sum / static_cast<double>(S::F) : v[0];
 }
 
+// The value of lane l - 1 (row_shr) or l + 1 (row_shl) within the lane's
+// 16-lane DPP row.  In both wave layouts x runs along consecutive lanes of a
+// row, so this is the x neighbour; a lane whose neighbour would come from
+// outside its row (or from the other x row of a 3D row) is an x boundary lane,
+// which takes its ghost value instead.  A VALU move, where __shfl goes
+// through the LDS unit.
+__device__ inline double lane_minus1(double v)
+{
+    const uint64_t b  = __builtin_bit_cast(uint64_t, v);
+    const int      lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
+    const int      l  = __builtin_amdgcn_update_dpp(lo, lo, 0x111, 0xF, 0xF, false); // row_shr:1
+    const int      h  = __builtin_amdgcn_update_dpp(hi, hi, 0x111, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(h)) << 32) | static_cast<uint32_t>(l));
+}
+
+__device__ inline double lane_plus1(double v)
+{
+    const uint64_t b  = __builtin_bit_cast(uint64_t, v);
+    const int      lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
+    const int      l  = __builtin_amdgcn_update_dpp(lo, lo, 0x101, 0xF, 0xF, false); // row_shl:1
+    const int      h  = __builtin_amdgcn_update_dpp(hi, hi, 0x101, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(h)) << 32) | static_cast<uint32_t>(l));
+}
+
 // Upwind update of one cell from its neighbours along each physical axis d
 // (lo[d] = u_L, hi[d] = u_R): the expression order of
 // amr_solver::time_step_cpu (amr_solver.hpp:302-347) and
@@ -507,8 +531,8 @@ __global__ __launch_bounds__(64) void k_step_3d8(const double* __restrict__ u, d
     for (int z = 0; z < 8; ++z)
     {
         const double uc = r[z];
-        const double xl = __shfl_up(uc, 1, 64);
-        const double xr = __shfl_down(uc, 1, 64);
+        const double xl = lane_minus1(uc);
+        const double xr = lane_plus1(uc);
         const double yl = __shfl_up(uc, 8, 64);
         const double yr = __shfl_down(uc, 8, 64);
         const double lx = (x == 0) ? gface[2][z * 8 + y] : xl;
@@ -593,8 +617,8 @@ __global__ __launch_bounds__(64) void k_step_3d8_run(const double* __restrict__ 
         for (int z = 0; z < 8; ++z)
         {
             const double uc = cur[z];
-            const double xl = __shfl_up(uc, 1, 64);
-            const double xr = __shfl_down(uc, 1, 64);
+            const double xl = lane_minus1(uc);
+            const double xr = lane_plus1(uc);
             const double yl = __shfl_up(uc, 8, 64);
             const double yr = __shfl_down(uc, 8, 64);
             const double xn = __shfl(nxt[z], lane & ~7, 64); // next leaf, x = 0, same (z, y)
@@ -664,8 +688,8 @@ __global__ __launch_bounds__(64) void k_step_2d16(const double* __restrict__ u, 
     {
         const int    y  = 4 * q + j;
         const double uc = r[j];
-        const double xl = __shfl_up(uc, 1, 64);
-        const double xr = __shfl_down(uc, 1, 64);
+        const double xl = lane_minus1(uc);
+        const double xr = lane_plus1(uc);
         const double lx = (x == 0) ? gface[2][y] : xl;
         const double hx = (x == 15) ? gface[3][y] : xr;
         const double hy = (j < 3) ? r[j < 3 ? j + 1 : 3] : ((q == 3) ? gface[1][x] : below);
@@ -737,8 +761,8 @@ __global__ __launch_bounds__(64) void k_step_2d16_run(const double* __restrict__
         {
             const int    y  = 4 * q + j;
             const double uc = cur[j];
-            const double xl = __shfl_up(uc, 1, 64);
-            const double xr = __shfl_down(uc, 1, 64);
+            const double xl = lane_minus1(uc);
+            const double xr = lane_plus1(uc);
             const double xn = __shfl(nxt[j], lane & ~15, 64); // next leaf, x = 0, same row
             const double lx = (x == 0) ? gface[2][y] : xl;
             const double hx = (x == 15) ? (more ? xn : gface[3][y]) : xr;
