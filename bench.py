@@ -71,9 +71,9 @@ class HipEvents:
         return float(ms.value)
 
 
-def traffic_from_profiles():
-    """HBM bytes per step-kernel launch from the committed PMC summary."""
-    p = ROOT / "profiles" / "k_step_3d8_c3_traffic.md"
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from its committed PMC summary."""
+    p = ROOT / "profiles" / f"{kernel}_c3_traffic.md"
     if not p.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
@@ -253,6 +253,7 @@ def main():
     ap.add_argument("--batched-instances", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     ap.add_argument("--variant", default="", help="step-kernel variant to run instead of autotuning")
+    ap.add_argument("--no-fusion", action="store_true", help="one step per launch (no two-step fusion)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
@@ -267,6 +268,8 @@ def main():
     g, s = workloads.build_c3(amr)
     leaves = g.size()
     cells = leaves * 512
+    s.set_step_fusion(not args.no_fusion)
+    fused = s.step_fusion() == (True, True)
     log(f"tree: {leaves} leaves, {cells} cells; warmup {args.warmup} steps")
     s.advance_batch_async(args.warmup)
     s.finish_advance_batch()
@@ -282,7 +285,9 @@ def main():
         s.set_step_variant(best)
         how = "fixed"
     else:
-        # per-variant timing (diagnostic), then the autotuner's choice
+        # per-variant timing of single-step launches (diagnostic), then the
+        # autotuner's choice (used when a step runs alone)
+        s.set_step_fusion(False)
         for v, name in enumerate(names):
             s.set_step_variant(v)
             g.synchronize()
@@ -292,8 +297,9 @@ def main():
             s.finish_advance_batch()
             per_variant[name] = round(ev.elapsed_ms(e0, e1) / 20 * 1e3, 2)
         best, best_us = s.autotune(20)
+        s.set_step_fusion(not args.no_fusion)
         how = "autotuned"
-        log(f"step variants (us/step): {per_variant}; autotuner -> {names[best]} ({best_us:.1f} us)")
+        log(f"single-step variants (us/step): {per_variant}; autotuner -> {names[best]} ({best_us:.1f} us)")
 
     # the read+write streaming rate of a plain copy of the same state size
     copy_ms = copy_kernel(cells, 20, ev)
@@ -307,11 +313,17 @@ def main():
     ev.record(e1, stream)
     _, executed = s.finish_advance_batch()
     wall = time.perf_counter() - t0
-    kernel_ms = ev.elapsed_ms(e0, e1) / args.steps
     assert executed == args.steps
+    # launches: pairs of steps when fused (a trailing odd step runs alone)
+    per_launch = 2 if fused else 1
+    launches = args.steps // per_launch
+    assert not fused or args.steps % 2 == 0, "time an even step count when fusing"
+    launch_ms = ev.elapsed_ms(e0, e1) / launches
+    step_ms = launch_ms / per_launch
     value = args.steps * cells / wall
-    achieved = BYTES_PER_CELL_UPDATE * cells / (kernel_ms * 1e-3) / 1e9
-    log(f"gpu: {wall * 1e3 / args.steps:.3f} ms/step wall, {kernel_ms * 1e3:.1f} us/step kernel, {achieved:.0f} GB/s")
+    achieved = BYTES_PER_CELL_UPDATE * cells * per_launch / (launch_ms * 1e-3) / 1e9
+    log(f"gpu: {wall * 1e3 / args.steps:.3f} ms/step wall, {launch_ms * 1e3:.1f} us/launch "
+        f"({per_launch} step(s) per launch), {achieved:.0f} GB/s algorithmic")
     g.close()
 
     batched = None
@@ -335,6 +347,12 @@ def main():
         }
         log(f"cpu: {rate:.3e} cell-updates/s on {threads} threads")
 
+    if fused:
+        kernel = "k_step2_3d8_uniform"
+        kernel_desc = "k_step2_3d8_uniform: two steps per launch (u^n read once, u^(n+2) written once)"
+    else:
+        kernel = "k_step_3d8"
+        kernel_desc = f"single-step kernel variant '{names[best]}' ({how})"
     out = {
         "metric": "FP64 cell-updates/s (3D uniform octree, % of HBM roofline)",
         "value": value,
@@ -360,10 +378,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(),
-            "kernel": f"step kernel variant '{names[best]}' ({how})",
-            "kernel_us": kernel_ms * 1e3,
-            "variants_us": per_variant,
+            "traffic": traffic_from_profiles(kernel),
+            "kernel": kernel_desc,
+            "launch_us": launch_ms * 1e3,
+            "steps_per_launch": per_launch,
+            "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE * cells * per_launch,
+            "single_step_variants_us": per_variant,
             "copy_GBps": copy_gbs,
             "frac_of_copy": achieved / copy_gbs,
         },

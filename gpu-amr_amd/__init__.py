@@ -89,6 +89,8 @@ _SIGS = {
     "amr_solver_set_step_variant": [ctypes.c_void_p, ctypes.c_int32],
     "amr_solver_get_step_variant": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)],
     "amr_solver_autotune": [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
+    "amr_solver_set_step_fusion": [ctypes.c_void_p, ctypes.c_int32],
+    "amr_solver_get_step_fusion": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
     "amr_ctx_synchronize": [ctypes.c_void_p],
     "amr_batch_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
     "amr_batch_destroy": [ctypes.c_void_p],
@@ -364,6 +366,17 @@ class AmrSolver:
         v, us = ctypes.c_int32(), ctypes.c_float()
         _check(self.tree.lib.amr_solver_autotune(self.tree.h, int(trial_steps), ctypes.byref(v), ctypes.byref(us)))
         return v.value, us.value
+
+    def set_step_fusion(self, enable):
+        """Two steps per launch on single-level trees (default on; results
+        are bit-identical either way)."""
+        _check(self.tree.lib.amr_solver_set_step_fusion(self.tree.h, 1 if enable else 0))
+
+    def step_fusion(self):
+        """(enabled, applies to the current tree)."""
+        e, a = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.tree.lib.amr_solver_get_step_fusion(self.tree.h, ctypes.byref(e), ctypes.byref(a)))
+        return bool(e.value), bool(a.value)
 
 
 class Batch:

@@ -8,6 +8,7 @@
 #include "amr_hip.h"
 #include "device_common.hpp"
 #include "kernels_adjoint.hpp"
+#include "kernels_fused.hpp"
 #include "kernels_step.hpp"
 #include "kernels_tree.hpp"
 
@@ -92,6 +93,8 @@ struct ops
                      adjoint_params const&, double*, double*, uint32_t, hipStream_t);
     void (*adj_regrid)(const double*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*, double*, uint32_t,
                        hipStream_t);
+    // two fused steps of a single-level tree (null: not built for the shape)
+    void (*step2)(const double*, double*, step2_params const&, uint32_t, hipStream_t);
 };
 
 template <int R, int N, int H>
@@ -208,10 +211,18 @@ struct instance
                            lam_old, n_old);
         check_launch("k_adj_regrid");
     }
+    static void step2(const double* u, double* un, step2_params const& prm, uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        if constexpr (R == 3 && N == 8)
+            hipLaunchKernelGGL(k_step2_3d8_uniform<false>, dim3(xcd_grid(n)), dim3(64), 0, s, u, un, prm, n);
+        check_launch("k_step2");
+    }
     static constexpr ops table()
     {
+        constexpr bool fused = R == 3 && N == 8;
         return ops{ R,       N,          H,       &step,     &variant_count, &variant_name, &variant_needs,
-                    &padded, &criterion, &emit, &adj_step, &adj_regrid };
+                    &padded, &criterion, &emit, &adj_step, &adj_regrid, fused ? &step2 : nullptr };
     }
 };
 
@@ -479,7 +490,8 @@ struct amr_ctx
     bool            broken        = false;
     int             variant       = 0; // step-kernel variant (autotuner)
     run_lists       runs;              // +x runs of the current table
-    uint64_t        sweeps        = 0; // steps launched (sets the sweep direction)
+    uint64_t        sweeps        = 0; // step launches (sets the sweep direction)
+    bool            fuse          = true; // two steps per launch on single-level trees
 
     ~amr_ctx()
     {
@@ -554,6 +566,37 @@ struct amr_ctx
         for (int l = 0; l <= depth; ++l)
             for (int d = 0; d < rank; ++d) prm.dtdx[l][d] = dt / cell_size(l, d);
         return prm;
+    }
+
+    // Whether the next two steps can run as one fused launch: the shape has
+    // the fused kernel and every leaf sits on the finest level (the leaves
+    // tile the domain, so n == 2^(rank * max_level) holds exactly then).
+    bool fusion_applies() const
+    {
+        return op->step2 && rank * max_level < 32 && n == (uint64_t{ 1 } << (rank * max_level));
+    }
+    bool fusable() const { return fuse && fusion_applies(); }
+
+    // Two steps (dt1 then dt2) in one launch; bit-identical to step(dt1),
+    // step(dt2).
+    void step2(double dt1, double dt2)
+    {
+        ensure_usable();
+        step2_params prm{};
+        for (int d = 0; d < 3; ++d)
+        {
+            prm.vel[d] = vel[d];
+            prm.hs[d]  = 0.5 * std::abs(vel[d]);
+        }
+        for (int d = 0; d < rank; ++d)
+        {
+            prm.dtdx[0][d] = dt1 / cell_size(max_level, d);
+            prm.dtdx[1][d] = dt2 / cell_size(max_level, d);
+        }
+        prm.level   = static_cast<uint32_t>(max_level);
+        prm.reverse = static_cast<uint32_t>(sweeps++ & 1);
+        op->step2(u[cur], u[1 - cur], prm, static_cast<uint32_t>(n), stream);
+        cur = 1 - cur;
     }
 
     void step(double dt)
@@ -1316,16 +1359,31 @@ int amr_solver_advance_batch_async(amr_ctx* ctx, uint64_t step_count, double rem
             ctx->pending_dt    = 0.0;
             ctx->pending_steps = 0;
             double remaining   = remaining_time;
-            for (uint64_t s = 0; s < step_count; ++s)
+            uint64_t s         = 0;
+            while (s < step_count)
             {
                 if (remaining <= 0.0) break;
                 const double dt      = ctx->compute_dt();
                 const double step_dt = std::min(dt, remaining);
                 if (step_dt <= 0.0) break;
+                const double rest = remaining - step_dt;
+                // the following step's dt, as the loop would take it
+                const double next_dt = (s + 1 < step_count && rest > 0.0) ? std::min(dt, rest) : 0.0;
+                if (next_dt > 0.0 && ctx->fusable())
+                {
+                    ctx->step2(step_dt, next_dt);
+                    ctx->pending_dt += step_dt;
+                    ctx->pending_dt += next_dt;
+                    remaining = rest - next_dt;
+                    ctx->pending_steps += 2;
+                    s += 2;
+                    continue;
+                }
                 ctx->step(step_dt);
                 ctx->pending_dt += step_dt;
-                remaining -= step_dt;
+                remaining = rest;
                 ++ctx->pending_steps;
+                ++s;
             }
         }
     );
@@ -1393,6 +1451,29 @@ int amr_solver_get_step_variant(amr_ctx* ctx, int32_t* variant)
         {
             if (!ctx || !variant) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             *variant = ctx->variant;
+        }
+    );
+}
+
+int amr_solver_set_step_fusion(amr_ctx* ctx, int32_t enable)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            ctx->fuse = enable != 0;
+        }
+    );
+}
+
+int amr_solver_get_step_fusion(amr_ctx* ctx, int32_t* enabled, int32_t* applies)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (enabled) *enabled = ctx->fuse ? 1 : 0;
+            if (applies) *applies = ctx->fusion_applies() ? 1 : 0;
         }
     );
 }

@@ -313,14 +313,18 @@ __device__ inline double gather_ghost_2d16(const double* __restrict__ u, const u
 // 16-lane DPP row.  In both wave layouts x runs along consecutive lanes of a
 // row, so this is the x neighbour; a lane whose neighbour would come from
 // outside its row (or from the other x row of a 3D row) is an x boundary lane,
-// which takes its ghost value instead.  A VALU move, where __shfl goes
-// through the LDS unit.
+// which takes its ghost value instead: such lanes read 0 here (bound_ctrl),
+// so the move needs no copy of the old value.  A VALU move, where __shfl
+// goes through the LDS unit.  Like every cross-lane read it must run with
+// all lanes active: evaluate it as a statement of its own, never as an
+// operand of ?: (which would run it only in the lanes taking that branch,
+// reading the others as invalid).
 __device__ inline double lane_minus1(double v)
 {
     const uint64_t b  = __builtin_bit_cast(uint64_t, v);
     const int      lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
-    const int      l  = __builtin_amdgcn_update_dpp(lo, lo, 0x111, 0xF, 0xF, false); // row_shr:1
-    const int      h  = __builtin_amdgcn_update_dpp(hi, hi, 0x111, 0xF, 0xF, false);
+    const int      l  = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xF, 0xF, true); // row_shr:1
+    const int      h  = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xF, 0xF, true);
     return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(h)) << 32) | static_cast<uint32_t>(l));
 }
 
@@ -328,8 +332,8 @@ __device__ inline double lane_plus1(double v)
 {
     const uint64_t b  = __builtin_bit_cast(uint64_t, v);
     const int      lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
-    const int      l  = __builtin_amdgcn_update_dpp(lo, lo, 0x101, 0xF, 0xF, false); // row_shl:1
-    const int      h  = __builtin_amdgcn_update_dpp(hi, hi, 0x101, 0xF, 0xF, false);
+    const int      l  = __builtin_amdgcn_update_dpp(0, lo, 0x101, 0xF, 0xF, true); // row_shl:1
+    const int      h  = __builtin_amdgcn_update_dpp(0, hi, 0x101, 0xF, 0xF, true);
     return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(h)) << 32) | static_cast<uint32_t>(l));
 }
 

@@ -179,6 +179,16 @@ int amr_solver_get_step_variant(amr_ctx* ctx, int32_t* variant);
  * changed), selects the fastest and returns its index and time per step. */
 int amr_solver_autotune(amr_ctx* ctx, uint32_t trial_steps, int32_t* chosen, float* best_us);
 
+/* Two-step fusion (no counterpart in the reference, which runs one halo
+ * exchange and one update per step, amr_solver.hpp:221-241).  On a tree
+ * whose leaves all sit on one level, advance_batch_async runs consecutive
+ * steps in pairs, each pair as one launch that reads u^n once and writes
+ * u^{n+2} once; the results are bit-identical to single steps.  Enabled by
+ * default.  `applies` reports whether the current tree qualifies (whether
+ * or not fusion is enabled). */
+int amr_solver_set_step_fusion(amr_ctx* ctx, int32_t enable);
+int amr_solver_get_step_fusion(amr_ctx* ctx, int32_t* enabled, int32_t* applies);
+
 /* Waits for all work queued on the context stream. */
 int amr_ctx_synchronize(amr_ctx* ctx);
 

@@ -5,6 +5,7 @@
 #include "kernels_step.hpp"
 #include "kernels_tree.hpp"
 #include "kernels_adjoint.hpp"
+#include "kernels_fused.hpp"
 
 #include <cstring>
 #include <vector>
@@ -336,6 +337,23 @@ int emu_adj_regrid(int rank, int nn, const double* lam_new, const uint32_t* maps
     EMU_SHAPES(X)
 #undef X
     return -1;
+}
+
+// The 27 slots at offsets (dx, dy, dz) in {-1, 0, 1}^3 (index 9 (dz+1) +
+// 3 (dy+1) + (dx+1)) of slot m in a single-level 3D tree at `level`, by the
+// fused kernel's dilated-integer arithmetic.
+int emu_uniform_neighbors(uint32_t level, uint32_t m, uint32_t* out)
+{
+    const morton_masks3 mk(level);
+    int                 i = 0;
+#define N3(DZ, DY, DX) out[i++] = mk.at<DX, DY, DZ>(m);
+#define ROW(DZ, DY) N3(DZ, DY, -1) N3(DZ, DY, 0) N3(DZ, DY, 1)
+#define PLANE(DZ) ROW(DZ, -1) ROW(DZ, 0) ROW(DZ, 1)
+    PLANE(-1) PLANE(0) PLANE(1)
+#undef PLANE
+#undef ROW
+#undef N3
+    return 0;
 }
 
 } // extern "C"

@@ -26,7 +26,7 @@ def load():
             build()
         _lib = ctypes.CDLL(str(LIB))
         for name in ("emu_build_table", "emu_step", "emu_padded", "emu_flags", "emu_regrid", "emu_adj_step",
-                     "emu_adj_regrid", "emu_runs"):
+                     "emu_adj_regrid", "emu_runs", "emu_uniform_neighbors"):
             getattr(_lib, name).restype = ctypes.c_int
     return _lib
 
@@ -152,3 +152,11 @@ def runs(rank, depth, keys, links, metas, K):
                             ctypes.c_uint32(K), _p(out))
     return out[:count]
 
+
+
+def uniform_neighbors(level, slot):
+    """The 27 neighbour slots (offsets (dx, dy, dz) in {-1,0,1}^3, dx fastest)
+    of a slot in a single-level 3D tree, by the fused kernel's arithmetic."""
+    out = np.zeros(27, dtype=np.uint32)
+    load().emu_uniform_neighbors(ctypes.c_uint32(level), ctypes.c_uint32(slot), _p(out))
+    return out

@@ -419,3 +419,57 @@ def test_run_variant_across_regrids_and_calls(amr, rank, n, start):
     s.finish_advance_batch()
     o.advance_batch(2)
     assert_fields(g.get_interior(), o.get_interior(), "after set_interior")
+
+
+@pytest.mark.parametrize("level,steps,seed", [(0, 5, 1), (1, 6, 2), (2, 9, 3), (3, 8, 4)])
+def test_fused_pairs_bit_identical(amr, level, steps, seed):
+    """Two steps per launch on single-level 3D trees: bit-identical to single
+    steps and to the oracle, including a pair whose second step is clamped
+    to the remaining time (even step counts) and an odd step count (the last
+    step runs alone)."""
+    say(f"case: fused steps, 3D 8^3 single level {level}, {steps} steps, seed {seed}")
+    vel = (1.0, -0.5, 0.25)
+    cap = (1 << (3 * level)) + 8
+    fused, sf, o = pair(amr, 3, 8, 1, 5, cap, vel, 0.3)
+    plain = amr.NdTree(3, 5, 8, 1, cap)
+    sp = amr.AmrSolver(plain, vel, 0.3)
+    sp.set_step_fusion(False)
+    for _ in range(level):
+        fused.refine_all()
+        plain.refine_all()
+        o.refine_all()
+    assert sf.step_fusion() == (True, True)
+    assert sp.step_fusion() == (False, True)
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(fused.size() * 512)
+    for t in (fused, plain, o):
+        t.set_interior(v)
+    o.halo_exchange()
+    dt = sf.compute_dt()
+    remaining = dt * (steps - 0.5)
+    sf.advance_batch_async(steps, remaining)
+    got_dt = sf.finish_advance_batch()
+    sp.advance_batch_async(steps, remaining)
+    assert got_dt == sp.finish_advance_batch() == o.advance_batch(steps, remaining)
+    got = fused.get_interior()
+    assert np.array_equal(got.view(np.uint64), plain.get_interior().view(np.uint64)), "fused vs single steps"
+    assert_fields(got, o.get_interior(), "fused vs oracle")
+
+
+def test_fusion_follows_the_topology(amr):
+    """Fusion applies exactly while every leaf sits on one level."""
+    say("case: fusion applicability across regrids")
+    g = amr.NdTree(3, 5, 8, 1, 600)
+    s = amr.AmrSolver(g, (1.0, 0.5, 0.25), 0.3)
+    g.refine_all()
+    assert s.step_fusion() == (True, True)
+    flags = np.zeros(g.size(), dtype=np.int8)
+    flags[3] = 1
+    assert g.reconstruct_tree(flags)
+    assert s.step_fusion() == (True, False)
+    g.refine_all()
+    assert s.step_fusion() == (True, False)
+    g2 = amr.NdTree(2, 7, 16, 1, 100)
+    s2 = amr.AmrSolver(g2, (1.0, 0.5), 0.4)
+    g2.refine_all()
+    assert s2.step_fusion() == (True, False), "no fused kernel for 2D patches"
