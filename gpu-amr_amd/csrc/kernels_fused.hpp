@@ -100,66 +100,68 @@ __global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restri
     const double*       own = P(leaf);
 
     // ---- loads --------------------------------------------------------------
-    // Every load is unconditional (a lane picks its side through the slot and
-    // offset), so that none of them waits behind a branch.
+    // The 18 neighbour slots are uniform over the wave and computed first;
+    // a lane picks its side with a select, so every load is unconditional
+    // and none waits behind a branch.
+    const uint32_t s_ym = mk.at<0, -1, 0>(leaf), s_yp = mk.at<0, 1, 0>(leaf);
+    const uint32_t s_xm = mk.at<-1, 0, 0>(leaf), s_xp = mk.at<1, 0, 0>(leaf);
+    const uint32_t s_zm = mk.at<0, 0, -1>(leaf), s_zp = mk.at<0, 0, 1>(leaf);
+    const uint32_t s_xmym = mk.at<-1, -1, 0>(leaf), s_xpym = mk.at<1, -1, 0>(leaf);
+    const uint32_t s_xmyp = mk.at<-1, 1, 0>(leaf), s_xpyp = mk.at<1, 1, 0>(leaf);
+    const uint32_t s_ymzm = mk.at<0, -1, -1>(leaf), s_ymzp = mk.at<0, -1, 1>(leaf);
+    const uint32_t s_ypzm = mk.at<0, 1, -1>(leaf), s_ypzp = mk.at<0, 1, 1>(leaf);
+    const uint32_t s_xmzm = mk.at<-1, 0, -1>(leaf), s_xmzp = mk.at<-1, 0, 1>(leaf);
+    const uint32_t s_xpzm = mk.at<1, 0, -1>(leaf), s_xpzp = mk.at<1, 0, 1>(leaf);
+    auto cell = [&](uint32_t slot, int off) { return u[static_cast<size_t>(slot) * NI + off]; };
+    const int ecol = llo ? 7 : 0, erow = llo ? 7 * 8 : 0, epl = hlo ? 7 * 64 : 0;
+
     // Ring cells, face layout (p, q) = (hi, lo): value r, second layer o,
     // adjacent core cell c, edge cells tq (q = -1 / 8 side) and tp (z = -1 / 8).
     double r[4], o[4], c[4], tq[4], tp[4];
     {
-        const double* ym = P(mk.at<0, -1, 0>(leaf));
-        const double* yp = P(mk.at<0, 1, 0>(leaf));
-        const int     zq = hi * 64 + lo; // (z, _, x) of a y face, without the row
-        r[0] = ym[zq + 7 * 8];
-        o[0] = ym[zq + 6 * 8];
-        c[0] = own[zq];
-        r[1] = yp[zq];
-        o[1] = yp[zq + 8];
-        c[1] = own[zq + 7 * 8];
-        // x side of a y face's q edge: leaf (-1 or +1, -+1, 0), column 7 or 0
-        const int ecol = llo ? 7 : 0;
-        tq[0] = P(llo ? mk.at<-1, -1, 0>(leaf) : mk.at<1, -1, 0>(leaf))[hi * 64 + 7 * 8 + ecol];
-        tq[1] = P(llo ? mk.at<-1, 1, 0>(leaf) : mk.at<1, 1, 0>(leaf))[hi * 64 + ecol];
-        // z side: leaf (0, -+1, -1 or +1), plane 7 or 0
-        const int epl = hlo ? 7 * 64 : 0;
-        tp[0] = P(hlo ? mk.at<0, -1, -1>(leaf) : mk.at<0, -1, 1>(leaf))[epl + 7 * 8 + lo];
-        tp[1] = P(hlo ? mk.at<0, 1, -1>(leaf) : mk.at<0, 1, 1>(leaf))[epl + lo];
-        const double* xm = P(mk.at<-1, 0, 0>(leaf));
-        const double* xp = P(mk.at<1, 0, 0>(leaf));
+        const int zq = hi * 64 + lo; // (z, _, x) of a y face, without the row
+        r[0]  = cell(s_ym, zq + 7 * 8);
+        o[0]  = cell(s_ym, zq + 6 * 8);
+        c[0]  = own[zq];
+        r[1]  = cell(s_yp, zq);
+        o[1]  = cell(s_yp, zq + 8);
+        c[1]  = own[zq + 7 * 8];
+        // y faces: x-side edge in leaf (-1 or +1, -+1, 0) at column 7 or 0,
+        // z-side edge in leaf (0, -+1, -1 or +1) at plane 7 or 0
+        tq[0] = cell(llo ? s_xmym : s_xpym, hi * 64 + 7 * 8 + ecol);
+        tq[1] = cell(llo ? s_xmyp : s_xpyp, hi * 64 + ecol);
+        tp[0] = cell(hlo ? s_ymzm : s_ymzp, epl + 7 * 8 + lo);
+        tp[1] = cell(hlo ? s_ypzm : s_ypzp, epl + lo);
         const int     zy = hi * 64 + lo * 8; // (z, y, _) of an x face
-        const double2 pm = *reinterpret_cast<const double2*>(xm + zy + 6);
-        const double2 pp = *reinterpret_cast<const double2*>(xp + zy);
-        r[2] = pm.y;
-        o[2] = pm.x;
-        c[2] = own[zy];
-        r[3] = pp.x;
-        o[3] = pp.y;
-        c[3] = own[zy + 7];
-        // y side of an x face's q edge: leaf (-+1, -1 or +1, 0), row 7 or 0
-        const int erow = llo ? 7 * 8 : 0;
-        tq[2] = P(llo ? mk.at<-1, -1, 0>(leaf) : mk.at<-1, 1, 0>(leaf))[hi * 64 + erow + 7];
-        tq[3] = P(llo ? mk.at<1, -1, 0>(leaf) : mk.at<1, 1, 0>(leaf))[hi * 64 + erow];
-        tp[2] = P(hlo ? mk.at<-1, 0, -1>(leaf) : mk.at<-1, 0, 1>(leaf))[epl + lo * 8 + 7];
-        tp[3] = P(hlo ? mk.at<1, 0, -1>(leaf) : mk.at<1, 0, 1>(leaf))[epl + lo * 8];
+        const double2 pm = *reinterpret_cast<const double2*>(u + static_cast<size_t>(s_xm) * NI + zy + 6);
+        const double2 pp = *reinterpret_cast<const double2*>(u + static_cast<size_t>(s_xp) * NI + zy);
+        r[2]  = pm.y;
+        o[2]  = pm.x;
+        c[2]  = own[zy];
+        r[3]  = pp.x;
+        o[3]  = pp.y;
+        c[3]  = own[zy + 7];
+        // x faces: y-side edge in leaf (-+1, -1 or +1, 0) at row 7 or 0,
+        // z-side edge in leaf (-+1, 0, -1 or +1) at plane 7 or 0
+        tq[2] = cell(llo ? s_xmym : s_xmyp, hi * 64 + erow + 7);
+        tq[3] = cell(llo ? s_xpym : s_xpyp, hi * 64 + erow);
+        tp[2] = cell(hlo ? s_xmzm : s_xmzp, epl + lo * 8 + 7);
+        tp[3] = cell(hlo ? s_xpzm : s_xpzp, epl + lo * 8);
     }
     // core column at z = -2..9, core layout (y, x) = (hi, lo)
     double a[12];
-    {
-        const double* zm = P(mk.at<0, 0, -1>(leaf));
-        const double* zp = P(mk.at<0, 0, 1>(leaf));
-        a[0] = zm[6 * 64 + lane];
-        a[1] = zm[7 * 64 + lane];
+    a[0] = cell(s_zm, 6 * 64 + lane);
+    a[1] = cell(s_zm, 7 * 64 + lane);
 #pragma unroll
-        for (int z = 0; z < 8; ++z) a[2 + z] = own[z * 64 + lane];
-        a[10] = zp[lane];
-        a[11] = zp[64 + lane];
-    }
+    for (int z = 0; z < 8; ++z) a[2 + z] = own[z * 64 + lane];
+    a[10] = cell(s_zp, lane);
+    a[11] = cell(s_zp, 64 + lane);
     // edge cells next to the z = -1 / 8 planes: ex at x = -1 / 8 (by the
     // lane's x side), ey at y = -1 / 8 (by its y side)
-    const int    xcol = llo ? 7 : 0, yrow = hlo ? 7 * 8 : 0;
-    const double exm  = P(llo ? mk.at<-1, 0, -1>(leaf) : mk.at<1, 0, -1>(leaf))[7 * 64 + hi * 8 + xcol];
-    const double exp  = P(llo ? mk.at<-1, 0, 1>(leaf) : mk.at<1, 0, 1>(leaf))[hi * 8 + xcol];
-    const double eym  = P(hlo ? mk.at<0, -1, -1>(leaf) : mk.at<0, 1, -1>(leaf))[7 * 64 + yrow + lo];
-    const double eyp  = P(hlo ? mk.at<0, -1, 1>(leaf) : mk.at<0, 1, 1>(leaf))[yrow + lo];
+    const double exm = cell(llo ? s_xmzm : s_xpzm, 7 * 64 + hi * 8 + ecol);
+    const double exp = cell(llo ? s_xmzp : s_xpzp, hi * 8 + ecol);
+    const double eym = cell(hlo ? s_ymzm : s_ypzm, 7 * 64 + (hlo ? 7 * 8 : 0) + lo);
+    const double eyp = cell(hlo ? s_ymzp : s_ypzp, (hlo ? 7 * 8 : 0) + lo);
 
     const double v[3] = { prm.vel[0], prm.vel[1], prm.vel[2] };
     const double h[3] = { prm.hs[0], prm.hs[1], prm.hs[2] };
