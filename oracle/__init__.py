@@ -66,6 +66,29 @@ def load():
     lib.oracle_step_kernel_only.argtypes = [vp, ctypes.c_double]
     lib.oracle_advance_batch.argtypes = [vp, sz, ctypes.c_double, dp, ctypes.POINTER(sz)]
     lib.oracle_criterion.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, i8p]
+    # Euler system
+    lib.oracle_euler_create.restype = vp
+    lib.oracle_euler_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, dp]
+    lib.oracle_euler_destroy.argtypes = [vp]
+    lib.oracle_euler_size.restype = sz
+    lib.oracle_euler_size.argtypes = [vp]
+    lib.oracle_euler_fields.argtypes = [vp]
+    lib.oracle_euler_keys.argtypes = [vp, u64p]
+    lib.oracle_euler_metadata.argtypes = [vp, vp]
+    lib.oracle_euler_reconstruct.argtypes = [vp, i8p, ctypes.POINTER(ctypes.c_int)]
+    for name in ("oracle_euler_set_interior", "oracle_euler_get_interior", "oracle_euler_get_padded", "oracle_euler_cell_centers"):
+        getattr(lib, name).argtypes = [vp, dp]
+    lib.oracle_euler_halo_exchange.argtypes = [vp]
+    lib.oracle_euler_set_params.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.oracle_euler_compute_dt.restype = ctypes.c_double
+    lib.oracle_euler_compute_dt.argtypes = [vp]
+    lib.oracle_euler_step.argtypes = [vp, ctypes.c_double]
+    lib.oracle_euler_advance_batch.argtypes = [vp, sz, ctypes.c_double, dp, ctypes.POINTER(sz)]
+    lib.oracle_euler_criterion.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, i8p]
+    lib.oracle_euler_prim_to_cons.argtypes = [ctypes.c_int, dp, dp, sz, ctypes.c_double]
+    lib.oracle_euler_flux.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.c_double]
+    lib.oracle_euler_max_speed.restype = ctypes.c_double
+    lib.oracle_euler_max_speed.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_double]
     _lib = lib
     return lib
 
@@ -187,6 +210,130 @@ class OracleTree:
         out = np.zeros(self.size(), dtype=np.int8)
         self._check(self.lib.oracle_criterion(self.h, int(kind), float(refine_thr), float(coarsen_thr), int(min_level), int(max_level), _ptr(out, ctypes.c_int8)))
         return out
+
+
+class OracleEuler:
+    """The reference's ndtree + amr_solver<..., EulerPhysics<R>, R> restated on
+    the CPU (EulerPhysics.hpp, amr_solver.hpp:105-413).  Host arrays are
+    field-major: all leaves of rho, then rho*u, rho*v, (rho*w), E."""
+
+    def __init__(self, rank, n, halo, depth, capacity, lengths=(1.0, 1.0, 1.0)):
+        self.lib = load()
+        self.rank, self.n, self.halo, self.depth = rank, n, halo, depth
+        self.nv = rank + 2
+        self.ni = n**rank
+        self.pad = n + 2 * halo
+        self.flat = self.pad**rank
+        L = np.ascontiguousarray(np.array(list(lengths) + [1.0] * (3 - len(lengths)), dtype=np.float64)[:3])
+        self.h = self.lib.oracle_euler_create(rank, n, halo, depth, capacity, _ptr(L, ctypes.c_double))
+        if not self.h:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.oracle_euler_destroy(h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+
+    def size(self):
+        return int(self.lib.oracle_euler_size(self.h))
+
+    def keys(self):
+        out = np.zeros(self.size(), dtype=np.uint64)
+        self._check(self.lib.oracle_euler_keys(self.h, _ptr(out, ctypes.c_uint64)))
+        return out
+
+    def metadata(self):
+        out = np.zeros(self.size() * 2 * self.rank, dtype=HALO_META_DTYPE)
+        self._check(self.lib.oracle_euler_metadata(self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def reconstruct(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int8)
+        assert f.shape == (self.size(),)
+        changed = ctypes.c_int(0)
+        self._check(self.lib.oracle_euler_reconstruct(self.h, _ptr(f, ctypes.c_int8), ctypes.byref(changed)))
+        return bool(changed.value)
+
+    def refine_all(self):
+        return self.reconstruct(np.ones(self.size(), dtype=np.int8))
+
+    def set_interior(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        assert v.size == self.nv * self.size() * self.ni
+        self._check(self.lib.oracle_euler_set_interior(self.h, _ptr(v, ctypes.c_double)))
+
+    def get_interior(self):
+        out = np.zeros(self.nv * self.size() * self.ni, dtype=np.float64)
+        self._check(self.lib.oracle_euler_get_interior(self.h, _ptr(out, ctypes.c_double)))
+        return out.reshape(self.nv, self.size(), *([self.n] * self.rank))
+
+    def get_padded(self):
+        out = np.zeros(self.nv * self.size() * self.flat, dtype=np.float64)
+        self._check(self.lib.oracle_euler_get_padded(self.h, _ptr(out, ctypes.c_double)))
+        return out.reshape(self.nv, self.size(), *([self.pad] * self.rank))
+
+    def cell_centers(self):
+        out = np.zeros(self.size() * self.ni * self.rank, dtype=np.float64)
+        self._check(self.lib.oracle_euler_cell_centers(self.h, _ptr(out, ctypes.c_double)))
+        return out.reshape(self.size(), self.ni, self.rank)
+
+    def halo_exchange(self):
+        self._check(self.lib.oracle_euler_halo_exchange(self.h))
+
+    def set_params(self, gamma, cfl, threads=1):
+        self._check(self.lib.oracle_euler_set_params(self.h, float(gamma), float(cfl), int(threads)))
+
+    def compute_dt(self):
+        return float(self.lib.oracle_euler_compute_dt(self.h))
+
+    def step(self, dt):
+        self._check(self.lib.oracle_euler_step(self.h, float(dt)))
+
+    def advance_batch(self, steps, remaining=np.finfo(np.float64).max):
+        s = ctypes.c_double(0.0)
+        ex = ctypes.c_size_t(0)
+        self._check(self.lib.oracle_euler_advance_batch(self.h, steps, float(remaining), ctypes.byref(s), ctypes.byref(ex)))
+        return s.value, ex.value
+
+    def criterion(self, kind, refine_thr, coarsen_thr, min_level, max_level):
+        out = np.zeros(self.size(), dtype=np.int8)
+        self._check(self.lib.oracle_euler_criterion(self.h, int(kind), float(refine_thr), float(coarsen_thr), int(min_level), int(max_level), _ptr(out, ctypes.c_int8)))
+        return out
+
+
+def euler_prim_to_cons(rank, prim, gamma):
+    """primitiveToConservative per cell; prim[..., rank + 2] -> cons[..., rank + 2]."""
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    cons = np.zeros_like(prim)
+    lib = load()
+    cells = prim.size // (rank + 2)
+    rc = lib.oracle_euler_prim_to_cons(rank, _ptr(prim, ctypes.c_double), _ptr(cons, ctypes.c_double), cells, float(gamma))
+    if rc != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return cons
+
+
+def euler_flux(rank, UL, UR, direction, gamma):
+    """rusanovFlux(UL, UR, flux, direction, gamma)."""
+    UL = np.ascontiguousarray(UL, dtype=np.float64)
+    UR = np.ascontiguousarray(UR, dtype=np.float64)
+    out = np.zeros(rank + 2, dtype=np.float64)
+    lib = load()
+    rc = lib.oracle_euler_flux(rank, _ptr(UL, ctypes.c_double), _ptr(UR, ctypes.c_double), _ptr(out, ctypes.c_double), int(direction), float(gamma))
+    if rc != 0:
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return out
+
+
+def euler_max_speed(rank, U, direction, gamma):
+    """getMaxSpeed of one conservative state."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return float(load().oracle_euler_max_speed(rank, _ptr(U, ctypes.c_double), int(direction), float(gamma)))
 
 
 def build(verbose=False):

@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -370,6 +371,7 @@ public:
     morton<R> const& keys() const { return m_morton; }
     double*       patch(std::size_t i) { return &m_cur[i * FLAT]; }
     double const* patch(std::size_t i) const { return &m_cur[i * FLAT]; }
+    double*       next_patch(std::size_t i) { return &m_next[i * FLAT]; }
     void          set_threads(int t) { m_threads = std::max(1, t); }
     void          set_velocity(std::array<double, 3> v) { m_vel = v; }
     void          set_cfl(double c) { m_cfl = c; }
@@ -1193,6 +1195,235 @@ private:
     int                                   m_threads = 1;
     std::array<double, 3>                 m_vel{ 1.0, 0.5, 0.0 };
     double                                m_cfl = 0.4;
+};
+
+// ---------------------------------------------------------------------------
+// Euler physics (include/solver/EulerPhysics.hpp) and the amr_solver step /
+// dt over its NV = R + 2 conservative fields.  The reference keeps one SoA
+// patch per field in one tree (FieldTags, EulerPhysics.hpp:22-30); here each
+// field lives in its own tree and all trees go through the same regrids, so
+// their topologies stay identical.
+// ---------------------------------------------------------------------------
+template <int R>
+struct euler_physics
+{
+    static constexpr int NV = R + 2;
+
+    // primitiveToConservative (EulerPhysics.hpp:39-62): [rho, u.., p] ->
+    // [rho, rho u.., E], E = p / (gamma - 1) + 0.5 rho |u|^2.
+    static void prim_to_cons(double const* prim, double* cons, double gamma)
+    {
+        const double rho = prim[0];
+        const double p   = prim[R + 1];
+        cons[0]          = rho;
+        double vel_sq    = 0.0;
+        for (int d = 0; d < R; ++d)
+        {
+            const double u = prim[1 + d];
+            cons[1 + d]    = rho * u;
+            vel_sq += u * u;
+        }
+        cons[R + 1] = p / (gamma - 1.0) + 0.5 * rho * vel_sq;
+    }
+
+    // The per-state terms of rusanovFlux / getMaxSpeed
+    // (EulerPhysics.hpp:80-94, 140-152).
+    struct terms
+    {
+        double p, a, u_dir;
+    };
+    static terms state(double const* U, int dir, double gamma)
+    {
+        const double inv_rho = 1.0 / U[0];
+        double       K       = 0.0;
+        for (int d = 0; d < R; ++d) K += U[1 + d] * U[1 + d];
+        K *= 0.5 * inv_rho;
+        const double p = (gamma - 1.0) * (U[R + 1] - K);
+        const double a = std::sqrt(gamma * p * inv_rho);
+        return { p, a, U[1 + dir] * inv_rho };
+    }
+
+    // std::max as the reference calls it: (a < b) ? b : a.
+    static double max_of(double a, double b) { return (a < b) ? b : a; }
+
+    // rusanovFlux (EulerPhysics.hpp:72-125).
+    static void rusanov(double const* UL, double const* UR, double* flux, int dir, double gamma)
+    {
+        const terms  L    = state(UL, dir, gamma);
+        const terms  Rr   = state(UR, dir, gamma);
+        const double smax = max_of(std::abs(L.u_dir) + L.a, std::abs(Rr.u_dir) + Rr.a);
+        flux[0]           = 0.5 * (UL[1 + dir] + UR[1 + dir] - smax * (UR[0] - UL[0]));
+        for (int d = 0; d < R; ++d)
+        {
+            double fl = UL[1 + d] * L.u_dir;
+            double fr = UR[1 + d] * Rr.u_dir;
+            if (d == dir)
+            {
+                fl += L.p;
+                fr += Rr.p;
+            }
+            flux[1 + d] = 0.5 * (fl + fr - smax * (UR[1 + d] - UL[1 + d]));
+        }
+        const double fle = L.u_dir * (UL[R + 1] + L.p);
+        const double fre = Rr.u_dir * (UR[R + 1] + Rr.p);
+        flux[R + 1]      = 0.5 * (fle + fre - smax * (UR[R + 1] - UL[R + 1]));
+    }
+
+    // getMaxSpeed (EulerPhysics.hpp:132-155).
+    static double max_speed(double const* U, int dir, double gamma)
+    {
+        const terms t = state(U, dir, gamma);
+        return std::abs(t.u_dir) + t.a;
+    }
+};
+
+template <int R, int N, int H>
+class euler_system
+{
+  public:
+    using tree_t               = tree<R, N, H>;
+    using phys                 = euler_physics<R>;
+    static constexpr int NV    = phys::NV;
+    static constexpr int FLAT  = tree_t::FLAT;
+    static constexpr int NI    = tree_t::NI;
+
+    euler_system(int depth, std::size_t capacity, std::array<double, 3> lengths)
+    {
+        for (auto& f : m_f) f = std::make_unique<tree_t>(depth, capacity, lengths);
+    }
+
+    tree_t&       field(int v) { return *m_f[v]; }
+    tree_t const& field(int v) const { return *m_f[v]; }
+    std::size_t   size() const { return m_f[0]->size(); }
+    void          set_params(double gamma, double cfl, int threads)
+    {
+        m_gamma = gamma;
+        m_cfl   = cfl;
+        m_threads = std::max(1, threads);
+        for (auto& f : m_f) f->set_threads(m_threads);
+    }
+
+    // ndtree::reconstruct_tree on every field (the flags fix the topology).
+    bool reconstruct(std::int8_t const* flags)
+    {
+        bool changed = false;
+        for (auto& f : m_f) changed = f->reconstruct_tree(flags);
+        return changed;
+    }
+    void halo_exchange_update()
+    {
+        for (auto& f : m_f) f->halo_exchange_update();
+    }
+
+    // compute_time_step_cpu (amr_solver.hpp:355-413) with getMaxSpeed.
+    double compute_time_step() const
+    {
+        std::vector<double> local(size(), std::numeric_limits<double>::max());
+        parallel_for(
+            size(), m_threads,
+            [&](std::size_t p)
+            {
+                const auto cs = m_f[0]->cell_sizes(m_f[0]->key_at(p));
+                double     dt = std::numeric_limits<double>::max();
+                tree_t::interior_for(
+                    [&](typename tree_t::idx_t const& idx)
+                    {
+                        const int l = tree_t::lin(idx);
+                        double    U[NV];
+                        for (int v = 0; v < NV; ++v) U[v] = m_f[v]->patch(p)[l];
+                        for (int d = 0; d < R; ++d)
+                        {
+                            const double speed = phys::max_speed(U, d, m_gamma);
+                            if (speed > 1e-12) dt = std::min(dt, cs[d] / speed);
+                        }
+                    }
+                );
+                local[p] = dt;
+            }
+        );
+        double dt = std::numeric_limits<double>::max();
+        for (double x : local) dt = std::min(dt, x);
+        return m_cfl * dt;
+    }
+
+    // time_step_cpu (amr_solver.hpp:265-353): per cell and physical dim d,
+    // fL = F(U[c - s_d], U[c]), fR = F(U[c], U[c + s_d]);
+    // total[k] -= dt / dx_d * (fR[k] - fL[k]); out = U + total.
+    void time_step_kernel(double dt)
+    {
+        parallel_for(
+            size(), m_threads,
+            [&](std::size_t p)
+            {
+                const auto    cs = m_f[0]->cell_sizes(m_f[0]->key_at(p));
+                double const* in[NV];
+                double*       out[NV];
+                for (int v = 0; v < NV; ++v)
+                {
+                    in[v]  = m_f[v]->patch(p);
+                    out[v] = m_f[v]->next_patch(p);
+                }
+                tree_t::interior_for(
+                    [&](typename tree_t::idx_t const& idx)
+                    {
+                        const int l = tree_t::lin(idx);
+                        double    total[NV] = {};
+                        double    Uc[NV];
+                        for (int v = 0; v < NV; ++v) Uc[v] = in[v][l];
+                        for (int d = 0; d < R; ++d)
+                        {
+                            const int s = tree_t::S[R - 1 - d];
+                            double    UL[NV], UR[NV], fL[NV], fR[NV];
+                            for (int v = 0; v < NV; ++v)
+                            {
+                                UL[v] = in[v][l - s];
+                                UR[v] = in[v][l + s];
+                            }
+                            phys::rusanov(UL, Uc, fL, d, m_gamma);
+                            phys::rusanov(Uc, UR, fR, d, m_gamma);
+                            const double dt_over_dx = dt / cs[d];
+                            for (int k = 0; k < NV; ++k) total[k] -= dt_over_dx * (fR[k] - fL[k]);
+                        }
+                        for (int v = 0; v < NV; ++v) out[v][l] = in[v][l] + total[v];
+                    }
+                );
+            }
+        );
+    }
+
+    void time_step(double dt)
+    {
+        time_step_kernel(dt);
+        for (auto& f : m_f) f->swap_buffers();
+        halo_exchange_update();
+    }
+
+    // advance_batch_async / finish_advance_batch CPU branch
+    // (amr_solver.hpp:221-262).
+    double advance_batch(std::size_t steps, double remaining, std::size_t* executed)
+    {
+        double      acc = 0.0;
+        std::size_t cnt = 0;
+        for (std::size_t s = 0; s < steps; ++s)
+        {
+            if (remaining <= 0.0) break;
+            const double dt      = compute_time_step();
+            const double step_dt = std::min(dt, remaining);
+            if (step_dt <= 0.0) break;
+            time_step(step_dt);
+            acc += step_dt;
+            remaining -= step_dt;
+            ++cnt;
+        }
+        if (executed) *executed = cnt;
+        return acc;
+    }
+
+  private:
+    std::array<std::unique_ptr<tree_t>, NV> m_f;
+    double                                  m_gamma   = 1.4;
+    double                                  m_cfl     = 0.3;
+    int                                     m_threads = 1;
 };
 
 } // namespace oracle

@@ -126,6 +126,102 @@ iface* make(int rank, int n, int h, int depth, std::size_t cap, std::array<doubl
     throw std::invalid_argument("no oracle instance for this (rank, patch size, halo)");
 }
 
+// The Euler system (several fields over one topology).
+struct eiface
+{
+    virtual ~eiface()                                                      = default;
+    virtual int         rank() const                                       = 0;
+    virtual int         fields() const                                     = 0;
+    virtual std::size_t size() const                                       = 0;
+    virtual void        keys(std::uint64_t* out) const                     = 0;
+    virtual void        metadata(oracle::halo_meta* out) const             = 0;
+    virtual bool        reconstruct(std::int8_t const* flags)              = 0;
+    virtual void        set_interior(double const* v)                      = 0;
+    virtual void        get_interior(double* v) const                      = 0;
+    virtual void        get_padded(double* v) const                        = 0;
+    virtual void        cell_centers(double* v) const                      = 0;
+    virtual void        halo()                                             = 0;
+    virtual void        params(double gamma, double cfl, int threads)      = 0;
+    virtual double      dt() const                                         = 0;
+    virtual void        step(double dt)                                    = 0;
+    virtual double      advance(std::size_t steps, double rem, std::size_t* ex) = 0;
+    virtual void        criterion(int kind, double r, double c, int mn, int mx, std::int8_t* f) const = 0;
+};
+
+// Field-major host arrays: all leaves of field 0, then field 1, ...
+template <int R, int N, int H>
+struct eimpl final : eiface
+{
+    using sys_t  = oracle::euler_system<R, N, H>;
+    using tree_t = typename sys_t::tree_t;
+    sys_t e;
+
+    eimpl(int depth, std::size_t cap, std::array<double, 3> L)
+        : e(depth, cap, L)
+    {
+    }
+    int         rank() const override { return R; }
+    int         fields() const override { return sys_t::NV; }
+    std::size_t size() const override { return e.size(); }
+    void        keys(std::uint64_t* out) const override
+    {
+        for (std::size_t i = 0; i < e.size(); ++i) out[i] = e.field(0).key_at(i);
+    }
+    void metadata(oracle::halo_meta* out) const override { e.field(0).export_metadata(out); }
+    bool reconstruct(std::int8_t const* flags) override { return e.reconstruct(flags); }
+    void set_interior(double const* v) override
+    {
+        const std::size_t n = e.size();
+        for (int f = 0; f < sys_t::NV; ++f)
+            for (std::size_t i = 0; i < n; ++i) e.field(f).set_interior(i, v + (f * n + i) * tree_t::NI);
+    }
+    void get_interior(double* v) const override
+    {
+        const std::size_t n = e.size();
+        for (int f = 0; f < sys_t::NV; ++f)
+            for (std::size_t i = 0; i < n; ++i) e.field(f).get_interior(i, v + (f * n + i) * tree_t::NI);
+    }
+    void get_padded(double* v) const override
+    {
+        const std::size_t n = e.size();
+        for (int f = 0; f < sys_t::NV; ++f)
+            for (std::size_t i = 0; i < n; ++i)
+                std::copy_n(e.field(f).patch(i), tree_t::FLAT, v + (f * n + i) * tree_t::FLAT);
+    }
+    void cell_centers(double* v) const override
+    {
+        for (std::size_t i = 0; i < e.size(); ++i) e.field(0).cell_centers(i, v + i * tree_t::NI * R);
+    }
+    void   halo() override { e.halo_exchange_update(); }
+    void   params(double gamma, double cfl, int threads) override { e.set_params(gamma, cfl, threads); }
+    double dt() const override { return e.compute_time_step(); }
+    void   step(double dt) override { e.time_step(dt); }
+    double advance(std::size_t steps, double rem, std::size_t* ex) override { return e.advance_batch(steps, rem, ex); }
+    void   criterion(int kind, double r, double c, int mn, int mx, std::int8_t* f) const override
+    {
+        // the reference's Euler drivers flag on the density field
+        // (benchmark/bench_fvm_solver_integration3D.b.cpp:86-107)
+        if (kind == 0)
+            e.field(0).criterion_max_value(r, c, mn, mx, f);
+        else
+            e.field(0).criterion_gradient(r, c, mn, mx, f);
+    }
+};
+
+eiface* make_euler(int rank, int n, int h, int depth, std::size_t cap, std::array<double, 3> L)
+{
+#define ORACLE_CASE(R_, N_, H_) \
+    if (rank == R_ && n == N_ && h == H_) return new eimpl<R_, N_, H_>(depth, cap, L);
+    ORACLE_CASE(2, 4, 1)
+    ORACLE_CASE(2, 8, 1)
+    ORACLE_CASE(2, 16, 1)
+    ORACLE_CASE(2, 64, 1)
+    ORACLE_CASE(3, 4, 1)
+    ORACLE_CASE(3, 8, 1)
+#undef ORACLE_CASE
+    throw std::invalid_argument("no Euler oracle instance for this (rank, patch size, halo)");
+}
+
 template <typename Fn>
 int guarded(Fn&& fn)
 {
@@ -229,6 +325,111 @@ int oracle_advance_batch(void* o, std::size_t steps, double remaining, double* d
 int oracle_criterion(void* o, int kind, double r, double c, int mn, int mx, std::int8_t* flags)
 {
     return guarded([&] { static_cast<iface*>(o)->criterion(kind, r, c, mn, mx, flags); });
+}
+
+// ---- Euler system -------------------------------------------------------
+
+void* oracle_euler_create(int rank, int n, int h, int depth, std::size_t capacity, double const* lengths)
+{
+    eiface* p = nullptr;
+    guarded([&] { p = make_euler(rank, n, h, depth, capacity, { lengths[0], lengths[1], lengths[2] }); });
+    return p;
+}
+void        oracle_euler_destroy(void* o) { delete static_cast<eiface*>(o); }
+std::size_t oracle_euler_size(void* o) { return static_cast<eiface*>(o)->size(); }
+int         oracle_euler_fields(void* o) { return static_cast<eiface*>(o)->fields(); }
+int         oracle_euler_keys(void* o, std::uint64_t* out)
+{
+    return guarded([&] { static_cast<eiface*>(o)->keys(out); });
+}
+int oracle_euler_metadata(void* o, void* out)
+{
+    return guarded([&] { static_cast<eiface*>(o)->metadata(static_cast<oracle::halo_meta*>(out)); });
+}
+int oracle_euler_reconstruct(void* o, std::int8_t const* flags, int* changed)
+{
+    return guarded(
+        [&]
+        {
+            const bool c = static_cast<eiface*>(o)->reconstruct(flags);
+            if (changed) *changed = c ? 1 : 0;
+        }
+    );
+}
+int oracle_euler_set_interior(void* o, double const* v)
+{
+    return guarded([&] { static_cast<eiface*>(o)->set_interior(v); });
+}
+int oracle_euler_get_interior(void* o, double* v)
+{
+    return guarded([&] { static_cast<eiface*>(o)->get_interior(v); });
+}
+int oracle_euler_get_padded(void* o, double* v)
+{
+    return guarded([&] { static_cast<eiface*>(o)->get_padded(v); });
+}
+int oracle_euler_cell_centers(void* o, double* v)
+{
+    return guarded([&] { static_cast<eiface*>(o)->cell_centers(v); });
+}
+int oracle_euler_halo_exchange(void* o)
+{
+    return guarded([&] { static_cast<eiface*>(o)->halo(); });
+}
+int oracle_euler_set_params(void* o, double gamma, double cfl, int threads)
+{
+    return guarded([&] { static_cast<eiface*>(o)->params(gamma, cfl, threads); });
+}
+double oracle_euler_compute_dt(void* o) { return static_cast<eiface*>(o)->dt(); }
+int    oracle_euler_step(void* o, double dt)
+{
+    return guarded([&] { static_cast<eiface*>(o)->step(dt); });
+}
+int oracle_euler_advance_batch(void* o, std::size_t steps, double remaining, double* dt_sum, std::size_t* executed)
+{
+    return guarded([&] { *dt_sum = static_cast<eiface*>(o)->advance(steps, remaining, executed); });
+}
+int oracle_euler_criterion(void* o, int kind, double r, double c, int mn, int mx, std::int8_t* flags)
+{
+    return guarded([&] { static_cast<eiface*>(o)->criterion(kind, r, c, mn, mx, flags); });
+}
+
+// Pointwise Euler functions (for the closed-form checks).
+int oracle_euler_prim_to_cons(int rank, double const* prim, double* cons, std::size_t cells, double gamma)
+{
+    return guarded(
+        [&]
+        {
+            const int nv = rank + 2;
+            for (std::size_t c = 0; c < cells; ++c)
+            {
+                if (rank == 2)
+                    oracle::euler_physics<2>::prim_to_cons(prim + c * nv, cons + c * nv, gamma);
+                else if (rank == 3)
+                    oracle::euler_physics<3>::prim_to_cons(prim + c * nv, cons + c * nv, gamma);
+                else
+                    throw std::invalid_argument("rank must be 2 or 3");
+            }
+        }
+    );
+}
+int oracle_euler_flux(int rank, double const* UL, double const* UR, double* flux, int dir, double gamma)
+{
+    return guarded(
+        [&]
+        {
+            if (rank == 2)
+                oracle::euler_physics<2>::rusanov(UL, UR, flux, dir, gamma);
+            else if (rank == 3)
+                oracle::euler_physics<3>::rusanov(UL, UR, flux, dir, gamma);
+            else
+                throw std::invalid_argument("rank must be 2 or 3");
+        }
+    );
+}
+double oracle_euler_max_speed(int rank, double const* U, int dir, double gamma)
+{
+    return rank == 2 ? oracle::euler_physics<2>::max_speed(U, dir, gamma) : oracle::euler_physics<3>::max_speed(U, dir, gamma);
 }
 
 } // extern "C"
