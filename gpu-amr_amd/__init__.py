@@ -89,6 +89,10 @@ _SIGS = {
     "amr_solver_set_step_variant": [ctypes.c_void_p, ctypes.c_int32],
     "amr_solver_get_step_variant": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)],
     "amr_solver_autotune": [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
+    "amr_tree_fields": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)],
+    "amr_euler_create": [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)],
+    "amr_euler_configure": [ctypes.c_void_p, ctypes.c_double, ctypes.c_double],
+    "amr_euler_set_primitive": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
     "amr_solver_set_step_fusion": [ctypes.c_void_p, ctypes.c_int32],
     "amr_solver_get_step_fusion": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
     "amr_ctx_synchronize": [ctypes.c_void_p],
@@ -171,7 +175,7 @@ def decode_keys(keys, rank):
 class NdTree:
     """Periodic quad-/octree of leaf patches on one GPU (ndtree.hpp:94-2157)."""
 
-    def __init__(self, rank, max_depth, patch_size, halo_width, capacity, domain_lengths=(1.0, 1.0, 1.0)):
+    def __init__(self, rank, max_depth, patch_size, halo_width, capacity, domain_lengths=(1.0, 1.0, 1.0), _euler=None):
         self.lib = load()
         cfg = TreeConfig()
         cfg.rank, cfg.max_depth, cfg.patch_size, cfg.halo_width = rank, max_depth, patch_size, halo_width
@@ -180,8 +184,15 @@ class NdTree:
         for i in range(3):
             cfg.domain_length[i] = L[i]
         h = ctypes.c_void_p()
-        _check(self.lib.amr_tree_create(ctypes.byref(cfg), ctypes.byref(h)))
+        if _euler is None:
+            _check(self.lib.amr_tree_create(ctypes.byref(cfg), ctypes.byref(h)))
+        else:
+            gamma, cfl = _euler
+            _check(self.lib.amr_euler_create(ctypes.byref(cfg), float(gamma), float(cfl), ctypes.byref(h)))
         self.h = h
+        nv = ctypes.c_int32()
+        _check(self.lib.amr_tree_fields(self.h, ctypes.byref(nv)))
+        self.fields = int(nv.value)
         self.rank, self.depth, self.n, self.halo = rank, max_depth, patch_size, halo_width
         self.lengths = tuple(L[:rank])
         self.ni = patch_size**rank
@@ -241,20 +252,26 @@ class NdTree:
         return out
 
     # -- fields -----------------------------------------------------------
+    def _field_shape(self, per_leaf):
+        lead = (self.fields,) if self.fields > 1 else ()
+        return lead + (self.size(),) + (per_leaf,) * self.rank
+
     def set_interior(self, values):
+        """Interiors of all leaves; with several fields, field-major
+        [fields, leaves, n, ...]."""
         v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
         _check(self.lib.amr_tree_set_interior(self.h, _vp(v), v.size))
 
     def get_interior(self):
-        out = np.zeros(self.size() * self.ni, dtype=np.float64)
+        out = np.zeros(self.fields * self.size() * self.ni, dtype=np.float64)
         _check(self.lib.amr_tree_get_interior(self.h, _vp(out), out.size))
-        return out.reshape((self.size(),) + (self.n,) * self.rank)
+        return out.reshape(self._field_shape(self.n))
 
     def get_patches(self):
         """Padded patches after halo_exchange_update (corners 0)."""
-        out = np.zeros(self.size() * self.flat, dtype=np.float64)
+        out = np.zeros(self.fields * self.size() * self.flat, dtype=np.float64)
         _check(self.lib.amr_tree_get_patches(self.h, _vp(out), out.size))
-        return out.reshape((self.size(),) + (self.pad,) * self.rank)
+        return out.reshape(self._field_shape(self.pad))
 
     def get_patch(self, i):
         return self.get_patches()[i]
@@ -377,6 +394,53 @@ class AmrSolver:
         e, a = ctypes.c_int32(), ctypes.c_int32()
         _check(self.tree.lib.amr_solver_get_step_fusion(self.tree.h, ctypes.byref(e), ctypes.byref(a)))
         return bool(e.value), bool(a.value)
+
+
+class EulerSolver:
+    """amr_solver<tree, physics, EulerPhysics<rank>, rank>(capacity, gamma,
+    cfl) (amr_solver.hpp:28-97, EulerPhysics.hpp) with its tree: the cells
+    carry [rho, rho u, rho v, (rho w), E]; `tree` is the NdTree view of the
+    same context (regrids, keys, fields [fields, leaves, n, ...])."""
+
+    def __init__(self, rank, max_depth, patch_size, halo_width, capacity, domain_lengths=(1.0, 1.0, 1.0),
+                 gamma=1.4, cfl=0.3):
+        self.tree = NdTree(rank, max_depth, patch_size, halo_width, capacity, domain_lengths, _euler=(gamma, cfl))
+        self.gamma, self.cfl = gamma, cfl
+
+    def get_tree(self):
+        return self.tree
+
+    def configure(self, gamma, cfl):
+        _check(self.tree.lib.amr_euler_configure(self.tree.h, float(gamma), float(cfl)))
+        self.gamma, self.cfl = gamma, cfl
+
+    def initialize(self, ic):
+        """initialize(ic) (amr_solver.hpp:105-145): ic(coords[..., rank]) ->
+        primitive values [..., rank + 2] = [rho, u, v, (w), p], converted to
+        conservative variables by primitiveToConservative."""
+        cen = self.tree.cell_centers()
+        prim = np.asarray(ic(cen), dtype=np.float64)  # [leaves, n^rank, fields]
+        v = np.ascontiguousarray(np.moveaxis(prim, -1, 0)).reshape(-1)
+        _check(self.tree.lib.amr_euler_set_primitive(self.tree.h, _vp(v), v.size))
+
+    def compute_dt(self):
+        dt = ctypes.c_double()
+        _check(self.tree.lib.amr_solver_compute_dt(self.tree.h, ctypes.byref(dt)))
+        return dt.value
+
+    def advance_batch_async(self, step_count, remaining_time=np.finfo(np.float64).max):
+        _check(self.tree.lib.amr_solver_advance_batch_async(self.tree.h, int(step_count), float(remaining_time)))
+
+    def finish_advance_batch(self):
+        """(summed dt, executed steps)."""
+        s, n = ctypes.c_double(), ctypes.c_uint64()
+        _check(self.tree.lib.amr_solver_finish_advance_batch(self.tree.h, ctypes.byref(s), ctypes.byref(n)))
+        return s.value, int(n.value)
+
+    def advance(self):
+        dt = ctypes.c_double()
+        _check(self.tree.lib.amr_solver_advance(self.tree.h, ctypes.byref(dt)))
+        return dt.value
 
 
 class Batch:

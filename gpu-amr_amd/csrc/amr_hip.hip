@@ -8,6 +8,7 @@
 #include "amr_hip.h"
 #include "device_common.hpp"
 #include "kernels_adjoint.hpp"
+#include "kernels_euler.hpp"
 #include "kernels_fused.hpp"
 #include "kernels_step.hpp"
 #include "kernels_tree.hpp"
@@ -95,6 +96,11 @@ struct ops
                        hipStream_t);
     // two fused steps of a single-level tree (null: not built for the shape)
     void (*step2)(const double*, double*, step2_params const&, uint32_t, hipStream_t);
+    // Euler fields (field-major, stride fstride doubles between fields)
+    void (*euler_dt)(const double*, size_t, const uint64_t*, euler_params const&, euler_scalars*, uint32_t,
+                     hipStream_t);
+    void (*euler_step)(const double*, double*, size_t, const uint64_t*, const face_link*, const uint32_t*,
+                       euler_params const&, const euler_scalars*, uint32_t, hipStream_t);
 };
 
 template <int R, int N, int H>
@@ -218,21 +224,40 @@ struct instance
             hipLaunchKernelGGL(k_step2_3d8_uniform<false>, dim3(xcd_grid(n)), dim3(64), 0, s, u, un, prm, n);
         check_launch("k_step2");
     }
+    static void euler_dt(const double* u, size_t fs, const uint64_t* keys, euler_params const& prm, euler_scalars* sc,
+                         uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_euler_dt<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, fs, keys, prm, sc, n);
+        check_launch("k_euler_dt");
+    }
+    static void euler_step(const double* u, double* un, size_t fs, const uint64_t* keys, const face_link* links,
+                           const uint32_t* metas, euler_params const& prm, const euler_scalars* sc, uint32_t n,
+                           hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_euler_step<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links, metas,
+                           prm, sc, n);
+        check_launch("k_euler_step");
+    }
     static constexpr ops table()
     {
         constexpr bool fused = R == 3 && N == 8;
         return ops{ R,       N,          H,       &step,     &variant_count, &variant_name, &variant_needs,
-                    &padded, &criterion, &emit, &adj_step, &adj_regrid, fused ? &step2 : nullptr };
+                    &padded, &criterion, &emit, &adj_step, &adj_regrid, fused ? &step2 : nullptr,
+                    &euler_dt, &euler_step };
     }
 };
 
 // Patch shapes built into this library: the configurations' 16^2 and 8^3
 // patches with one ghost layer, the reference test's 10^2 and 50^2 patches
-// with two, and small shapes for exhaustive parity tests.
+// with two, the reference's 2D Euler benchmark's 64^2 patches
+// (benchmark/bench_fvm_solver_integration.b.cpp:35-37), and small shapes for
+// exhaustive parity tests.
 const ops kOps[] = {
     instance<2, 16, 1>::table(), instance<3, 8, 1>::table(), instance<2, 10, 2>::table(),
     instance<2, 50, 2>::table(), instance<2, 4, 1>::table(), instance<2, 4, 2>::table(),
-    instance<2, 8, 1>::table(),  instance<3, 4, 1>::table(),
+    instance<2, 8, 1>::table(),  instance<3, 4, 1>::table(), instance<2, 64, 1>::table(),
 };
 
 ops const* find_ops(int rank, int n, int h)
@@ -493,14 +518,24 @@ struct amr_ctx
     uint64_t        sweeps        = 0; // step launches (sets the sweep direction)
     bool            fuse          = true; // two steps per launch on single-level trees
 
+    // fields: 1 (advection) or rank + 2 conservative Euler variables,
+    // field-major (field v of slot s at u + v * fstride() + s * ni)
+    int             nvar  = 1;
+    double          gamma = 1.4;
+    euler_scalars*  esc   = nullptr; // device scalars of an Euler batch
+    euler_scalars*  h_esc = nullptr; // pinned copy
+    size_t          fstride() const { return static_cast<size_t>(cap) * static_cast<size_t>(ni); }
+    bool            euler() const { return nvar > 1; }
+
     ~amr_ctx()
     {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : { (void*)u[0], (void*)u[1], (void*)keys[0], (void*)keys[1], (void*)links, (void*)metas,
                          (void*)padded, (void*)flags, (void*)mark, (void*)head, (void*)counts, (void*)pos,
-                         (void*)block_sums, (void*)summary })
+                         (void*)block_sums, (void*)summary, (void*)esc })
             if (p) (void)hipFree(p);
         if (h_summary) (void)hipHostFree(h_summary);
+        if (h_esc) (void)hipHostFree(h_esc);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -573,7 +608,7 @@ struct amr_ctx
     // tile the domain, so n == 2^(rank * max_level) holds exactly then).
     bool fusion_applies() const
     {
-        return op->step2 && rank * max_level < 32 && n == (uint64_t{ 1 } << (rank * max_level));
+        return nvar == 1 && op->step2 && rank * max_level < 32 && n == (uint64_t{ 1 } << (rank * max_level));
     }
     bool fusable() const { return fuse && fusion_applies(); }
 
@@ -627,6 +662,57 @@ struct amr_ctx
         return variant;
     }
 
+    // ---- Euler (EulerPhysics<R>, amr_solver.hpp:155-413) -----------------
+    euler_params make_euler_params() const
+    {
+        euler_params prm{};
+        prm.gamma = gamma;
+        for (int l = 0; l <= depth; ++l)
+            for (int d = 0; d < rank; ++d) prm.cs[l][d] = cell_size(l, d);
+        return prm;
+    }
+
+    // advance_batch_async: per step, the dt reduction, the dt logic and the
+    // update are queued back to back; dt never visits the host.
+    void euler_advance(uint64_t steps, double remaining)
+    {
+        ensure_usable();
+        euler_params prm = make_euler_params();
+        hipLaunchKernelGGL(k_euler_begin, dim3(1), dim3(1), 0, stream, esc, remaining);
+        check_launch("k_euler_begin");
+        const uint32_t nn = static_cast<uint32_t>(n);
+        for (uint64_t s = 0; s < steps; ++s)
+        {
+            op->euler_dt(u[cur], fstride(), keys[kc], prm, esc, nn, stream);
+            hipLaunchKernelGGL(k_euler_finalize, dim3(1), dim3(1), 0, stream, esc, cfl);
+            check_launch("k_euler_finalize");
+            prm.reverse = static_cast<uint32_t>(sweeps++ & 1);
+            op->euler_step(u[cur], u[1 - cur], fstride(), keys[kc], links, metas, prm, esc, nn, stream);
+            cur = 1 - cur;
+        }
+    }
+
+    euler_scalars euler_read()
+    {
+        AMR_HIP_CHECK(hipMemcpyAsync(h_esc, esc, sizeof(euler_scalars), hipMemcpyDeviceToHost, stream));
+        sync();
+        return *h_esc;
+    }
+
+    // compute_time_step_cpu: cfl * min over cells and dims of dx / speed.
+    double euler_dt_now()
+    {
+        ensure_usable();
+        const euler_params prm = make_euler_params();
+        hipLaunchKernelGGL(k_euler_begin, dim3(1), dim3(1), 0, stream, esc, 0.0);
+        check_launch("k_euler_begin");
+        op->euler_dt(u[cur], fstride(), keys[kc], prm, esc, static_cast<uint32_t>(n), stream);
+        const euler_scalars e = euler_read();
+        double raw;
+        std::memcpy(&raw, &e.dtmin_bits, sizeof(raw));
+        return cfl * raw;
+    }
+
     void scan(uint32_t count) { exclusive_scan(counts, pos, count, block_sums, summary, stream); }
 
     // ndtree::reconstruct_tree (ndtree.hpp:1249-1271) with the flags already
@@ -670,7 +756,9 @@ struct amr_ctx
         if (s.err & 2u) throw amr_error(AMR_ERR_UNSUPPORTED, "coarsening to the root level is not supported");
         if (s.n_refine == 0 && s.n_coarsen == 0) return false;
         if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
-        op->emit(keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, stream);
+        for (int v = 0; v < nvar; ++v)
+            op->emit(keys[kc], u[cur] + v * fstride(), nn, depth, mark, head, counts, pos, keys[1 - kc],
+                     u[1 - cur] + v * fstride(), summary, stream);
         cur = 1 - cur;
         kc  = 1 - kc;
         n   = s.total;
@@ -1065,6 +1153,56 @@ extern "C"
 
 const char* amr_last_error(void) { return g_err.c_str(); }
 
+// ndtree(capacity) with `nvar` fields per cell (ndtree.hpp:320-425).
+static std::unique_ptr<amr_ctx> create_ctx(const amr_tree_config* cfg, int nvar)
+{
+    if (cfg->rank != 2 && cfg->rank != 3) throw amr_error(AMR_ERR_ARGUMENT, "rank must be 2 or 3");
+    if (cfg->max_depth < 0 || cfg->max_depth > kMaxDepth || cfg->rank * cfg->max_depth + 6 > 64)
+        throw amr_error(AMR_ERR_ARGUMENT, "max_depth out of range");
+    if (cfg->capacity < 1 || cfg->capacity > 0xFFFFFFFFull / 8)
+        throw amr_error(AMR_ERR_ARGUMENT, "capacity out of range");
+    ops const* op = find_ops(cfg->rank, cfg->patch_size, cfg->halo_width);
+    if (!op) throw amr_error(AMR_ERR_UNSUPPORTED, "no kernels built for this (rank, patch_size, halo_width)");
+    auto c   = std::make_unique<amr_ctx>();
+    c->cfg   = *cfg;
+    c->op    = op;
+    c->rank  = cfg->rank;
+    c->depth = cfg->max_depth;
+    c->ndir  = 2 * cfg->rank;
+    c->cap   = cfg->capacity;
+    c->nvar  = nvar;
+    c->ni    = cfg->rank == 2 ? cfg->patch_size * cfg->patch_size
+                              : cfg->patch_size * cfg->patch_size * cfg->patch_size;
+    const int P = cfg->patch_size + 2 * cfg->halo_width;
+    c->flat     = cfg->rank == 2 ? P * P : P * P * P;
+    AMR_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const size_t cap = cfg->capacity;
+    for (int b = 0; b < 2; ++b)
+    {
+        c->u[b]    = dev_alloc<double>(cap * c->ni * nvar);
+        c->keys[b] = dev_alloc<uint64_t>(cap);
+        AMR_HIP_CHECK(hipMemsetAsync(c->u[b], 0, cap * c->ni * nvar * sizeof(double), c->stream));
+    }
+    c->links      = dev_alloc<face_link>(cap * c->ndir);
+    c->metas      = dev_alloc<uint32_t>(cap * c->ndir);
+    c->flags      = dev_alloc<int8_t>(cap);
+    c->mark       = dev_alloc<uint8_t>(cap);
+    c->head       = dev_alloc<uint8_t>(cap);
+    c->counts     = dev_alloc<uint32_t>(cap);
+    c->pos        = dev_alloc<uint32_t>(cap);
+    c->block_sums = dev_alloc<uint32_t>(cap / kScanBlock + 2);
+    c->summary    = dev_alloc<regrid_summary>(1);
+    AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_summary), sizeof(regrid_summary), 0));
+    // the periodic root (ndtree.hpp:411-421)
+    const uint64_t root = 0;
+    AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
+    c->n         = 1;
+    c->max_level = 0;
+    if (run_lists::wanted(op)) c->runs.allocate(cap);
+    c->build_table();
+    return c;
+}
+
 int amr_tree_create(const amr_tree_config* cfg, amr_ctx** out)
 {
     return guarded(
@@ -1072,49 +1210,24 @@ int amr_tree_create(const amr_tree_config* cfg, amr_ctx** out)
         {
             if (!cfg || !out) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             *out = nullptr;
-            if (cfg->rank != 2 && cfg->rank != 3) throw amr_error(AMR_ERR_ARGUMENT, "rank must be 2 or 3");
-            if (cfg->max_depth < 0 || cfg->max_depth > kMaxDepth || cfg->rank * cfg->max_depth + 6 > 64)
-                throw amr_error(AMR_ERR_ARGUMENT, "max_depth out of range");
-            if (cfg->capacity < 1 || cfg->capacity > 0xFFFFFFFFull / 8)
-                throw amr_error(AMR_ERR_ARGUMENT, "capacity out of range");
-            ops const* op = find_ops(cfg->rank, cfg->patch_size, cfg->halo_width);
-            if (!op) throw amr_error(AMR_ERR_UNSUPPORTED, "no kernels built for this (rank, patch_size, halo_width)");
-            auto c   = std::make_unique<amr_ctx>();
-            c->cfg   = *cfg;
-            c->op    = op;
-            c->rank  = cfg->rank;
-            c->depth = cfg->max_depth;
-            c->ndir  = 2 * cfg->rank;
-            c->cap   = cfg->capacity;
-            c->ni    = cfg->rank == 2 ? cfg->patch_size * cfg->patch_size
-                                      : cfg->patch_size * cfg->patch_size * cfg->patch_size;
-            const int P = cfg->patch_size + 2 * cfg->halo_width;
-            c->flat     = cfg->rank == 2 ? P * P : P * P * P;
-            AMR_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-            const size_t cap = cfg->capacity;
-            for (int b = 0; b < 2; ++b)
-            {
-                c->u[b]    = dev_alloc<double>(cap * c->ni);
-                c->keys[b] = dev_alloc<uint64_t>(cap);
-                AMR_HIP_CHECK(hipMemsetAsync(c->u[b], 0, cap * c->ni * sizeof(double), c->stream));
-            }
-            c->links      = dev_alloc<face_link>(cap * c->ndir);
-            c->metas      = dev_alloc<uint32_t>(cap * c->ndir);
-            c->flags      = dev_alloc<int8_t>(cap);
-            c->mark       = dev_alloc<uint8_t>(cap);
-            c->head       = dev_alloc<uint8_t>(cap);
-            c->counts     = dev_alloc<uint32_t>(cap);
-            c->pos        = dev_alloc<uint32_t>(cap);
-            c->block_sums = dev_alloc<uint32_t>(cap / kScanBlock + 2);
-            c->summary    = dev_alloc<regrid_summary>(1);
-            AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_summary), sizeof(regrid_summary), 0));
-            // the periodic root (ndtree.hpp:411-421)
-            const uint64_t root = 0;
-            AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
-            c->n         = 1;
-            c->max_level = 0;
-            if (run_lists::wanted(op)) c->runs.allocate(cap);
-            c->build_table();
+            *out = create_ctx(cfg, 1).release();
+        }
+    );
+}
+
+int amr_euler_create(const amr_tree_config* cfg, double gamma, double cfl, amr_ctx** out)
+{
+    return guarded(
+        [&]
+        {
+            if (!cfg || !out) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *out = nullptr;
+            if (!(gamma > 1.0) || !(cfl > 0.0)) throw amr_error(AMR_ERR_ARGUMENT, "need gamma > 1 and cfl > 0");
+            auto c   = create_ctx(cfg, cfg->rank + 2);
+            c->gamma = gamma;
+            c->cfl   = cfl;
+            c->esc   = dev_alloc<euler_scalars>(1);
+            AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_esc), sizeof(euler_scalars), 0));
             *out = c.release();
         }
     );
@@ -1234,9 +1347,11 @@ int amr_tree_set_interior(amr_ctx* ctx, const double* host, uint64_t count)
         [&]
         {
             if (!ctx || (!host && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            if (count != ctx->n * ctx->ni) throw amr_error(AMR_ERR_ARGUMENT, "count must be size * n^rank");
-            AMR_HIP_CHECK(hipMemcpyAsync(ctx->u[ctx->cur], host, count * sizeof(double), hipMemcpyHostToDevice,
-                                         ctx->stream));
+            const uint64_t per = ctx->n * ctx->ni;
+            if (count != per * ctx->nvar) throw amr_error(AMR_ERR_ARGUMENT, "count must be fields * size * n^rank");
+            for (int v = 0; v < ctx->nvar; ++v)
+                AMR_HIP_CHECK(hipMemcpyAsync(ctx->u[ctx->cur] + v * ctx->fstride(), host + v * per, per * sizeof(double),
+                                             hipMemcpyHostToDevice, ctx->stream));
             ctx->sync();
         }
     );
@@ -1248,9 +1363,11 @@ int amr_tree_get_interior(amr_ctx* ctx, double* host, uint64_t count)
         [&]
         {
             if (!ctx || (!host && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            if (count < ctx->n * ctx->ni) throw amr_error(AMR_ERR_ARGUMENT, "buffer smaller than size * n^rank");
-            AMR_HIP_CHECK(hipMemcpyAsync(host, ctx->u[ctx->cur], ctx->n * ctx->ni * sizeof(double),
-                                         hipMemcpyDeviceToHost, ctx->stream));
+            const uint64_t per = ctx->n * ctx->ni;
+            if (count < per * ctx->nvar) throw amr_error(AMR_ERR_ARGUMENT, "buffer smaller than fields * size * n^rank");
+            for (int v = 0; v < ctx->nvar; ++v)
+                AMR_HIP_CHECK(hipMemcpyAsync(host + v * per, ctx->u[ctx->cur] + v * ctx->fstride(), per * sizeof(double),
+                                             hipMemcpyDeviceToHost, ctx->stream));
             ctx->sync();
         }
     );
@@ -1259,8 +1376,10 @@ int amr_tree_get_interior(amr_ctx* ctx, double* host, uint64_t count)
 static void materialize_padded(amr_ctx* ctx)
 {
     ctx->ensure_usable();
-    if (!ctx->padded) ctx->padded = dev_alloc<double>(ctx->cap * ctx->flat);
-    ctx->op->padded(ctx->u[ctx->cur], ctx->padded, ctx->links, ctx->metas, static_cast<uint32_t>(ctx->n), ctx->stream);
+    if (!ctx->padded) ctx->padded = dev_alloc<double>(ctx->cap * ctx->flat * ctx->nvar);
+    for (int v = 0; v < ctx->nvar; ++v)
+        ctx->op->padded(ctx->u[ctx->cur] + v * ctx->fstride(), ctx->padded + v * ctx->cap * ctx->flat, ctx->links,
+                        ctx->metas, static_cast<uint32_t>(ctx->n), ctx->stream);
 }
 
 int amr_tree_get_patches(amr_ctx* ctx, double* host, uint64_t count)
@@ -1269,10 +1388,13 @@ int amr_tree_get_patches(amr_ctx* ctx, double* host, uint64_t count)
         [&]
         {
             if (!ctx || (!host && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            if (count < ctx->n * ctx->flat) throw amr_error(AMR_ERR_ARGUMENT, "buffer smaller than size * (n+2h)^rank");
+            const uint64_t per = ctx->n * ctx->flat;
+            if (count < per * ctx->nvar)
+                throw amr_error(AMR_ERR_ARGUMENT, "buffer smaller than fields * size * (n+2h)^rank");
             materialize_padded(ctx);
-            AMR_HIP_CHECK(hipMemcpyAsync(host, ctx->padded, ctx->n * ctx->flat * sizeof(double), hipMemcpyDeviceToHost,
-                                         ctx->stream));
+            for (int v = 0; v < ctx->nvar; ++v)
+                AMR_HIP_CHECK(hipMemcpyAsync(host + v * per, ctx->padded + v * ctx->cap * ctx->flat, per * sizeof(double),
+                                             hipMemcpyDeviceToHost, ctx->stream));
             ctx->sync();
         }
     );
@@ -1330,6 +1452,7 @@ int amr_solver_configure(amr_ctx* ctx, const double velocity[3], double cfl)
         {
             if (!ctx || !velocity) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             if (!(cfl > 0.0)) throw amr_error(AMR_ERR_ARGUMENT, "cfl must be positive");
+            if (ctx->euler()) throw amr_error(AMR_ERR_UNSUPPORTED, "an Euler context has no advection velocity");
             for (int d = 0; d < 3; ++d) ctx->vel[d] = velocity[d];
             ctx->cfl = cfl;
         }
@@ -1342,7 +1465,7 @@ int amr_solver_compute_dt(amr_ctx* ctx, double* dt)
         [&]
         {
             if (!ctx || !dt) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            *dt = ctx->compute_dt();
+            *dt = ctx->euler() ? ctx->euler_dt_now() : ctx->compute_dt();
         }
     );
 }
@@ -1358,6 +1481,13 @@ int amr_solver_advance_batch_async(amr_ctx* ctx, uint64_t step_count, double rem
             if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             ctx->pending_dt    = 0.0;
             ctx->pending_steps = 0;
+            if (ctx->euler())
+            {
+                // dt depends on the state: the device computes it every step
+                // and the host reads the sums in finish_advance_batch
+                ctx->euler_advance(step_count, remaining_time);
+                return;
+            }
             double remaining   = remaining_time;
             uint64_t s         = 0;
             while (s < step_count)
@@ -1396,6 +1526,12 @@ int amr_solver_finish_advance_batch(amr_ctx* ctx, double* dt_sum, uint64_t* exec
         {
             if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             ctx->sync();
+            if (ctx->euler())
+            {
+                const euler_scalars e = ctx->euler_read();
+                ctx->pending_dt       = e.acc;
+                ctx->pending_steps    = e.executed;
+            }
             if (dt_sum) *dt_sum = ctx->pending_dt;
             if (executed_steps) *executed_steps = ctx->pending_steps;
         }
@@ -1483,9 +1619,64 @@ int amr_solver_autotune(amr_ctx* ctx, uint32_t trial_steps, int32_t* chosen, flo
     return guarded(
         [&]
         {
+            if (ctx && ctx->euler()) throw amr_error(AMR_ERR_UNSUPPORTED, "the step variants are advection kernels");
             if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             const int v = ctx->autotune(trial_steps, best_us);
             if (chosen) *chosen = v;
+        }
+    );
+}
+
+int amr_tree_fields(const amr_ctx* ctx, int32_t* fields)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !fields) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            *fields = ctx->nvar;
+        }
+    );
+}
+
+int amr_euler_configure(amr_ctx* ctx, double gamma, double cfl)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (!ctx->euler()) throw amr_error(AMR_ERR_UNSUPPORTED, "not an Euler context");
+            if (!(gamma > 1.0) || !(cfl > 0.0)) throw amr_error(AMR_ERR_ARGUMENT, "need gamma > 1 and cfl > 0");
+            ctx->gamma = gamma;
+            ctx->cfl   = cfl;
+        }
+    );
+}
+
+int amr_euler_set_primitive(amr_ctx* ctx, const double* host_prim, uint64_t count)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host_prim && count)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (!ctx->euler()) throw amr_error(AMR_ERR_UNSUPPORTED, "not an Euler context");
+            const uint64_t per = ctx->n * ctx->ni;
+            const int      nv  = ctx->nvar;
+            if (count != per * nv) throw amr_error(AMR_ERR_ARGUMENT, "count must be fields * size * n^rank");
+            std::vector<double> cons(count);
+            double              prim[5], out[5];
+            for (uint64_t c = 0; c < per; ++c)
+            {
+                for (int v = 0; v < nv; ++v) prim[v] = host_prim[v * per + c];
+                if (ctx->rank == 2)
+                    euler_flux<2>::prim_to_cons(prim, out, ctx->gamma);
+                else
+                    euler_flux<3>::prim_to_cons(prim, out, ctx->gamma);
+                for (int v = 0; v < nv; ++v) cons[v * per + c] = out[v];
+            }
+            for (int v = 0; v < nv; ++v)
+                AMR_HIP_CHECK(hipMemcpyAsync(ctx->u[ctx->cur] + v * ctx->fstride(), cons.data() + v * per,
+                                             per * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            ctx->sync();
         }
     );
 }
@@ -1495,6 +1686,9 @@ int amr_batch_create(amr_ctx* const* trees, uint32_t count, amr_batch** out)
     return guarded(
         [&]
         {
+            for (uint32_t t = 0; trees && t < count; ++t)
+                if (trees[t] && trees[t]->euler())
+                    throw amr_error(AMR_ERR_UNSUPPORTED, "batches hold advection trees only");
             if (!trees || !out || count == 0) throw amr_error(AMR_ERR_ARGUMENT, "null argument or empty batch");
             if (count > 65535) throw amr_error(AMR_ERR_ARGUMENT, "at most 65535 instances per batch");
             *out = nullptr;
@@ -1672,6 +1866,7 @@ int amr_tape_record(amr_ctx* ctx, const amr_criterion* criterion, uint64_t steps
     return guarded(
         [&]
         {
+            if (ctx && ctx->euler()) throw amr_error(AMR_ERR_UNSUPPORTED, "the adjoint tape records advection solves");
             if (!ctx || !out) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             *out = nullptr;
             ctx->ensure_usable();

@@ -1,0 +1,280 @@
+// kernels_euler.hpp -- compressible Euler equations through the same
+// patch-AMR step: the Rusanov (local Lax-Friedrichs) flux of
+// include/solver/EulerPhysics.hpp, the time step of amr_solver::time_step_cpu
+// over R + 2 conservative fields, and the CFL time step reduced on the
+// device.
+//
+// Fields are stored one after another (field-major): field v of leaf slot s
+// starts at u + v * fstride + s * n^R, so the ghost rules, the regrid
+// transfer and the criteria of the scalar path apply to each field as they
+// are.  Every expression keeps the reference's order (no contraction), so
+// results are bit-identical to the CPU oracle.
+#pragma once
+
+#include "kernels_step.hpp"
+
+#pragma clang fp contract(off)
+
+namespace amrhip
+{
+
+// Per-launch constants of the Euler kernels: gamma, the cell sizes of every
+// level (physics_system::cell_sizes, formed on the host), the sweep direction.
+struct euler_params
+{
+    double   gamma;
+    double   cs[kMaxLevels][3];
+    uint32_t reverse;
+};
+
+// Device scalars of one advance_batch_async: the running minimum of
+// dx / speed (as the bits of a non-negative double, so that an integer
+// atomicMin orders it), the dt of the current step, the summed dt, the
+// remaining time, the executed step count and a stop flag (the CPU loop's
+// break, amr_solver.hpp:225-241).
+struct euler_scalars
+{
+    uint64_t dtmin_bits;
+    double   step_dt;
+    double   acc;
+    double   remaining;
+    uint64_t executed;
+    uint32_t stopped;
+    uint32_t pad;
+};
+
+template <int R>
+struct euler_flux
+{
+    static constexpr int NV = R + 2;
+
+    // primitiveToConservative (EulerPhysics.hpp:39-62).
+    __host__ __device__ static void prim_to_cons(const double* prim, double* cons, double gamma)
+    {
+        const double rho = prim[0];
+        const double p   = prim[R + 1];
+        cons[0]          = rho;
+        double vsq       = 0.0;
+        for (int d = 0; d < R; ++d)
+        {
+            const double vel = prim[1 + d];
+            cons[1 + d]      = rho * vel;
+            vsq += vel * vel;
+        }
+        cons[R + 1] = p / (gamma - 1.0) + 0.5 * rho * vsq;
+    }
+
+    // Pressure, sound speed and velocity along `dir` of one conservative
+    // state, as rusanovFlux and getMaxSpeed form them
+    // (EulerPhysics.hpp:80-94, 140-152).
+    __host__ __device__ static void state(const double* U, int dir, double gamma, double& p, double& a, double& un)
+    {
+        const double inv_rho = 1.0 / U[0];
+        double       kin     = 0.0;
+        for (int d = 0; d < R; ++d) kin += U[1 + d] * U[1 + d];
+        kin *= 0.5 * inv_rho;
+        p  = (gamma - 1.0) * (U[R + 1] - kin);
+        a  = sqrt(gamma * p * inv_rho);
+        un = U[1 + dir] * inv_rho;
+    }
+
+    // rusanovFlux (EulerPhysics.hpp:72-125); smax = std::max(l, r), i.e.
+    // (l < r) ? r : l.
+    __host__ __device__ static void rusanov(const double* UL, const double* UR, double* flux, int dir, double gamma)
+    {
+        double pl, al, ul, pr, ar, ur;
+        state(UL, dir, gamma, pl, al, ul);
+        state(UR, dir, gamma, pr, ar, ur);
+        const double sl = fabs(ul) + al, sr = fabs(ur) + ar;
+        const double smax = (sl < sr) ? sr : sl;
+        flux[0]           = 0.5 * (UL[1 + dir] + UR[1 + dir] - smax * (UR[0] - UL[0]));
+        for (int d = 0; d < R; ++d)
+        {
+            double fl = UL[1 + d] * ul;
+            double fr = UR[1 + d] * ur;
+            if (d == dir)
+            {
+                fl += pl;
+                fr += pr;
+            }
+            flux[1 + d] = 0.5 * (fl + fr - smax * (UR[1 + d] - UL[1 + d]));
+        }
+        const double fle = ul * (UL[R + 1] + pl);
+        const double fre = ur * (UR[R + 1] + pr);
+        flux[R + 1]      = 0.5 * (fle + fre - smax * (UR[R + 1] - UL[R + 1]));
+    }
+
+    // getMaxSpeed (EulerPhysics.hpp:132-155).
+    __host__ __device__ static double max_speed(const double* U, int dir, double gamma)
+    {
+        double p, a, un;
+        state(U, dir, gamma, p, a, un);
+        return fabs(un) + a;
+    }
+};
+
+// The NV values of the cell with layout coordinates c of `leaf`: interior
+// cells from the leaf, ghost cells (one coordinate just outside) through
+// the scalar ghost rules, field by field.
+template <int R, int N>
+__device__ inline void euler_cell(const double* __restrict__ u, size_t fstride, uint32_t leaf,
+                                  const face_link* __restrict__ links, const uint32_t* __restrict__ metas, const int* c,
+                                  double* U)
+{
+    using S          = patch_shape<R, N>;
+    constexpr int NV = R + 2;
+    int dir = -1;
+    for (int d = 0; d < R; ++d)
+    {
+        if (c[d] < 0) dir = 2 * d;
+        if (c[d] >= N) dir = 2 * d + 1;
+    }
+    if (dir < 0)
+    {
+        const size_t at = static_cast<size_t>(leaf) * S::NI + S::lin(c);
+        for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + at];
+        return;
+    }
+    const size_t e = static_cast<size_t>(leaf) * S::NDIR + dir;
+    for (int v = 0; v < NV; ++v) U[v] = ghost_value<R, N>(u + v * fstride, links[e], metas[e], dir, c);
+}
+
+// compute_time_step_cpu (amr_solver.hpp:355-413): min over the leaf's cells
+// and dims of dx_d / speed (speed > 1e-12), folded into the launch-wide
+// minimum.  The minimum is exact and order-free, so any reduction order
+// gives the oracle's value.
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_euler_dt(const double* __restrict__ u, size_t fstride,
+                                                 const uint64_t* __restrict__ keys, euler_params prm,
+                                                 euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    using S          = patch_shape<R, N>;
+    constexpr int NV = R + 2;
+    __shared__ double part[NT / 64];
+    const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
+    if (leaf >= nleaves) return;
+    const int lv = key_level(keys[leaf]);
+    double    dt = 1.7976931348623157e308;
+    for (int i = threadIdx.x; i < S::NI; i += NT)
+    {
+        double U[NV];
+        for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + static_cast<size_t>(leaf) * S::NI + i];
+        for (int d = 0; d < R; ++d)
+        {
+            const double speed = euler_flux<R>::max_speed(U, d, prm.gamma);
+            if (speed > 1e-12)
+            {
+                const double q = prm.cs[lv][d] / speed;
+                dt             = (q < dt) ? q : dt;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const double o = __shfl_down(dt, off, 64);
+        dt             = (o < dt) ? o : dt;
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int w = 1; w < NT / 64; ++w) dt = (part[w] < dt) ? part[w] : dt;
+        // non-negative doubles order like their bit patterns
+        atomicMin(reinterpret_cast<unsigned long long*>(&sc->dtmin_bits),
+                  static_cast<unsigned long long>(__double_as_longlong(dt)));
+    }
+}
+
+// The CPU loop's per-step dt logic (amr_solver.hpp:225-241) on the device:
+// stop when the remaining time is used up, else dt = min(cfl * dtmin,
+// remaining); reset the minimum for the next step.
+__global__ void k_euler_finalize(euler_scalars* __restrict__ sc, double cfl)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double raw = __longlong_as_double(static_cast<long long>(sc->dtmin_bits));
+    sc->dtmin_bits   = static_cast<uint64_t>(__double_as_longlong(1.7976931348623157e308));
+    double step_dt   = 0.0;
+    if (!sc->stopped && sc->remaining > 0.0)
+    {
+        const double dt = cfl * raw;
+        step_dt         = (sc->remaining < dt) ? sc->remaining : dt;
+    }
+    if (step_dt <= 0.0)
+    {
+        sc->stopped = 1;
+        sc->step_dt = 0.0;
+        return;
+    }
+    sc->step_dt = step_dt;
+    sc->acc += step_dt;
+    sc->remaining -= step_dt;
+    sc->executed += 1;
+}
+
+// Batch start: remaining time, zero sums, dt minimum reset (a kernel, not a
+// memset, so that a captured graph replays it).
+__global__ void k_euler_begin(euler_scalars* __restrict__ sc, double remaining)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    sc->dtmin_bits = static_cast<uint64_t>(__double_as_longlong(1.7976931348623157e308));
+    sc->step_dt    = 0.0;
+    sc->acc        = 0.0;
+    sc->remaining  = remaining;
+    sc->executed   = 0;
+    sc->stopped    = 0;
+}
+
+// time_step_cpu (amr_solver.hpp:265-353) for the Euler fields: per cell and
+// physical dim d, fL = F(U[c - e_d], U[c]), fR = F(U[c], U[c + e_d]),
+// total[k] -= dt / dx_d * (fR[k] - fL[k]); out = U + total.  A step the dt
+// logic stopped copies the state (the CPU loop runs no step).
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u, double* __restrict__ un,
+                                                   size_t fstride, const uint64_t* __restrict__ keys,
+                                                   const face_link* __restrict__ links,
+                                                   const uint32_t* __restrict__ metas, euler_params prm,
+                                                   const euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    using S          = patch_shape<R, N>;
+    constexpr int NV = R + 2;
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    if (leaf >= nleaves) return;
+    const double dt   = sc->step_dt;
+    const size_t base = static_cast<size_t>(leaf) * S::NI;
+    if (!(dt > 0.0))
+    {
+        for (int i = threadIdx.x; i < S::NI; i += NT)
+            for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = u[v * fstride + base + i];
+        return;
+    }
+    const int lv = key_level(keys[leaf]);
+    double    dtdx[R];
+    for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
+    for (int i = threadIdx.x; i < S::NI; i += NT)
+    {
+        int c[R];
+        S::coords(i, c);
+        double Uc[NV], total[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            Uc[v]    = u[v * fstride + base + i];
+            total[v] = 0.0;
+        }
+        for (int d = 0; d < R; ++d)
+        {
+            const int j = R - 1 - d; // layout dim of physical axis d
+            double    UL[NV], UR[NV], fL[NV], fR[NV];
+            c[j] -= 1;
+            euler_cell<R, N>(u, fstride, leaf, links, metas, c, UL);
+            c[j] += 2;
+            euler_cell<R, N>(u, fstride, leaf, links, metas, c, UR);
+            c[j] -= 1;
+            euler_flux<R>::rusanov(UL, Uc, fL, d, prm.gamma);
+            euler_flux<R>::rusanov(Uc, UR, fR, d, prm.gamma);
+            for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
+        }
+        for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = Uc[v] + total[v];
+    }
+}
+
+} // namespace amrhip

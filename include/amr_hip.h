@@ -127,13 +127,16 @@ int amr_tree_compute_flags(amr_ctx* ctx, const amr_criterion* crit, int8_t* host
 
 /* ---- fields ------------------------------------------------------------ */
 
-/* Write / read the interiors of all leaves (size * n^rank doubles, leaf
- * after leaf): the host side of amr_solver::initialize (amr_solver.hpp:105-145)
- * and sync_current_to_device / sync_current_from_device (ndtree.hpp:583-591). */
+/* Write / read the interiors of all leaves (fields * size * n^rank doubles:
+ * field after field, leaf after leaf within a field; fields = 1 for
+ * advection, rank + 2 for Euler): the host side of amr_solver::initialize
+ * (amr_solver.hpp:105-145) and sync_current_to_device /
+ * sync_current_from_device (ndtree.hpp:583-591). */
 int amr_tree_set_interior(amr_ctx* ctx, const double* host, uint64_t count);
 int amr_tree_get_interior(amr_ctx* ctx, double* host, uint64_t count);
 /* Padded patches after ndtree::halo_exchange_update (ndtree.hpp:1862-1877):
- * size * (n+2h)^rank doubles; corner cells are 0 (never written). */
+ * fields * size * (n+2h)^rank doubles, field-major; corner cells are 0
+ * (never written). */
 int amr_tree_get_patches(amr_ctx* ctx, double* host, uint64_t count);
 /* ndtree::halo_exchange_update (ndtree.hpp:1862-1877): fills the face halos
  * of the device padded view.  Time steps gather ghosts from neighbour
@@ -145,6 +148,10 @@ int amr_tree_swap_buffers(amr_ctx* ctx);
  * context's stream, for callers that share device memory (PyTorch). */
 int amr_tree_device_interior(amr_ctx* ctx, double** dev_ptr);
 int amr_tree_stream(amr_ctx* ctx, void** hip_stream);
+
+/* Number of fields per cell: 1 for advection, rank + 2 for Euler (the
+ * FieldTags of the equation, include/solver/EulerPhysics.hpp:22-30). */
+int amr_tree_fields(const amr_ctx* ctx, int32_t* fields);
 
 /* ---- solver (include/solver/amr_solver.hpp) ---------------------------- */
 
@@ -164,6 +171,29 @@ int amr_solver_finish_advance_batch(amr_ctx* ctx, double* dt_sum, uint64_t* exec
 /* advance() = advance_batch_async(1) + finish_advance_batch
  * (amr_solver.hpp:149-153). */
 int amr_solver_advance(amr_ctx* ctx, double* dt);
+
+/* ---- Euler physics (include/solver/EulerPhysics.hpp) ------------------- */
+
+/* amr_solver<tree, physics, EulerPhysics<rank>, rank>(capacity, gamma, cfl)
+ * (amr_solver.hpp:28-97) with its tree: a context whose cells carry the
+ * rank + 2 conservative variables [rho, rho u, rho v, (rho w), E].  The tree
+ * entry points (regrid by flags or criterion -- evaluated on rho, as the
+ * reference's Euler drivers do, benchmark/bench_fvm_solver_integration3D.b.cpp:86-107
+ * -- keys, tables, fields, padded view) apply to it; amr_solver_compute_dt,
+ * amr_solver_advance_batch_async, amr_solver_finish_advance_batch and
+ * amr_solver_advance run the Euler step: Rusanov fluxes
+ * (EulerPhysics::rusanovFlux, EulerPhysics.hpp:72-125) and, every step, the
+ * CFL dt from the maximum wave speed (compute_time_step_cpu,
+ * amr_solver.hpp:355-413; getMaxSpeed, EulerPhysics.hpp:132-155) clamped to
+ * the remaining time, computed on the device.  Step variants, fusion, batches
+ * and the adjoint tape are advection-only (AMR_ERR_UNSUPPORTED). */
+int amr_euler_create(const amr_tree_config* cfg, double gamma, double cfl, amr_ctx** out);
+int amr_euler_configure(amr_ctx* ctx, double gamma, double cfl);
+/* amr_solver::initialize's conversion (amr_solver.hpp:130-134,
+ * EulerPhysics::primitiveToConservative, EulerPhysics.hpp:39-62): primitive
+ * values [rho, u, v, (w), p], field-major like amr_tree_set_interior, are
+ * converted per cell and written as the current state. */
+int amr_euler_set_primitive(amr_ctx* ctx, const double* host_prim, uint64_t count);
 
 /* ---- step-kernel autotuner (no counterpart in the reference) ----------- */
 

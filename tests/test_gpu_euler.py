@@ -1,0 +1,194 @@
+"""Euler physics through the HIP path (amr_euler_* and the solver entry
+points) against the CPU oracle (OracleEuler), bit for bit: fields, the
+per-step CFL dt (summed) and the executed step count, on uniform and
+adaptive trees in 2D and 3D, including the reference's benchmark shapes
+(benchmark/bench_fvm_solver_integration.b.cpp: 64^2 patches;
+bench_fvm_solver_integration3D.b.cpp: 8^3 patches, domain 1000^3, acoustic
+pulse, max-density regrid criterion)."""
+import sys
+
+import numpy as np
+import pytest
+
+from amrlib import load_package
+from oracle import OracleEuler, euler_prim_to_cons
+from test_gpu_parity import assert_fields
+
+pytestmark = pytest.mark.gpu
+THREADS = 16
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def amr():
+    return load_package()
+
+
+def pair(amr, rank, n, depth, cap, lengths=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3):
+    s = amr.EulerSolver(rank, depth, n, 1, cap, lengths, gamma, cfl)
+    o = OracleEuler(rank, n, 1, depth, 2 * cap + 64, lengths)
+    o.set_params(gamma, cfl, threads=THREADS)
+    return s, o
+
+
+def pulse(rank, lengths, amp=10.0, vel=(0.3, -0.2, 0.1)):
+    """The reference's acoustic pulse (bench_fvm_solver_integration3D.b.cpp:149-187)
+    plus a uniform velocity so that no direction is degenerate."""
+    L = np.array(lengths[:rank], dtype=np.float64)
+
+    def ic(cen):
+        r2 = np.zeros(cen.shape[:-1])
+        for d in range(rank):
+            r2 = r2 + (cen[..., d] - 0.5 * L[d]) ** 2
+        pert = amp * np.exp(-r2 / (0.01 * L[0] * L[0]))
+        prim = np.zeros(cen.shape[:-1] + (rank + 2,))
+        prim[..., 0] = 0.5 + pert * 0.2
+        for d in range(rank):
+            prim[..., 1 + d] = vel[d]
+        prim[..., rank + 1] = 1.0 + pert
+        return prim
+
+    return ic
+
+
+def init_both(s, o, ic):
+    s.initialize(ic)
+    prim = ic(o.cell_centers())
+    cons = euler_prim_to_cons(o.rank, prim.reshape(-1, o.nv), s.gamma).reshape(prim.shape)
+    o.set_interior(np.moveaxis(cons, -1, 0))
+    o.halo_exchange()
+    assert_fields(s.tree.get_interior(), o.get_interior(), "initial state")
+
+
+def assert_topology(s, o, what):
+    np.testing.assert_array_equal(s.tree.keys(), o.keys(), err_msg=f"{what}: keys")
+
+
+@pytest.mark.parametrize("rank,n,levels,steps", [(2, 16, 2, 10), (2, 8, 3, 12), (3, 8, 2, 6), (3, 4, 2, 8)])
+def test_euler_uniform(amr, rank, n, levels, steps):
+    say(f"case: Euler uniform rank={rank} n={n} levels={levels} steps={steps}")
+    depth = 7 if rank == 2 else 5
+    s, o = pair(amr, rank, n, depth, (1 << (rank * levels)) + 8)
+    for _ in range(levels):
+        s.tree.refine_all()
+        o.refine_all()
+    assert_topology(s, o, "uniform")
+    init_both(s, o, pulse(rank, (1.0, 1.0, 1.0), amp=2.0))
+    assert s.compute_dt() == o.compute_dt()
+    s.advance_batch_async(steps)
+    got = s.finish_advance_batch()
+    exp = o.advance_batch(steps)
+    assert got == exp, (got, exp)
+    assert_fields(s.tree.get_interior(), o.get_interior(), "after steps")
+
+
+def test_euler_remaining_time(amr):
+    say("case: Euler remaining-time clamp and stop")
+    s, o = pair(amr, 2, 16, 7, 64)
+    for _ in range(2):
+        s.tree.refine_all()
+        o.refine_all()
+    init_both(s, o, pulse(2, (1.0, 1.0, 1.0), amp=1.0))
+    dt0 = o.compute_dt()
+    rem = 2.5 * dt0
+    s.advance_batch_async(10, rem)
+    got = s.finish_advance_batch()
+    exp = o.advance_batch(10, rem)
+    assert got == exp and got[1] == 3, (got, exp)
+    assert_fields(s.tree.get_interior(), o.get_interior(), "clamped batch")
+    before = s.tree.get_interior().copy()
+    s.advance_batch_async(4, 0.0)
+    assert s.finish_advance_batch() == (0.0, 0)
+    assert np.array_equal(s.tree.get_interior().view(np.uint64), before.view(np.uint64))
+
+
+@pytest.mark.parametrize("rank,n,depth,start,cycles,every", [(2, 16, 7, 2, 4, 5), (2, 8, 7, 2, 5, 4), (3, 4, 5, 1, 4, 3)])
+def test_euler_adaptive(amr, rank, n, depth, start, cycles, every):
+    """Regrids by the density criterion (max value, refine > 0.53, coarsen <
+    0.49) between batches, as the reference's Euler drivers do."""
+    say(f"case: Euler adaptive rank={rank} n={n} cycles={cycles}")
+    cap = (1 << (rank * min(depth, start + cycles))) + 64
+    s, o = pair(amr, rank, n, depth, cap)
+    for _ in range(start):
+        s.tree.refine_all()
+        o.refine_all()
+    init_both(s, o, pulse(rank, (1.0, 1.0, 1.0)))
+    crit = amr.make_criterion(amr.CRITERION_MAX_VALUE, 0.53, 0.49, 1, min(depth, start + cycles))
+    for it in range(cycles):
+        s.advance_batch_async(every)
+        got = s.finish_advance_batch()
+        exp = o.advance_batch(every)
+        assert got == exp, (it, got, exp)
+        fo = o.criterion(0, 0.53, 0.49, 1, min(depth, start + cycles))
+        np.testing.assert_array_equal(s.tree.compute_flags(crit), fo, err_msg=f"cycle {it}: flags")
+        assert s.tree.reconstruct_tree(crit) == o.reconstruct(fo)
+        o.halo_exchange()
+        assert_topology(s, o, f"cycle {it}")
+        assert_fields(s.tree.get_interior(), o.get_interior(), f"cycle {it}")
+    say("leaves at end:", s.tree.size())
+
+
+def test_euler_reference_2d_patch_shape(amr):
+    """64^2 patches, domain 1000^2 (bench_fvm_solver_integration.b.cpp:35-40)."""
+    say("case: Euler 2D 64^2 patches, domain 1000^2")
+    L = (1000.0, 1000.0)
+    s, o = pair(amr, 2, 64, 7, 64 + 8, lengths=L)
+    for _ in range(2):
+        s.tree.refine_all()
+        o.refine_all()
+    init_both(s, o, pulse(2, L, vel=(0.0, 0.0)))
+    s.advance_batch_async(6)
+    got = s.finish_advance_batch()
+    assert got == o.advance_batch(6)
+    assert_fields(s.tree.get_interior(), o.get_interior(), "64^2 after 6 steps")
+
+
+def test_euler_reference_3d_benchmark_start(amr):
+    """The reference's 3D Euler benchmark (8^3 patches, depth 5, domain
+    1000^3, gamma 1.4, cfl 0.3, two refine-all, acoustic pulse, regrid every
+    5 steps by max density 0.53 / 0.49 on levels 1..5): its first 4 cycles."""
+    say("case: Euler 3D reference benchmark, 4 cycles")
+    L = (1000.0, 1000.0, 1000.0)
+    s, o = pair(amr, 3, 8, 5, 20000, lengths=L)
+    for _ in range(2):
+        s.tree.refine_all()
+        o.refine_all()
+    init_both(s, o, pulse(3, L, vel=(0.0, 0.0, 0.0)))
+    crit = amr.make_criterion(amr.CRITERION_MAX_VALUE, 0.53, 0.49, 1, 5)
+    t, tmax = 0.0, 200.0
+    for it in range(4):
+        s.advance_batch_async(5, tmax - t)
+        got = s.finish_advance_batch()
+        exp = o.advance_batch(5, tmax - t)
+        assert got == exp, (it, got, exp)
+        t += got[0]
+        fo = o.criterion(0, 0.53, 0.49, 1, 5)
+        np.testing.assert_array_equal(s.tree.compute_flags(crit), fo)
+        assert s.tree.reconstruct_tree(crit) == o.reconstruct(fo)
+        o.halo_exchange()
+        assert_topology(s, o, f"cycle {it}")
+        assert_fields(s.tree.get_interior(), o.get_interior(), f"cycle {it}")
+    say("leaves:", s.tree.size(), "t =", t)
+
+
+def test_euler_rejects_advection_only_paths(amr):
+    say("case: Euler context on advection-only entry points")
+    s = amr.EulerSolver(2, 7, 16, 1, 32)
+    s.tree.refine_all()
+    lib = s.tree.lib
+    import ctypes
+
+    v, us = ctypes.c_int32(), ctypes.c_float()
+    assert lib.amr_solver_autotune(s.tree.h, 2, ctypes.byref(v), ctypes.byref(us)) == 4
+    vel = (ctypes.c_double * 3)(1.0, 0.5, 0.0)
+    assert lib.amr_solver_configure(s.tree.h, vel, 0.4) == 4
+    arr = (ctypes.c_void_p * 1)(s.tree.h)
+    b = ctypes.c_void_p()
+    assert lib.amr_batch_create(arr, 1, ctypes.byref(b)) == 4
+    assert s.tree.fields == 4
+    with pytest.raises(Exception):
+        amr.EulerSolver(2, 7, 16, 1, 32, gamma=1.0)
