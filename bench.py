@@ -241,6 +241,84 @@ def adjoint_section(amr, steps=300, checkpoint_every=20, reps=3):
     }
 
 
+def euler_section(amr, rank=3, cpu_cycles=6, threads=16):
+    """The reference's Euler benchmark (bench_fvm_solver_integration3D.b.cpp
+    in 3D): acoustic pulse on the adaptive tree, advance_batch_async(every,
+    tmax - t), regrid by density, finish_advance_batch, until tmax; its own
+    metric, cell-updates/s including the regrids (executed steps x leaves
+    before the regrid x n^R, over the loop's wall-clock).  The CPU oracle
+    runs the first `cpu_cycles` cycles of the same loop as the baseline."""
+    from gpu_amr_amd import workloads
+
+    solver, crit, cfg = workloads.euler_benchmark(amr, rank)
+    tree = solver.tree
+    ni = cfg["patch"] ** rank
+
+    def loop(s, t_max, max_cycles):
+        t, upd, steps, cycles, lo, hi = 0.0, 0, 0, 0, s.tree.size(), s.tree.size()
+        while t < t_max and cycles < max_cycles:
+            before = s.tree.size()
+            s.advance_batch_async(cfg["every"], t_max - t)
+            s.tree.reconstruct_tree(crit)
+            dt, ex = s.finish_advance_batch()
+            t += dt
+            upd += ex * before * ni
+            steps += ex
+            cycles += 1
+            lo, hi = min(lo, s.tree.size()), max(hi, s.tree.size())
+        return t, upd, steps, cycles, lo, hi
+
+    tree.synchronize()
+    t0 = time.perf_counter()
+    t, upd, steps, cycles, lo, hi = loop(solver, cfg["tmax"], 100000)
+    wall = time.perf_counter() - t0
+    rate = upd / wall
+    log(f"euler {rank}D: t={t:.3f} in {steps} steps / {cycles} regrids, leaves {lo}..{hi}, "
+        f"{wall * 1e3:.1f} ms, {rate:.3e} cell-updates/s (incl. regrid)")
+    tree.close()
+
+    # CPU baseline: the oracle on the first cpu_cycles cycles
+    if str(ROOT) not in sys.path:
+        sys.path.insert(0, str(ROOT))
+    from oracle import OracleEuler, euler_prim_to_cons
+
+    o = OracleEuler(rank, cfg["patch"], 1, cfg["depth"], 2 * cfg["capacity"] + 64, (1000.0,) * rank)
+    o.set_params(1.4, 0.3, threads=threads)
+    for _ in range(cfg["initial"]):
+        o.refine_all()
+    prim = workloads.euler_pulse(rank)(o.cell_centers())
+    cons = euler_prim_to_cons(rank, prim.reshape(-1, rank + 2), 1.4).reshape(prim.shape)
+    o.set_interior(np.moveaxis(cons, -1, 0))
+    o.halo_exchange()
+    ot, oupd, osteps = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(cpu_cycles):
+        before = o.size()
+        dt, ex = o.advance_batch(cfg["every"], cfg["tmax"] - ot)
+        o.reconstruct(o.criterion(0, 0.53, 0.49, 1, cfg["max_level"]))
+        o.halo_exchange()
+        ot += dt
+        oupd += ex * before * ni
+        osteps += ex
+    cwall = time.perf_counter() - t0
+    log(f"euler {rank}D cpu: {osteps} steps in {cwall:.1f} s, {oupd / cwall:.3e} cell-updates/s on {threads} threads")
+    return {
+        "workload": f"reference Euler benchmark {rank}D: acoustic pulse, {cfg['patch']}^{rank} FP64 patches, "
+                    f"domain 1000^{rank}, gamma 1.4, cfl 0.3, regrid every {cfg['every']} steps by density "
+                    f"(0.53 / 0.49, levels 1..{cfg['max_level']}), tmax {cfg['tmax']:g}",
+        "metric": "cell-updates/s including regrids",
+        "value": rate,
+        "t_end": t,
+        "steps": steps,
+        "regrids": cycles,
+        "leaves_min": lo,
+        "leaves_max": hi,
+        "wall_ms": wall * 1e3,
+        "cpu_baseline": {"value": oupd / cwall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                         "sample": f"first {cpu_cycles} regrid cycles ({osteps} steps) of the same loop, {cwall:.1f} s"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +332,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     ap.add_argument("--variant", default="", help="step-kernel variant to run instead of autotuning")
     ap.add_argument("--no-fusion", action="store_true", help="one step per launch (no two-step fusion)")
+    ap.add_argument("--no-euler", action="store_true")
+    ap.add_argument("--euler-only", action="store_true", help="only the Euler section (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
@@ -264,6 +344,10 @@ def main():
 
     amr = package()
     from gpu_amr_amd import workloads
+
+    if args.euler_only:
+        print(json.dumps({"euler_3d": euler_section(amr, 3, cpu_cycles=1)}), flush=True)
+        return
 
     g, s = workloads.build_c3(amr)
     leaves = g.size()
@@ -334,6 +418,10 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_section(amr)
 
+    euler = None
+    if not args.no_euler:
+        euler = {"3d": euler_section(amr, 3), "2d": euler_section(amr, 2, cpu_cycles=2)}
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -390,6 +478,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "euler": euler,
     }
     print(json.dumps(out), flush=True)
 

@@ -89,7 +89,7 @@ struct ops
     void (*criterion)(int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double, double,
                       int, int, int8_t*, uint32_t, hipStream_t);
     void (*emit)(const uint64_t*, const double*, uint32_t, int, const uint8_t*, const uint8_t*, const uint32_t*,
-                 const uint32_t*, uint64_t*, double*, regrid_summary*, hipStream_t);
+                 const uint32_t*, uint64_t*, double*, regrid_summary*, int, size_t, hipStream_t);
     void (*adj_step)(const double*, const double*, const uint64_t*, const face_link*, const uint32_t*,
                      adjoint_params const&, double*, double*, uint32_t, hipStream_t);
     void (*adj_regrid)(const double*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*, double*, uint32_t,
@@ -193,11 +193,11 @@ struct instance
     }
     static void emit(const uint64_t* keys, const double* u, uint32_t n, int depth, const uint8_t* mark,
                      const uint8_t* head, const uint32_t* counts, const uint32_t* pos, uint64_t* keys_out,
-                     double* u_out, regrid_summary* summary, hipStream_t s)
+                     double* u_out, regrid_summary* summary, int nfields, size_t fstride, hipStream_t s)
     {
         if (n == 0) return;
-        hipLaunchKernelGGL((k_emit<R, N, NT>), dim3(n), dim3(NT), 0, s, keys, u, n, depth, mark, head, counts, pos,
-                           keys_out, u_out, summary);
+        hipLaunchKernelGGL((k_emit<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, keys, u, n, depth, mark, head, counts,
+                           pos, keys_out, u_out, summary, nfields, fstride);
         check_launch("k_emit");
     }
     static void adj_step(const double* lam_next, const double* u, const uint64_t* keys, const face_link* links,
@@ -236,8 +236,14 @@ struct instance
                            hipStream_t s)
     {
         if (n == 0) return;
-        hipLaunchKernelGGL((k_euler_step<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links, metas,
-                           prm, sc, n);
+        // LDS tile of every field where it fits, else neighbours from memory
+        constexpr bool tiled = (R + 2) * tile_shape<R, N>::TS * sizeof(double) <= 48 * 1024;
+        if constexpr (tiled)
+            hipLaunchKernelGGL((k_euler_step_tile<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links,
+                               metas, prm, sc, n);
+        else
+            hipLaunchKernelGGL((k_euler_step<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links,
+                               metas, prm, sc, n);
         check_launch("k_euler_step");
     }
     static constexpr ops table()
@@ -756,9 +762,8 @@ struct amr_ctx
         if (s.err & 2u) throw amr_error(AMR_ERR_UNSUPPORTED, "coarsening to the root level is not supported");
         if (s.n_refine == 0 && s.n_coarsen == 0) return false;
         if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
-        for (int v = 0; v < nvar; ++v)
-            op->emit(keys[kc], u[cur] + v * fstride(), nn, depth, mark, head, counts, pos, keys[1 - kc],
-                     u[1 - cur] + v * fstride(), summary, stream);
+        op->emit(keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, nvar,
+                 fstride(), stream);
         cur = 1 - cur;
         kc  = 1 - kc;
         n   = s.total;
@@ -1059,7 +1064,7 @@ struct amr_tape
     void replay_transfer(regrid_record const& r, const double* src, double* dst)
     {
         op->emit(topos[static_cast<size_t>(r.topo_before)].keys, src, static_cast<uint32_t>(r.n_old), depth, r.mark,
-                 r.head, r.counts, r.pos, scratch_keys, dst, scratch_summary, stream);
+                 r.head, r.counts, r.pos, scratch_keys, dst, scratch_summary, 1, 0, stream);
     }
 
     void forward_step(uint64_t k, const double* src, double* dst)

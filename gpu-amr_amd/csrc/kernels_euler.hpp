@@ -64,28 +64,34 @@ struct euler_flux
         cons[R + 1] = p / (gamma - 1.0) + 0.5 * rho * vsq;
     }
 
-    // Pressure, sound speed and velocity along `dir` of one conservative
-    // state, as rusanovFlux and getMaxSpeed form them
-    // (EulerPhysics.hpp:80-94, 140-152).
-    __host__ __device__ static void state(const double* U, int dir, double gamma, double& p, double& a, double& un)
+    // 1/rho, pressure and sound speed of one conservative state, as
+    // rusanovFlux and getMaxSpeed form them (EulerPhysics.hpp:80-94,
+    // 140-152); the velocity along a direction is U[1 + dir] * inv_rho.  A
+    // state is formed once per cell and reused for every face and direction
+    // (the same operands give the same bits).
+    struct state
     {
-        const double inv_rho = 1.0 / U[0];
-        double       kin     = 0.0;
+        double inv_rho, p, a;
+    };
+    __host__ __device__ static state state_of(const double* U, double gamma)
+    {
+        state        st;
+        st.inv_rho = 1.0 / U[0];
+        double kin = 0.0;
         for (int d = 0; d < R; ++d) kin += U[1 + d] * U[1 + d];
-        kin *= 0.5 * inv_rho;
-        p  = (gamma - 1.0) * (U[R + 1] - kin);
-        a  = sqrt(gamma * p * inv_rho);
-        un = U[1 + dir] * inv_rho;
+        kin *= 0.5 * st.inv_rho;
+        st.p = (gamma - 1.0) * (U[R + 1] - kin);
+        st.a = sqrt(gamma * st.p * st.inv_rho);
+        return st;
     }
 
-    // rusanovFlux (EulerPhysics.hpp:72-125); smax = std::max(l, r), i.e.
-    // (l < r) ? r : l.
-    __host__ __device__ static void rusanov(const double* UL, const double* UR, double* flux, int dir, double gamma)
+    // rusanovFlux (EulerPhysics.hpp:72-125) from the two states; smax =
+    // std::max(l, r), i.e. (l < r) ? r : l.
+    __host__ __device__ static void rusanov(const double* UL, state const& L, const double* UR, state const& Rs,
+                                            double* flux, int dir)
     {
-        double pl, al, ul, pr, ar, ur;
-        state(UL, dir, gamma, pl, al, ul);
-        state(UR, dir, gamma, pr, ar, ur);
-        const double sl = fabs(ul) + al, sr = fabs(ur) + ar;
+        const double ul = UL[1 + dir] * L.inv_rho, ur = UR[1 + dir] * Rs.inv_rho;
+        const double sl = fabs(ul) + L.a, sr = fabs(ur) + Rs.a;
         const double smax = (sl < sr) ? sr : sl;
         flux[0]           = 0.5 * (UL[1 + dir] + UR[1 + dir] - smax * (UR[0] - UL[0]));
         for (int d = 0; d < R; ++d)
@@ -94,22 +100,25 @@ struct euler_flux
             double fr = UR[1 + d] * ur;
             if (d == dir)
             {
-                fl += pl;
-                fr += pr;
+                fl += L.p;
+                fr += Rs.p;
             }
             flux[1 + d] = 0.5 * (fl + fr - smax * (UR[1 + d] - UL[1 + d]));
         }
-        const double fle = ul * (UL[R + 1] + pl);
-        const double fre = ur * (UR[R + 1] + pr);
+        const double fle = ul * (UL[R + 1] + L.p);
+        const double fre = ur * (UR[R + 1] + Rs.p);
         flux[R + 1]      = 0.5 * (fle + fre - smax * (UR[R + 1] - UL[R + 1]));
     }
 
-    // getMaxSpeed (EulerPhysics.hpp:132-155).
-    __host__ __device__ static double max_speed(const double* U, int dir, double gamma)
+    __host__ __device__ static void rusanov(const double* UL, const double* UR, double* flux, int dir, double gamma)
     {
-        double p, a, un;
-        state(U, dir, gamma, p, a, un);
-        return fabs(un) + a;
+        rusanov(UL, state_of(UL, gamma), UR, state_of(UR, gamma), flux, dir);
+    }
+
+    // getMaxSpeed (EulerPhysics.hpp:132-155).
+    __host__ __device__ static double max_speed(const double* U, state const& st, int dir)
+    {
+        return fabs(U[1 + dir] * st.inv_rho) + st.a;
     }
 };
 
@@ -159,9 +168,10 @@ __global__ __launch_bounds__(NT) void k_euler_dt(const double* __restrict__ u, s
     {
         double U[NV];
         for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + static_cast<size_t>(leaf) * S::NI + i];
+        const auto st = euler_flux<R>::state_of(U, prm.gamma);
         for (int d = 0; d < R; ++d)
         {
-            const double speed = euler_flux<R>::max_speed(U, d, prm.gamma);
+            const double speed = euler_flux<R>::max_speed(U, st, d);
             if (speed > 1e-12)
             {
                 const double q = prm.cs[lv][d] / speed;
@@ -179,9 +189,13 @@ __global__ __launch_bounds__(NT) void k_euler_dt(const double* __restrict__ u, s
     if (threadIdx.x == 0)
     {
         for (int w = 1; w < NT / 64; ++w) dt = (part[w] < dt) ? part[w] : dt;
-        // non-negative doubles order like their bit patterns
-        atomicMin(reinterpret_cast<unsigned long long*>(&sc->dtmin_bits),
-                  static_cast<unsigned long long>(__double_as_longlong(dt)));
+        // non-negative doubles order like their bit patterns; most leaves
+        // do not lower the minimum and skip the atomic (a stale read only
+        // costs an extra one), which keeps thousands of workgroups from
+        // queueing on one address
+        const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(dt));
+        if (bits < *reinterpret_cast<volatile unsigned long long*>(&sc->dtmin_bits))
+            atomicMin(reinterpret_cast<unsigned long long*>(&sc->dtmin_bits), bits);
     }
 }
 
@@ -260,6 +274,7 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
             Uc[v]    = u[v * fstride + base + i];
             total[v] = 0.0;
         }
+        const auto stc = euler_flux<R>::state_of(Uc, prm.gamma);
         for (int d = 0; d < R; ++d)
         {
             const int j = R - 1 - d; // layout dim of physical axis d
@@ -269,8 +284,67 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
             c[j] += 2;
             euler_cell<R, N>(u, fstride, leaf, links, metas, c, UR);
             c[j] -= 1;
-            euler_flux<R>::rusanov(UL, Uc, fL, d, prm.gamma);
-            euler_flux<R>::rusanov(Uc, UR, fR, d, prm.gamma);
+            euler_flux<R>::rusanov(UL, euler_flux<R>::state_of(UL, prm.gamma), Uc, stc, fL, d);
+            euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
+            for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
+        }
+        for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = Uc[v] + total[v];
+    }
+}
+
+// k_euler_step with the leaf's fields staged in LDS: every field's (n+2)^R
+// tile (interior plus face ghosts, gathered once per ghost cell by the same
+// rules) is loaded cooperatively, then each cell reads its neighbours from
+// LDS.  For patch shapes whose tiles fit (3D 8^3: 40 KB).
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict__ u, double* __restrict__ un,
+                                                        size_t fstride, const uint64_t* __restrict__ keys,
+                                                        const face_link* __restrict__ links,
+                                                        const uint32_t* __restrict__ metas, euler_params prm,
+                                                        const euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    using S          = patch_shape<R, N>;
+    using T          = tile_shape<R, N>;
+    constexpr int NV = R + 2;
+    __shared__ double tile[NV][T::TS];
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    if (leaf >= nleaves) return;
+    const double dt   = sc->step_dt;
+    const size_t base = static_cast<size_t>(leaf) * S::NI;
+    if (!(dt > 0.0))
+    {
+        for (int i = threadIdx.x; i < S::NI; i += NT)
+            for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = u[v * fstride + base + i];
+        return;
+    }
+    for (int v = 0; v < NV; ++v) stage_tile<R, N, NT>(tile[v], u + v * fstride, leaf, links, metas);
+    __syncthreads();
+    const int lv = key_level(keys[leaf]);
+    double    dtdx[R];
+    for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
+    for (int i = threadIdx.x; i < S::NI; i += NT)
+    {
+        int c[R];
+        S::coords(i, c);
+        const int t = T::idx(c);
+        double    Uc[NV], total[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            Uc[v]    = tile[v][t];
+            total[v] = 0.0;
+        }
+        const auto stc = euler_flux<R>::state_of(Uc, prm.gamma);
+        for (int d = 0; d < R; ++d)
+        {
+            const int s = T::stride(R - 1 - d); // physical axis d
+            double    UL[NV], UR[NV], fL[NV], fR[NV];
+            for (int v = 0; v < NV; ++v)
+            {
+                UL[v] = tile[v][t - s];
+                UR[v] = tile[v][t + s];
+            }
+            euler_flux<R>::rusanov(UL, euler_flux<R>::state_of(UL, prm.gamma), Uc, stc, fL, d);
+            euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
             for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
         }
         for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = Uc[v] + total[v];

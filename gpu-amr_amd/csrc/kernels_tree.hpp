@@ -473,10 +473,11 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ keys, 
                                              uint32_t n, int depth, const uint8_t* __restrict__ mark,
                                              const uint8_t* __restrict__ head, const uint32_t* __restrict__ counts,
                                              const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys_out,
-                                             double* __restrict__ u_out, regrid_summary* __restrict__ summary)
+                                             double* __restrict__ u_out, regrid_summary* __restrict__ summary,
+                                             int nfields, size_t fstride)
 {
     using S          = patch_shape<R, N>;
-    const uint32_t i = blockIdx.x;
+    const uint32_t i = xcd_item(blockIdx.x, gridDim.x);
     if (i >= n) return;
     const uint32_t cnt = counts[i];
     if (cnt == 0) return;
@@ -486,26 +487,34 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ keys, 
     const uint32_t dst    = pos[i];
     for (uint32_t j = 0; j < cnt; ++j)
     {
-        double* out = u_out + static_cast<size_t>(dst + j) * S::NI;
-        if (refine)
+        // every field of the leaf goes through the same transfer
+        for (int f = 0; f < nfields; ++f)
         {
-            const double* src = u + static_cast<size_t>(i) * S::NI;
-            for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = refine_value<R, N>(src, static_cast<int>(j), t);
-        }
-        else if (fam)
-        {
-            for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = restrict_value<R, N>(u, i, t);
-        }
-        else
-        {
-            const double* src = u + static_cast<size_t>(i) * S::NI;
-            for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = src[t];
+            const double* uf  = u + f * fstride;
+            double*       out = u_out + f * fstride + static_cast<size_t>(dst + j) * S::NI;
+            if (refine)
+            {
+                const double* src = uf + static_cast<size_t>(i) * S::NI;
+                for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = refine_value<R, N>(src, static_cast<int>(j), t);
+            }
+            else if (fam)
+            {
+                for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = restrict_value<R, N>(uf, i, t);
+            }
+            else
+            {
+                const double* src = uf + static_cast<size_t>(i) * S::NI;
+                for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = src[t];
+            }
         }
         if (threadIdx.x == 0)
         {
-            const uint64_t k = emit_key<R>(key, depth, refine, fam, static_cast<int>(j));
+            const uint64_t k  = emit_key<R>(key, depth, refine, fam, static_cast<int>(j));
             keys_out[dst + j] = k;
-            atomicMax(&summary->max_level, static_cast<uint32_t>(key_level(k)));
+            // the maximum only grows: most leaves see it already reached and
+            // skip the atomic (a stale read only costs an extra atomic)
+            const uint32_t lv = static_cast<uint32_t>(key_level(k));
+            if (lv > *reinterpret_cast<volatile uint32_t*>(&summary->max_level)) atomicMax(&summary->max_level, lv);
         }
     }
 }

@@ -131,3 +131,47 @@ def c5_setup(amr, max_level=6, capacity=4096 + 64, max_adapt=8):
         history.append(flags)
         solver.initialize(ic)
     return tree, solver, crit, history, ic
+
+
+# ---------------------------------------------------------------------------
+# Euler: the reference's acoustic-pulse benchmarks
+# (benchmark/bench_fvm_solver_integration.b.cpp, 2D: 64^2 patches, depth 7,
+# three refine-all, regrid every 10 steps, levels 1..6, tmax 50;
+# benchmark/bench_fvm_solver_integration3D.b.cpp, 3D: 8^3 patches, depth 5,
+# two refine-all, regrid every 5 steps, levels 1..5, tmax 200).  Domain
+# 1000^R, gamma 1.4, cfl 0.3, density criterion refine > 0.53 / coarsen < 0.49.
+# ---------------------------------------------------------------------------
+EULER_CONFIGS = {
+    2: dict(patch=64, depth=7, initial=3, every=10, max_level=6, tmax=50.0, capacity=10000),
+    3: dict(patch=8, depth=5, initial=2, every=5, max_level=5, tmax=200.0, capacity=50000),
+}
+
+
+def euler_pulse(rank, length=1000.0):
+    """Primitive acoustic pulse: rho = 0.5 + 0.2 a, u = 0, p = 1 + a with
+    a = 10 exp(-r^2 / (0.01 L^2)) around the domain centre."""
+    def ic(c):
+        r2 = np.zeros(c.shape[:-1])
+        for d in range(rank):
+            dd = c[..., d] - 0.5 * length
+            r2 = r2 + dd * dd
+        pert = 10.0 * np.exp(-r2 / (0.01 * length * length))
+        prim = np.zeros(c.shape[:-1] + (rank + 2,))
+        prim[..., 0] = 0.5 + pert * 0.2
+        prim[..., rank + 1] = 1.0 + pert
+        return prim
+
+    return ic
+
+
+def euler_benchmark(amr, rank):
+    """Solver (tree refined and initialised) and criterion of the reference's
+    Euler benchmark in `rank` dimensions."""
+    cfg = EULER_CONFIGS[rank]
+    L = (1000.0,) * rank
+    solver = amr.EulerSolver(rank, cfg["depth"], cfg["patch"], 1, cfg["capacity"], L, 1.4, 0.3)
+    for _ in range(cfg["initial"]):
+        solver.tree.refine_all()
+    solver.initialize(euler_pulse(rank))
+    crit = amr.make_criterion(amr.CRITERION_MAX_VALUE, 0.53, 0.49, 1, cfg["max_level"])
+    return solver, crit, cfg
