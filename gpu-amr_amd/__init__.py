@@ -90,6 +90,7 @@ _SIGS = {
     "amr_solver_get_step_variant": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)],
     "amr_solver_autotune": [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)],
     "amr_tree_fields": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)],
+    "amr_tree_write_vtk": [ctypes.c_void_p, ctypes.c_char_p],
     "amr_euler_create": [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)],
     "amr_euler_configure": [ctypes.c_void_p, ctypes.c_double, ctypes.c_double],
     "amr_euler_set_primitive": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
@@ -272,6 +273,11 @@ class NdTree:
         out = np.zeros(self.fields * self.size() * self.flat, dtype=np.float64)
         _check(self.lib.amr_tree_get_patches(self.h, _vp(out), out.size))
         return out.reshape(self._field_shape(self.pad))
+
+    def write_vtk(self, path):
+        """vtk_print::print (vtk_print.hpp:53-254): legacy-VTK binary
+        unstructured grid of the current leaves and fields."""
+        _check(self.lib.amr_tree_write_vtk(self.h, str(path).encode()))
 
     def get_patch(self, i):
         return self.get_patches()[i]

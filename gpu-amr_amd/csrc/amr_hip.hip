@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -1435,6 +1436,152 @@ int amr_tree_device_interior(amr_ctx* ctx, double** dev_ptr)
             if (!ctx || !dev_ptr) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             *dev_ptr = ctx->u[ctx->cur];
             ctx->sync(); // queued steps have written it
+        }
+    );
+}
+
+// ---- legacy VTK output (include/ndtree/vtk_print.hpp:53-254) -------------
+
+extern "C++"
+{
+namespace
+{
+// vtk_print::write_binary: big-endian bytes of the value converted to T.
+template <typename T>
+void vtk_be(std::ofstream& f, T v)
+{
+    unsigned char b[sizeof(T)];
+    std::memcpy(b, &v, sizeof(T));
+    std::reverse(b, b + sizeof(T));
+    f.write(reinterpret_cast<const char*>(b), sizeof(T));
+}
+
+// Field names of the reference's cell types (include/solver/cell_types.hpp).
+const char* vtk_field_name(int rank, int nvar, int v)
+{
+    if (nvar == 1) return "Scalar";
+    static const char* const n2[] = { "Rho", "Rhou", "Rhov", "E2D" };
+    static const char* const n3[] = { "Rho", "Rhou", "Rhov", "Rhow", "E3D" };
+    return rank == 2 ? n2[v] : n3[v];
+}
+} // namespace
+} // extern "C++"
+
+int amr_tree_write_vtk(amr_ctx* ctx, const char* path)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || !path) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            ctx->ensure_usable();
+            const int      R = ctx->rank, n = ctx->cfg.patch_size, h = ctx->cfg.halo_width;
+            const uint64_t leaves = ctx->n, ni = static_cast<uint64_t>(ctx->ni), cells = leaves * ni;
+            std::vector<uint64_t> keys(leaves);
+            std::vector<double>   u(cells * ctx->nvar);
+            AMR_HIP_CHECK(hipMemcpyAsync(keys.data(), ctx->keys[ctx->kc], leaves * sizeof(uint64_t),
+                                         hipMemcpyDeviceToHost, ctx->stream));
+            for (int v = 0; v < ctx->nvar; ++v)
+                AMR_HIP_CHECK(hipMemcpyAsync(u.data() + v * cells, ctx->u[ctx->cur] + v * ctx->fstride(),
+                                             cells * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+            std::ofstream f(path, std::ios::binary);
+            if (!f) throw amr_error(AMR_ERR_ARGUMENT, std::string("cannot open ") + path);
+            // vtk_print::write_header
+            f << "# vtk DataFile Version 3.0\n";
+            f << "AMR Tree Structure\n";
+            f << "BINARY\n";
+            f << "DATASET UNSTRUCTURED_GRID\n";
+            const int box = 1 << R;
+            f << "POINTS " << cells * box << " double\n";
+            const double maxc = static_cast<double>(1u << ctx->depth);
+            for (uint64_t i = 0; i < leaves; ++i)
+            {
+                uint32_t c[3] = { 0, 0, 0 };
+                if (R == 2)
+                    deinterleave<2>(keys[i] >> 6, c);
+                else
+                    deinterleave<3>(keys[i] >> 6, c);
+                const int lv = key_level(keys[i]);
+                double    org[3], cs[3];
+                for (int d = 0; d < R; ++d)
+                {
+                    // physics_system::patch_coord / cell_sizes
+                    org[d] = ctx->cfg.domain_length[d] * static_cast<double>(c[d]) / maxc;
+                    cs[d]  = ctx->cell_size(lv, d);
+                }
+                for (uint64_t l = 0; l < ni; ++l)
+                {
+                    // interior cell l, row-major with x last
+                    const int    ix = static_cast<int>(l % n), iy = static_cast<int>((l / n) % n);
+                    const double x  = org[0] + static_cast<double>(ix) * cs[0];
+                    const double y  = org[1] + static_cast<double>(iy) * cs[1];
+                    if (R == 2)
+                    {
+                        const double q[4][2] = { { x, y }, { x + cs[0], y }, { x + cs[0], y + cs[1] }, { x, y + cs[1] } };
+                        for (auto const& p : q)
+                        {
+                            vtk_be<double>(f, p[0]);
+                            vtk_be<double>(f, p[1]);
+                            vtk_be<double>(f, 0.0);
+                        }
+                    }
+                    else
+                    {
+                        const int    iz = static_cast<int>(l / (static_cast<uint64_t>(n) * n));
+                        const double z  = org[2] + static_cast<double>(iz) * cs[2];
+                        // VTK_VOXEL corner order
+                        for (int k = 0; k < 8; ++k)
+                        {
+                            vtk_be<double>(f, (k & 1) ? x + cs[0] : x);
+                            vtk_be<double>(f, (k & 2) ? y + cs[1] : y);
+                            vtk_be<double>(f, (k & 4) ? z + cs[2] : z);
+                        }
+                    }
+                }
+            }
+            f << "CELLS " << cells << ' ' << cells * (box + 1) << '\n';
+            for (uint64_t i = 0; i < cells; ++i)
+            {
+                vtk_be<int>(f, box);
+                for (int j = 0; j < box; ++j) vtk_be<int>(f, static_cast<int>(i * box + j));
+            }
+            f << "CELL_TYPES " << cells << '\n';
+            for (uint64_t i = 0; i < cells; ++i) vtk_be<int>(f, R == 2 ? 9 : 11);
+            f << "CELL_DATA " << cells << '\n';
+            f << "SCALARS cell_index int 1\n";
+            f << "LOOKUP_TABLE default\n";
+            for (uint64_t i = 0; i < cells; ++i) vtk_be<int>(f, static_cast<int>(i));
+            // The reference evaluates is_halo_cell on the interior element
+            // index j read as a padded linear index (vtk_print.hpp:203-214);
+            // the flags are reproduced as written.
+            f << "SCALARS is_halo int 1\n";
+            f << "LOOKUP_TABLE default\n";
+            const int P = n + 2 * h;
+            std::vector<int> halo_flag(ni);
+            for (uint64_t j = 0; j < ni; ++j)
+            {
+                uint64_t rem = j;
+                int      flag = 0;
+                for (int d = 0; d < R; ++d)
+                {
+                    uint64_t stride = 1;
+                    for (int e = d + 1; e < R; ++e) stride *= static_cast<uint64_t>(P);
+                    const uint64_t rel = rem / stride;
+                    if (rel < static_cast<uint64_t>(h) || rel >= static_cast<uint64_t>(P - h)) flag = 1;
+                    rem %= stride;
+                    if (flag) break;
+                }
+                halo_flag[j] = flag;
+            }
+            for (uint64_t i = 0; i < leaves; ++i)
+                for (uint64_t j = 0; j < ni; ++j) vtk_be<int>(f, halo_flag[j]);
+            for (int v = 0; v < ctx->nvar; ++v)
+            {
+                f << "SCALARS " << vtk_field_name(R, ctx->nvar, v) << " double 1\n";
+                f << "LOOKUP_TABLE default\n";
+                for (uint64_t i = 0; i < cells; ++i) vtk_be<double>(f, u[v * cells + i]);
+            }
+            if (!f) throw amr_error(AMR_ERR_ARGUMENT, std::string("write failed: ") + path);
         }
     );
 }

@@ -149,6 +149,13 @@ int amr_tree_swap_buffers(amr_ctx* ctx);
 int amr_tree_device_interior(amr_ctx* ctx, double** dev_ptr);
 int amr_tree_stream(amr_ctx* ctx, void** hip_stream);
 
+/* vtk_print<physics_system>::print (include/ndtree/vtk_print.hpp:53-254):
+ * writes the legacy-VTK binary (big-endian) unstructured grid of the current
+ * leaves to `path` -- one quad / voxel per interior cell, cell_index,
+ * is_halo (as the reference computes it) and one double scalar per field
+ * ("Scalar"; Euler: "Rho", "Rhou", "Rhov", ("Rhow"), "E2D" / "E3D"). */
+int amr_tree_write_vtk(amr_ctx* ctx, const char* path);
+
 /* Number of fields per cell: 1 for advection, rank + 2 for Euler (the
  * FieldTags of the equation, include/solver/EulerPhysics.hpp:22-30). */
 int amr_tree_fields(const amr_ctx* ctx, int32_t* fields);
