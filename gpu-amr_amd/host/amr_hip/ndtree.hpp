@@ -5,6 +5,8 @@
 //   amr::ndt::morton::morton_id<Depth, Rank>   (include/morton/morton_id.hpp:20-450)
 //   amr::ndt::tree::ndtree<...>                (include/ndtree/ndtree.hpp:94-2157)
 //   amr_solver<Tree, Geometry, Equation, DIM>  (include/solver/amr_solver.hpp:28-544)
+//     -- advection: amr_solver<Tree>; Euler: euler_solver<Tree>
+//   vtk_print<Physics_System>                  (include/ndtree/vtk_print.hpp:16-280)
 // Every call goes through the C ABI of include/amr_hip.h; errors surface as
 // std::runtime_error, as the reference's device layer reports them
 // (src/cuda/device_buffer.cu:17-27).
@@ -15,10 +17,12 @@
 #include <algorithm>
 #include <array>
 #include <cstdint>
+#include <filesystem>
 #include <functional>
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <string_view>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -135,15 +139,23 @@ public:
 
     explicit ndtree(std::size_t capacity, std::array<double, 3> lengths = { 1.0, 1.0, 1.0 })
     {
-        amr_tree_config cfg{};
-        cfg.rank       = static_cast<int32_t>(Rank);
-        cfg.max_depth  = static_cast<int32_t>(Depth);
-        cfg.patch_size = static_cast<int32_t>(N);
-        cfg.halo_width = static_cast<int32_t>(Halo);
-        cfg.capacity   = capacity;
-        for (int i = 0; i < 3; ++i) cfg.domain_length[i] = lengths[i];
+        const auto cfg = config(capacity, lengths);
         check(amr_tree_create(&cfg, &m_ctx), "ndtree");
         m_lengths = lengths;
+        refresh_keys();
+    }
+    // The tree of an Euler solver: Rank + 2 conservative fields per cell
+    // (EulerPhysics<Rank>::FieldTags, EulerPhysics.hpp:22-30).
+    struct euler_fields
+    {
+        double gamma, cfl;
+    };
+    ndtree(std::size_t capacity, std::array<double, 3> lengths, euler_fields e)
+    {
+        const auto cfg = config(capacity, lengths);
+        check(amr_euler_create(&cfg, e.gamma, e.cfl, &m_ctx), "ndtree (Euler)");
+        m_lengths = lengths;
+        m_fields  = Rank + 2;
         refresh_keys();
     }
     ~ndtree()
@@ -154,6 +166,7 @@ public:
     ndtree& operator=(ndtree const&) = delete;
 
     amr_ctx* handle() const noexcept { return m_ctx; }
+    unsigned fields() const noexcept { return m_fields; }
 
     std::size_t size() const noexcept { return m_keys.size(); }
     patch_index_t get_node_index_at(linear_index_t i) const { return patch_index_t(m_keys.at(i)); }
@@ -189,16 +202,21 @@ public:
         m_view_valid = false;
     }
 
-    // Padded patch ((N+2*Halo)^Rank doubles, halos exchanged, corners 0) by
+    // Padded patch ((N+2*Halo)^Rank doubles, halos exchanged, corners 0) of
+    // field `field` (get_patch<Field>, field index in FieldTags order) by
     // linear index or by patch index (ndtree.hpp:477-512).  The padded view
     // is downloaded once and reused until the state changes.
-    std::vector<double> get_patch(linear_index_t i) const
+    std::vector<double> get_patch(linear_index_t i, unsigned field = 0) const
     {
-        auto const& all = view();
-        return std::vector<double>(all.begin() + static_cast<std::ptrdiff_t>(i * s_flat),
-                                   all.begin() + static_cast<std::ptrdiff_t>((i + 1) * s_flat));
+        auto const&       all = view();
+        const std::size_t at  = (static_cast<std::size_t>(field) * size() + i) * s_flat;
+        return std::vector<double>(all.begin() + static_cast<std::ptrdiff_t>(at),
+                                   all.begin() + static_cast<std::ptrdiff_t>(at + s_flat));
     }
-    std::vector<double> get_patch(patch_index_t k) const { return get_patch(linear_index_of(k)); }
+    std::vector<double> get_patch(patch_index_t k, unsigned field = 0) const
+    {
+        return get_patch(linear_index_of(k), field);
+    }
     std::vector<double> get_patches() const { return view(); }
     linear_index_t linear_index_of(patch_index_t k) const
     {
@@ -207,9 +225,10 @@ public:
         return static_cast<linear_index_t>(it - m_keys.begin());
     }
     void invalidate_view() noexcept { m_view_valid = false; }
+    // All fields' interiors, field-major (field, leaf, cell).
     std::vector<double> get_interior() const
     {
-        std::vector<double> out(size() * s_interior);
+        std::vector<double> out(m_fields * size() * s_interior);
         check(amr_tree_get_interior(m_ctx, out.data(), out.size()), "get_interior");
         return out;
     }
@@ -258,11 +277,23 @@ public:
     }
 
 private:
+    static amr_tree_config config(std::size_t capacity, std::array<double, 3> const& lengths)
+    {
+        amr_tree_config cfg{};
+        cfg.rank       = static_cast<int32_t>(Rank);
+        cfg.max_depth  = static_cast<int32_t>(Depth);
+        cfg.patch_size = static_cast<int32_t>(N);
+        cfg.halo_width = static_cast<int32_t>(Halo);
+        cfg.capacity   = capacity;
+        for (int i = 0; i < 3; ++i) cfg.domain_length[i] = lengths[i];
+        return cfg;
+    }
+
     std::vector<double> const& view() const
     {
         if (!m_view_valid)
         {
-            m_view.resize(size() * s_flat);
+            m_view.resize(m_fields * size() * s_flat);
             check(amr_tree_get_patches(m_ctx, m_view.data(), m_view.size()), "get_patches");
             m_view_valid = true;
         }
@@ -277,7 +308,8 @@ private:
         check(amr_tree_get_keys(m_ctx, m_keys.data(), n), "get_keys");
     }
 
-    amr_ctx*                    m_ctx = nullptr;
+    amr_ctx*                    m_ctx    = nullptr;
+    unsigned                    m_fields = 1;
     std::vector<std::uint64_t>  m_keys;
     std::array<double, 3>       m_lengths{ 1.0, 1.0, 1.0 };
     mutable std::vector<double> m_view;
@@ -334,6 +366,96 @@ public:
 
 private:
     Tree m_tree;
+};
+
+// amr_solver<Tree, physics_system, EulerPhysics<Rank>, Rank>(capacity, gamma,
+// cfl) (amr_solver.hpp:28-97, include/solver/EulerPhysics.hpp): the tree
+// carries [rho, rho u, rho v, (rho w), E]; dt follows the maximum wave speed
+// every step, on the device.
+template <typename Tree>
+class euler_solver
+{
+public:
+    static constexpr unsigned s_nvar = Tree::s_rank + 2;
+
+    euler_solver(std::size_t capacity, double gamma, double cfl, std::array<double, 3> lengths = { 1.0, 1.0, 1.0 })
+        : m_tree(capacity, lengths, typename Tree::euler_fields{ gamma, cfl })
+    {
+    }
+
+    Tree& get_tree() { return m_tree; }
+
+    // initialize(ic): ic(std::array<double, Rank> centre) -> primitive
+    // std::array<double, Rank + 2> [rho, u, v, (w), p]; converted by
+    // primitiveToConservative (amr_solver.hpp:105-145).
+    template <typename IC>
+    void initialize(IC&& ic)
+    {
+        const std::size_t   per = m_tree.size() * Tree::s_interior;
+        std::vector<double> prim(per * s_nvar);
+        for (std::size_t p = 0; p < m_tree.size(); ++p)
+        {
+            const auto k = m_tree.get_node_index_at(p);
+            for (std::size_t c = 0; c < Tree::s_interior; ++c)
+            {
+                const auto v = ic(m_tree.cell_center(k, c));
+                for (unsigned f = 0; f < s_nvar; ++f) prim[f * per + p * Tree::s_interior + c] = v[f];
+            }
+        }
+        check(amr_euler_set_primitive(m_tree.handle(), prim.data(), prim.size()), "initialize");
+        m_tree.invalidate_view();
+    }
+
+    double compute_time_step()
+    {
+        double dt = 0.0;
+        check(amr_solver_compute_dt(m_tree.handle(), &dt), "compute_time_step");
+        return dt;
+    }
+    double advance()
+    {
+        double dt = 0.0;
+        check(amr_solver_advance(m_tree.handle(), &dt), "advance");
+        m_tree.invalidate_view();
+        return dt;
+    }
+    void advance_batch_async(std::size_t steps, double remaining = std::numeric_limits<double>::max())
+    {
+        check(amr_solver_advance_batch_async(m_tree.handle(), steps, remaining), "advance_batch_async");
+        m_tree.invalidate_view();
+    }
+    double finish_advance_batch(std::size_t* executed = nullptr)
+    {
+        double        dt = 0.0;
+        std::uint64_t n  = 0;
+        check(amr_solver_finish_advance_batch(m_tree.handle(), &dt, &n), "finish_advance_batch");
+        if (executed) *executed = static_cast<std::size_t>(n);
+        return dt;
+    }
+
+private:
+    Tree m_tree;
+};
+
+// vtk_print<physics_system> (include/ndtree/vtk_print.hpp:16-62): files go
+// to vtk_output/<base><extension>, the directory created on construction.
+class vtk_print
+{
+public:
+    explicit vtk_print(std::string_view base_filename)
+        : m_base(base_filename)
+    {
+        std::filesystem::create_directory("vtk_output");
+    }
+    template <typename Tree>
+    void print(Tree const& tree, std::string const& extension) const
+    {
+        const std::string path = "vtk_output/" + m_base + extension;
+        check(amr_tree_write_vtk(tree.handle(), path.c_str()), "vtk_print");
+    }
+
+private:
+    std::string m_base;
 };
 
 } // namespace amr::hip

@@ -192,3 +192,19 @@ def test_euler_rejects_advection_only_paths(amr):
     assert s.tree.fields == 4
     with pytest.raises(Exception):
         amr.EulerSolver(2, 7, 16, 1, 32, gamma=1.0)
+
+
+def test_host_api_euler_and_vtk(tmp_path):
+    """The Euler solver and vtk_print through the amr::hip C++ host API
+    (tests/cpp/host_api_euler_vtk.cpp)."""
+    import subprocess
+
+    from amrlib import ROOT
+
+    say("case: C++ host API, Euler benchmark loop and vtk_print")
+    exe = ROOT / "tests" / "cpp" / "host_api_euler_vtk.bin"
+    assert exe.exists(), "build the native tests first (__graft_entry__.build())"
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240, cwd=tmp_path)
+    say(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert (tmp_path / "vtk_output" / "host_api_euler.vtk").exists()
