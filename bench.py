@@ -336,7 +336,8 @@ def main():
     ap.add_argument("--euler-only", action="store_true", help="only the Euler section (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the leaves would shard by contiguous "
+              "Morton ranges with one ghost-face exchange per step).", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 5)

@@ -1148,8 +1148,8 @@ public:
         }
     }
 
-    // Undivided-gradient criterion (defined by this build; the reference has
-    // none): g = max over interior cells c and layout dims of
+    // Undivided-gradient criterion (the reference has none; defined here):
+    // g = max over interior cells c and layout dims of
     // max(|u[c+s]-u[c]|, |u[c]-u[c-s]|) using the exchanged face halos.
     void criterion_gradient(double refine_thr, double coarsen_thr, int min_level, int max_level,
                             std::int8_t* flags) const
