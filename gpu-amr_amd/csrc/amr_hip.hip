@@ -237,11 +237,16 @@ struct instance
                            hipStream_t s)
     {
         if (n == 0) return;
-        // LDS tile of every field where it fits, else neighbours from memory
-        constexpr bool tiled = (R + 2) * tile_shape<R, N>::TS * sizeof(double) <= 48 * 1024;
-        if constexpr (tiled)
+        // LDS tiles of every field where they fit, else LDS strips of the
+        // patch, else neighbours from memory
+        constexpr size_t tile_bytes = (R + 2) * tile_shape<R, N>::TS * sizeof(double);
+        constexpr int    B          = euler_strip_depth<R, N>(48 * 1024);
+        if constexpr (tile_bytes <= 48 * 1024)
             hipLaunchKernelGGL((k_euler_step_tile<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links,
                                metas, prm, sc, n);
+        else if constexpr (B > 0)
+            hipLaunchKernelGGL((k_euler_step_strip<R, N, B, NT>), dim3(xcd_grid(static_cast<uint64_t>(n) * (N / B))),
+                               dim3(NT), 0, s, u, un, fs, keys, links, metas, prm, sc, n);
         else
             hipLaunchKernelGGL((k_euler_step<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links,
                                metas, prm, sc, n);

@@ -122,9 +122,56 @@ struct euler_flux
     }
 };
 
+// ghost_value for NV fields stored fstride apart: the face record is decoded
+// once and the source cells of every field are loaded together.  Each
+// field's arithmetic (the copy, the injection, the fine mean summed from 0.0
+// in row-major order) is ghost_value's, so the values are the same bits.
+template <int R, int N, int NV>
+__device__ inline void ghost_values(const double* __restrict__ u, size_t fstride, const face_link& L, uint32_t meta,
+                                    int dir, const int* c, double* out)
+{
+    using S            = patch_shape<R, N>;
+    const uint32_t rel = rel_of(meta);
+    if (rel == REL_SAME || rel == REL_COARSER)
+    {
+        const size_t at = ghost_source<R, N>(L.ids[0], meta, dir, c);
+        for (int v = 0; v < NV; ++v) out[v] = u[v * fstride + at];
+        return;
+    }
+    if (rel != REL_FINER)
+    {
+        for (int v = 0; v < NV; ++v) out[v] = 0.0;
+        return;
+    }
+    const int dim = dir >> 1;
+    const int nrm = (dir & 1) ? c[dim] - N : c[dim] + N;
+    int       k = 0, bit = 0, base[R];
+    for (int d = 0; d < R; ++d)
+    {
+        const int w = (d == dim) ? nrm : c[d];
+        base[d]     = (w * 2) % N;
+        if (d != dim)
+        {
+            k |= (c[d] / S::HALF) << bit;
+            ++bit;
+        }
+    }
+    const size_t q = static_cast<size_t>(L.ids[k]) * S::NI;
+    double       sum[NV];
+    for (int v = 0; v < NV; ++v) sum[v] = 0.0;
+    for (int o = 0; o < S::F; ++o)
+    {
+        int cc[R];
+        for (int d = 0; d < R; ++d) cc[d] = base[d] + ((o >> (R - 1 - d)) & 1);
+        const size_t at = q + S::lin(cc);
+        for (int v = 0; v < NV; ++v) sum[v] += u[v * fstride + at];
+    }
+    for (int v = 0; v < NV; ++v) out[v] = sum[v] / static_cast<double>(S::F);
+}
+
 // The NV values of the cell with layout coordinates c of `leaf`: interior
 // cells from the leaf, ghost cells (one coordinate just outside) through
-// the scalar ghost rules, field by field.
+// the ghost rules.
 template <int R, int N>
 __device__ inline void euler_cell(const double* __restrict__ u, size_t fstride, uint32_t leaf,
                                   const face_link* __restrict__ links, const uint32_t* __restrict__ metas, const int* c,
@@ -145,7 +192,37 @@ __device__ inline void euler_cell(const double* __restrict__ u, size_t fstride, 
         return;
     }
     const size_t e = static_cast<size_t>(leaf) * S::NDIR + dir;
-    for (int v = 0; v < NV; ++v) U[v] = ghost_value<R, N>(u + v * fstride, links[e], metas[e], dir, c);
+    ghost_values<R, N, NV>(u, fstride, links[e], metas[e], dir, c, U);
+}
+
+// Every field's (n+2)^R LDS tile of `leaf` (field v at tile + v * TS): the interior cells, then the
+// face-ghost cells with one decode of the face record per ghost cell for
+// all fields.
+template <int R, int N, int NT, int NV>
+__device__ inline void stage_fields(double* tile, const double* __restrict__ u,
+                                    size_t fstride, uint32_t leaf, const face_link* __restrict__ links,
+                                    const uint32_t* __restrict__ metas)
+{
+    using S           = patch_shape<R, N>;
+    using T           = tile_shape<R, N>;
+    const size_t base = static_cast<size_t>(leaf) * S::NI;
+    for (int i = threadIdx.x; i < S::NI; i += NT)
+    {
+        int c[R];
+        S::coords(i, c);
+        const int t = T::idx(c);
+        for (int v = 0; v < NV; ++v) tile[v * T::TS + t] = u[v * fstride + base + i];
+    }
+    for (int g = threadIdx.x; g < S::NDIR * S::FACE; g += NT)
+    {
+        int          c[R];
+        const int    dir = ghost_cell<R, N>(g, c);
+        const size_t e   = static_cast<size_t>(leaf) * S::NDIR + dir;
+        double       w[NV];
+        ghost_values<R, N, NV>(u, fstride, links[e], metas[e], dir, c, w);
+        const int t = T::idx(c);
+        for (int v = 0; v < NV; ++v) tile[v * T::TS + t] = w[v];
+    }
 }
 
 // compute_time_step_cpu (amr_solver.hpp:355-413): min over the leaf's cells
@@ -317,7 +394,7 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
             for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = u[v * fstride + base + i];
         return;
     }
-    for (int v = 0; v < NV; ++v) stage_tile<R, N, NT>(tile[v], u + v * fstride, leaf, links, metas);
+    stage_fields<R, N, NT, NV>(&tile[0][0], u, fstride, leaf, links, metas);
     __syncthreads();
     const int lv = key_level(keys[leaf]);
     double    dtdx[R];
@@ -349,6 +426,145 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
         }
         for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = Uc[v] + total[v];
     }
+}
+
+// Strip of B layers (along layout dim 0, the slowest) of an n^R patch with
+// one ghost layer all round: (B + 2) * (n + 2)^(R-1) LDS cells per field.
+template <int R, int N, int B>
+struct strip_shape
+{
+    static_assert(N % B == 0, "strips tile the patch");
+    static constexpr int TP = N + 2;
+    static constexpr int PL = (R == 2) ? TP : TP * TP; // tile cells per layer
+    static constexpr int CL = (R == 2) ? N : N * N;    // patch cells per layer
+    static constexpr int TS = (B + 2) * PL;
+    static constexpr int NS = N / B; // strips per patch
+    // tile index of patch cell c in the strip whose first layer is r0
+    __device__ static int idx(const int* c, int r0)
+    {
+        int l = c[0] - r0 + 1;
+        for (int d = 1; d < R; ++d) l = l * TP + (c[d] + 1);
+        return l;
+    }
+    // stride of layout dim j inside the tile
+    __host__ __device__ static constexpr int stride(int j) { return j == 0 ? PL : (j == R - 1 ? 1 : TP); }
+};
+
+// k_euler_step_tile for patches whose whole tile does not fit in LDS (2D
+// 64^2: 139 KB): one workgroup per strip of B layers, so that a leaf is n / B
+// workgroups.  The strip tile holds the strip's cells, the neighbouring
+// layers of the same patch (or the face ghosts beyond the patch) and the
+// face ghosts along the other dims; every cell's update reads the same
+// operands as in k_euler_step_tile, so the results are the same bits.
+template <int R, int N, int B, int NT>
+__global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restrict__ u, double* __restrict__ un,
+                                                         size_t fstride, const uint64_t* __restrict__ keys,
+                                                         const face_link* __restrict__ links,
+                                                         const uint32_t* __restrict__ metas, euler_params prm,
+                                                         const euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    using S          = patch_shape<R, N>;
+    using T          = strip_shape<R, N, B>;
+    constexpr int NV = R + 2;
+    __shared__ double tile[NV][T::TS];
+    const uint32_t item = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    const uint32_t leaf = item / T::NS;
+    if (leaf >= nleaves) return;
+    const int    r0    = static_cast<int>(item % T::NS) * B;
+    const double dt    = sc->step_dt;
+    const size_t base  = static_cast<size_t>(leaf) * S::NI;
+    const size_t first = base + static_cast<size_t>(r0) * T::CL; // the strip's first cell
+    if (!(dt > 0.0))
+    {
+        for (int i = threadIdx.x; i < B * T::CL; i += NT)
+            for (int v = 0; v < NV; ++v) un[v * fstride + first + i] = u[v * fstride + first + i];
+        return;
+    }
+    for (int q = threadIdx.x; q < T::TS; q += NT)
+    {
+        int c[R], r = q;
+        for (int d = R - 1; d >= 0; --d)
+        {
+            const int ext = (d == 0) ? B + 2 : T::TP;
+            c[d]          = r % ext - 1;
+            r /= ext;
+        }
+        c[0] += r0;
+        int outside = 0, dir = 0;
+        for (int d = 0; d < R; ++d)
+        {
+            if (c[d] < 0)
+            {
+                ++outside;
+                dir = 2 * d;
+            }
+            else if (c[d] >= N)
+            {
+                ++outside;
+                dir = 2 * d + 1;
+            }
+        }
+        if (outside > 1) continue; // edge / corner cells are never read
+        double w[NV];
+        if (outside == 0)
+        {
+            const size_t at = base + S::lin(c);
+            for (int v = 0; v < NV; ++v) w[v] = u[v * fstride + at];
+        }
+        else
+        {
+            const size_t e = static_cast<size_t>(leaf) * S::NDIR + dir;
+            ghost_values<R, N, NV>(u, fstride, links[e], metas[e], dir, c, w);
+        }
+        for (int v = 0; v < NV; ++v) tile[v][q] = w[v];
+    }
+    __syncthreads();
+    const int lv = key_level(keys[leaf]);
+    double    dtdx[R];
+    for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
+    for (int i = threadIdx.x; i < B * T::CL; i += NT)
+    {
+        int c[R], r = i % T::CL;
+        c[0] = r0 + i / T::CL;
+        for (int d = R - 1; d >= 1; --d)
+        {
+            c[d] = r % N;
+            r /= N;
+        }
+        const int t = T::idx(c, r0);
+        double    Uc[NV], total[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            Uc[v]    = tile[v][t];
+            total[v] = 0.0;
+        }
+        const auto stc = euler_flux<R>::state_of(Uc, prm.gamma);
+        for (int d = 0; d < R; ++d)
+        {
+            const int s = T::stride(R - 1 - d); // physical axis d
+            double    UL[NV], UR[NV], fL[NV], fR[NV];
+            for (int v = 0; v < NV; ++v)
+            {
+                UL[v] = tile[v][t - s];
+                UR[v] = tile[v][t + s];
+            }
+            euler_flux<R>::rusanov(UL, euler_flux<R>::state_of(UL, prm.gamma), Uc, stc, fL, d);
+            euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
+            for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
+        }
+        for (int v = 0; v < NV; ++v) un[v * fstride + first + i] = Uc[v] + total[v];
+    }
+}
+
+// Largest strip depth B (a divisor of n, at most n) whose strip tiles of all
+// R + 2 fields fit in `bytes` of LDS; 0 if none does.
+template <int R, int N>
+constexpr int euler_strip_depth(size_t bytes)
+{
+    for (int b = N; b >= 1; --b)
+        if (N % b == 0 && (R + 2) * (b + 2) * static_cast<size_t>(R == 2 ? N + 2 : (N + 2) * (N + 2)) * 8 <= bytes)
+            return b;
+    return 0;
 }
 
 } // namespace amrhip

@@ -152,22 +152,31 @@ __host__ __device__ inline void stage_interior(double* tile, const double* __res
 
 // Ghost item g in [0, 2R * N^(R-1)): direction g / N^(R-1), tangential
 // position g % N^(R-1) in row-major order over the non-normal dims.
+// Returns the direction and sets c to the (padded-frame) cell of ghost item g.
 template <int R, int N>
-__host__ __device__ inline void stage_ghost(double* tile, const double* __restrict__ u, uint32_t leaf,
-                                            const face_link* __restrict__ links, const uint32_t* __restrict__ metas,
-                                            int g)
+__host__ __device__ inline int ghost_cell(int g, int* c)
 {
     using S       = patch_shape<R, N>;
     const int dir = g / S::FACE, dim = dir >> 1;
-    int       c[R], r = g % S::FACE;
+    int       r   = g % S::FACE;
     for (int d = R - 1; d >= 0; --d)
     {
         if (d == dim) continue;
         c[d] = r % N;
         r /= N;
     }
-    c[dim]                          = (dir & 1) ? N : -1;
-    const size_t e                  = static_cast<size_t>(leaf) * S::NDIR + dir;
+    c[dim] = (dir & 1) ? N : -1;
+    return dir;
+}
+
+template <int R, int N>
+__host__ __device__ inline void stage_ghost(double* tile, const double* __restrict__ u, uint32_t leaf,
+                                            const face_link* __restrict__ links, const uint32_t* __restrict__ metas,
+                                            int g)
+{
+    int            c[R];
+    const int      dir             = ghost_cell<R, N>(g, c);
+    const size_t   e               = static_cast<size_t>(leaf) * patch_shape<R, N>::NDIR + dir;
     tile[tile_shape<R, N>::idx(c)] = ghost_value<R, N>(u, links[e], metas[e], dir, c);
 }
 
