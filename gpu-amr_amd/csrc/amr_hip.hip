@@ -101,7 +101,7 @@ struct ops
     void (*euler_dt)(const double*, size_t, const uint64_t*, euler_params const&, euler_scalars*, uint32_t,
                      hipStream_t);
     void (*euler_step)(const double*, double*, size_t, const uint64_t*, const face_link*, const uint32_t*,
-                       euler_params const&, const euler_scalars*, uint32_t, hipStream_t);
+                       euler_params const&, euler_scalars*, uint32_t, hipStream_t);
 };
 
 template <int R, int N, int H>
@@ -233,7 +233,7 @@ struct instance
         check_launch("k_euler_dt");
     }
     static void euler_step(const double* u, double* un, size_t fs, const uint64_t* keys, const face_link* links,
-                           const uint32_t* metas, euler_params const& prm, const euler_scalars* sc, uint32_t n,
+                           const uint32_t* metas, euler_params const& prm, euler_scalars* sc, uint32_t n,
                            hipStream_t s)
     {
         if (n == 0) return;
@@ -684,8 +684,10 @@ struct amr_ctx
         return prm;
     }
 
-    // advance_batch_async: per step, the dt reduction, the dt logic and the
-    // update are queued back to back; dt never visits the host.
+    // advance_batch_async: the dt reduction of the batch's first state, then
+    // per step the dt logic and the update, which also folds the next
+    // step's dt minimum over the cells it writes (the values the reduction
+    // would read back, so the same minimum); dt never visits the host.
     void euler_advance(uint64_t steps, double remaining)
     {
         ensure_usable();
@@ -693,12 +695,13 @@ struct amr_ctx
         hipLaunchKernelGGL(k_euler_begin, dim3(1), dim3(1), 0, stream, esc, remaining);
         check_launch("k_euler_begin");
         const uint32_t nn = static_cast<uint32_t>(n);
+        if (steps > 0) op->euler_dt(u[cur], fstride(), keys[kc], prm, esc, nn, stream);
         for (uint64_t s = 0; s < steps; ++s)
         {
-            op->euler_dt(u[cur], fstride(), keys[kc], prm, esc, nn, stream);
             hipLaunchKernelGGL(k_euler_finalize, dim3(1), dim3(1), 0, stream, esc, cfl);
             check_launch("k_euler_finalize");
             prm.reverse = static_cast<uint32_t>(sweeps++ & 1);
+            prm.next_dt = s + 1 < steps ? 1u : 0u;
             op->euler_step(u[cur], u[1 - cur], fstride(), keys[kc], links, metas, prm, esc, nn, stream);
             cur = 1 - cur;
         }

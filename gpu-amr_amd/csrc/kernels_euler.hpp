@@ -25,6 +25,7 @@ struct euler_params
     double   gamma;
     double   cs[kMaxLevels][3];
     uint32_t reverse;
+    uint32_t next_dt; // step kernels: fold the next step's dt minimum over their output
 };
 
 // Device scalars of one advance_batch_async: the running minimum of
@@ -225,6 +226,51 @@ __device__ inline void stage_fields(double* tile, const double* __restrict__ u,
     }
 }
 
+// compute_time_step_cpu's inner loop (amr_solver.hpp:380-400) for one cell
+// of cell sizes cs: dt = min(dt, cs[d] / speed_d) over the dims whose speed
+// exceeds 1e-12.
+template <int R>
+__device__ inline double cell_dt(const double* U, const double* cs, double gamma, double dt)
+{
+    const auto st = euler_flux<R>::state_of(U, gamma);
+    for (int d = 0; d < R; ++d)
+    {
+        const double speed = euler_flux<R>::max_speed(U, st, d);
+        if (speed > 1e-12)
+        {
+            const double q = cs[d] / speed;
+            dt             = (q < dt) ? q : dt;
+        }
+    }
+    return dt;
+}
+
+// Workgroup minimum of the lanes' dt, folded into the launch-wide minimum.
+// Every lane of the workgroup must call it.
+template <int NT>
+__device__ inline void fold_dt_min(double dt, euler_scalars* __restrict__ sc)
+{
+    __shared__ double part[NT / 64];
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const double o = __shfl_down(dt, off, 64);
+        dt             = (o < dt) ? o : dt;
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int w = 1; w < NT / 64; ++w) dt = (part[w] < dt) ? part[w] : dt;
+        // non-negative doubles order like their bit patterns; most
+        // workgroups do not lower the minimum and skip the atomic (a stale
+        // read only costs an extra one), which keeps thousands of
+        // workgroups from queueing on one address
+        const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(dt));
+        if (bits < *reinterpret_cast<volatile unsigned long long*>(&sc->dtmin_bits))
+            atomicMin(reinterpret_cast<unsigned long long*>(&sc->dtmin_bits), bits);
+    }
+}
+
 // compute_time_step_cpu (amr_solver.hpp:355-413): min over the leaf's cells
 // and dims of dx_d / speed (speed > 1e-12), folded into the launch-wide
 // minimum.  The minimum is exact and order-free, so any reduction order
@@ -236,7 +282,6 @@ __global__ __launch_bounds__(NT) void k_euler_dt(const double* __restrict__ u, s
 {
     using S          = patch_shape<R, N>;
     constexpr int NV = R + 2;
-    __shared__ double part[NT / 64];
     const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
     if (leaf >= nleaves) return;
     const int lv = key_level(keys[leaf]);
@@ -245,35 +290,9 @@ __global__ __launch_bounds__(NT) void k_euler_dt(const double* __restrict__ u, s
     {
         double U[NV];
         for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + static_cast<size_t>(leaf) * S::NI + i];
-        const auto st = euler_flux<R>::state_of(U, prm.gamma);
-        for (int d = 0; d < R; ++d)
-        {
-            const double speed = euler_flux<R>::max_speed(U, st, d);
-            if (speed > 1e-12)
-            {
-                const double q = prm.cs[lv][d] / speed;
-                dt             = (q < dt) ? q : dt;
-            }
-        }
+        dt = cell_dt<R>(U, prm.cs[lv], prm.gamma, dt);
     }
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        const double o = __shfl_down(dt, off, 64);
-        dt             = (o < dt) ? o : dt;
-    }
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = dt;
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        for (int w = 1; w < NT / 64; ++w) dt = (part[w] < dt) ? part[w] : dt;
-        // non-negative doubles order like their bit patterns; most leaves
-        // do not lower the minimum and skip the atomic (a stale read only
-        // costs an extra one), which keeps thousands of workgroups from
-        // queueing on one address
-        const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(dt));
-        if (bits < *reinterpret_cast<volatile unsigned long long*>(&sc->dtmin_bits))
-            atomicMin(reinterpret_cast<unsigned long long*>(&sc->dtmin_bits), bits);
-    }
+    fold_dt_min<NT>(dt, sc);
 }
 
 // The CPU loop's per-step dt logic (amr_solver.hpp:225-241) on the device:
@@ -324,7 +343,7 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
                                                    size_t fstride, const uint64_t* __restrict__ keys,
                                                    const face_link* __restrict__ links,
                                                    const uint32_t* __restrict__ metas, euler_params prm,
-                                                   const euler_scalars* __restrict__ sc, uint32_t nleaves)
+                                                   euler_scalars* __restrict__ sc, uint32_t nleaves)
 {
     using S          = patch_shape<R, N>;
     constexpr int NV = R + 2;
@@ -341,6 +360,7 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
     const int lv = key_level(keys[leaf]);
     double    dtdx[R];
     for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
+    double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
     for (int i = threadIdx.x; i < S::NI; i += NT)
     {
         int c[R];
@@ -365,8 +385,15 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
             euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
             for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
         }
-        for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = Uc[v] + total[v];
+        double o[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            o[v]                              = Uc[v] + total[v];
+            un[v * fstride + base + i] = o[v];
+        }
+        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
     }
+    if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
 }
 
 // k_euler_step with the leaf's fields staged in LDS: every field's (n+2)^R
@@ -378,7 +405,7 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
                                                         size_t fstride, const uint64_t* __restrict__ keys,
                                                         const face_link* __restrict__ links,
                                                         const uint32_t* __restrict__ metas, euler_params prm,
-                                                        const euler_scalars* __restrict__ sc, uint32_t nleaves)
+                                                        euler_scalars* __restrict__ sc, uint32_t nleaves)
 {
     using S          = patch_shape<R, N>;
     using T          = tile_shape<R, N>;
@@ -399,6 +426,7 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
     const int lv = key_level(keys[leaf]);
     double    dtdx[R];
     for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
+    double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
     for (int i = threadIdx.x; i < S::NI; i += NT)
     {
         int c[R];
@@ -424,8 +452,15 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
             euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
             for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
         }
-        for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = Uc[v] + total[v];
+        double o[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            o[v]                              = Uc[v] + total[v];
+            un[v * fstride + base + i] = o[v];
+        }
+        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
     }
+    if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
 }
 
 // Strip of B layers (along layout dim 0, the slowest) of an n^R patch with
@@ -461,7 +496,7 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
                                                          size_t fstride, const uint64_t* __restrict__ keys,
                                                          const face_link* __restrict__ links,
                                                          const uint32_t* __restrict__ metas, euler_params prm,
-                                                         const euler_scalars* __restrict__ sc, uint32_t nleaves)
+                                                         euler_scalars* __restrict__ sc, uint32_t nleaves)
 {
     using S          = patch_shape<R, N>;
     using T          = strip_shape<R, N, B>;
@@ -522,6 +557,7 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
     const int lv = key_level(keys[leaf]);
     double    dtdx[R];
     for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
+    double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
     for (int i = threadIdx.x; i < B * T::CL; i += NT)
     {
         int c[R], r = i % T::CL;
@@ -552,8 +588,15 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
             euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
             for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
         }
-        for (int v = 0; v < NV; ++v) un[v * fstride + first + i] = Uc[v] + total[v];
+        double o[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            o[v]                              = Uc[v] + total[v];
+            un[v * fstride + first + i] = o[v];
+        }
+        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
     }
+    if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
 }
 
 // Largest strip depth B (a divisor of n, at most n) whose strip tiles of all
