@@ -347,7 +347,8 @@ def main():
     from gpu_amr_amd import workloads
 
     if args.euler_only:
-        print(json.dumps({"euler_3d": euler_section(amr, 3, cpu_cycles=1)}), flush=True)
+        print(json.dumps({"euler_3d": euler_section(amr, 3, cpu_cycles=1),
+                          "euler_2d": euler_section(amr, 2, cpu_cycles=1)}), flush=True)
         return
 
     g, s = workloads.build_c3(amr)

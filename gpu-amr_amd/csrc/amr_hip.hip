@@ -240,7 +240,7 @@ struct instance
         // LDS tiles of every field where they fit, else LDS strips of the
         // patch, else neighbours from memory
         constexpr size_t tile_bytes = (R + 2) * tile_shape<R, N>::TS * sizeof(double);
-        constexpr int    B          = euler_strip_depth<R, N>(48 * 1024);
+        constexpr int    B          = euler_strip_depth<R, N>(40 * 1024);
         if constexpr (tile_bytes <= 48 * 1024)
             hipLaunchKernelGGL((k_euler_step_tile<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links,
                                metas, prm, sc, n);

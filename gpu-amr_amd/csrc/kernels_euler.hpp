@@ -489,8 +489,10 @@ struct strip_shape
 // 64^2: 139 KB): one workgroup per strip of B layers, so that a leaf is n / B
 // workgroups.  The strip tile holds the strip's cells, the neighbouring
 // layers of the same patch (or the face ghosts beyond the patch) and the
-// face ghosts along the other dims; every cell's update reads the same
-// operands as in k_euler_step_tile, so the results are the same bits.
+// face ghosts along the other dims.  The state (1/rho, pressure, sound
+// speed) of every tile cell is formed once into LDS and read by the cell and
+// its neighbours; every update reads the same operands as in
+// k_euler_step_tile, so the results are the same bits.
 template <int R, int N, int B, int NT>
 __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restrict__ u, double* __restrict__ un,
                                                          size_t fstride, const uint64_t* __restrict__ keys,
@@ -502,6 +504,7 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
     using T          = strip_shape<R, N, B>;
     constexpr int NV = R + 2;
     __shared__ double tile[NV][T::TS];
+    __shared__ double st_inv[T::TS], st_p[T::TS], st_a[T::TS];
     const uint32_t item = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
     const uint32_t leaf = item / T::NS;
     if (leaf >= nleaves) return;
@@ -554,6 +557,18 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
         for (int v = 0; v < NV; ++v) tile[v][q] = w[v];
     }
     __syncthreads();
+    // every tile cell's state once (edge / corner cells hold stale LDS
+    // values; their states are never read)
+    for (int q = threadIdx.x; q < T::TS; q += NT)
+    {
+        double U[NV];
+        for (int v = 0; v < NV; ++v) U[v] = tile[v][q];
+        const auto st = euler_flux<R>::state_of(U, prm.gamma);
+        st_inv[q]     = st.inv_rho;
+        st_p[q]       = st.p;
+        st_a[q]       = st.a;
+    }
+    __syncthreads();
     const int lv = key_level(keys[leaf]);
     double    dtdx[R];
     for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
@@ -574,7 +589,7 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
             Uc[v]    = tile[v][t];
             total[v] = 0.0;
         }
-        const auto stc = euler_flux<R>::state_of(Uc, prm.gamma);
+        const typename euler_flux<R>::state stc{st_inv[t], st_p[t], st_a[t]};
         for (int d = 0; d < R; ++d)
         {
             const int s = T::stride(R - 1 - d); // physical axis d
@@ -584,8 +599,10 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
                 UL[v] = tile[v][t - s];
                 UR[v] = tile[v][t + s];
             }
-            euler_flux<R>::rusanov(UL, euler_flux<R>::state_of(UL, prm.gamma), Uc, stc, fL, d);
-            euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
+            const typename euler_flux<R>::state stl{st_inv[t - s], st_p[t - s], st_a[t - s]};
+            const typename euler_flux<R>::state str{st_inv[t + s], st_p[t + s], st_a[t + s]};
+            euler_flux<R>::rusanov(UL, stl, Uc, stc, fL, d);
+            euler_flux<R>::rusanov(Uc, stc, UR, str, fR, d);
             for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
         }
         double o[NV];
@@ -599,13 +616,13 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
     if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
 }
 
-// Largest strip depth B (a divisor of n, at most n) whose strip tiles of all
-// R + 2 fields fit in `bytes` of LDS; 0 if none does.
+// Largest strip depth B (a divisor of n) whose strip tiles of the R + 2
+// fields and the three state arrays fit in `bytes` of LDS; 0 if none does.
 template <int R, int N>
 constexpr int euler_strip_depth(size_t bytes)
 {
     for (int b = N; b >= 1; --b)
-        if (N % b == 0 && (R + 2) * (b + 2) * static_cast<size_t>(R == 2 ? N + 2 : (N + 2) * (N + 2)) * 8 <= bytes)
+        if (N % b == 0 && (R + 5) * (b + 2) * static_cast<size_t>(R == 2 ? N + 2 : (N + 2) * (N + 2)) * 8 <= bytes)
             return b;
     return 0;
 }
