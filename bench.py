@@ -333,7 +333,8 @@ def main():
     ap.add_argument("--variant", default="", help="step-kernel variant to run instead of autotuning")
     ap.add_argument("--no-fusion", action="store_true", help="one step per launch (no two-step fusion)")
     ap.add_argument("--no-euler", action="store_true")
-    ap.add_argument("--euler-only", action="store_true", help="only the Euler section (profiling)")
+    ap.add_argument("--euler-only", nargs="?", const="both", choices=["both", "3d", "2d"],
+                    help="only the Euler benchmarks (profiling): both, or the 3D or 2D one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the leaves would shard by contiguous "
@@ -347,8 +348,12 @@ def main():
     from gpu_amr_amd import workloads
 
     if args.euler_only:
-        print(json.dumps({"euler_3d": euler_section(amr, 3, cpu_cycles=1),
-                          "euler_2d": euler_section(amr, 2, cpu_cycles=1)}), flush=True)
+        out = {}
+        if args.euler_only in ("both", "3d"):
+            out["euler_3d"] = euler_section(amr, 3, cpu_cycles=1)
+        if args.euler_only in ("both", "2d"):
+            out["euler_2d"] = euler_section(amr, 2, cpu_cycles=1)
+        print(json.dumps(out), flush=True)
         return
 
     g, s = workloads.build_c3(amr)
