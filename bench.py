@@ -241,40 +241,42 @@ def adjoint_section(amr, steps=300, checkpoint_every=20, reps=3):
     }
 
 
-def euler_section(amr, rank=3, cpu_cycles=6, threads=16):
-    """The reference's Euler benchmark (bench_fvm_solver_integration3D.b.cpp
-    in 3D): acoustic pulse on the adaptive tree, advance_batch_async(every,
-    tmax - t), regrid by density, finish_advance_batch, until tmax; its own
-    metric, cell-updates/s including the regrids (executed steps x leaves
-    before the regrid x n^R, over the loop's wall-clock).  The CPU oracle
-    runs the first `cpu_cycles` cycles of the same loop as the baseline."""
+def euler_section(amr, name="3d", cpu_cycles=6, threads=16):
+    """One of the reference's Euler benchmarks (`name`: "3d", "2d",
+    "2d_active"; bench_fvm_solver_integration{3D,,_active_amr}.b.cpp): the
+    pulse(s) on the adaptive tree, advance_batch_async(every, tmax - t),
+    regrid by density, finish_advance_batch, until tmax; its own metric,
+    cell-updates/s including the regrids (executed steps x leaves before the
+    regrid x n^R, over the loop's wall-clock).  The CPU oracle runs the first
+    `cpu_cycles` cycles of the same loop as the baseline."""
     from gpu_amr_amd import workloads
 
-    solver, crit, cfg = workloads.euler_benchmark(amr, rank)
+    solver, crit, cfg = workloads.euler_benchmark(amr, name)
+    rank = cfg["rank"]
     tree = solver.tree
     ni = cfg["patch"] ** rank
 
     def loop(s, t_max, max_cycles):
-        t, upd, steps, cycles, lo, hi = 0.0, 0, 0, 0, s.tree.size(), s.tree.size()
+        t, upd, steps, cycles, changed, lo, hi = 0.0, 0, 0, 0, 0, s.tree.size(), s.tree.size()
         while t < t_max and cycles < max_cycles:
             before = s.tree.size()
             s.advance_batch_async(cfg["every"], t_max - t)
-            s.tree.reconstruct_tree(crit)
+            changed += bool(s.tree.reconstruct_tree(crit))
             dt, ex = s.finish_advance_batch()
             t += dt
             upd += ex * before * ni
             steps += ex
             cycles += 1
             lo, hi = min(lo, s.tree.size()), max(hi, s.tree.size())
-        return t, upd, steps, cycles, lo, hi
+        return t, upd, steps, cycles, changed, lo, hi
 
     tree.synchronize()
     t0 = time.perf_counter()
-    t, upd, steps, cycles, lo, hi = loop(solver, cfg["tmax"], 100000)
+    t, upd, steps, cycles, changed, lo, hi = loop(solver, cfg["tmax"], 100000)
     wall = time.perf_counter() - t0
     rate = upd / wall
-    log(f"euler {rank}D: t={t:.3f} in {steps} steps / {cycles} regrids, leaves {lo}..{hi}, "
-        f"{wall * 1e3:.1f} ms, {rate:.3e} cell-updates/s (incl. regrid)")
+    log(f"euler {name}: t={t:.3f} in {steps} steps / {cycles} regrids ({changed} changed the tree), "
+        f"leaves {lo}..{hi}, {wall * 1e3:.1f} ms, {rate:.3e} cell-updates/s (incl. regrid)")
     tree.close()
 
     # CPU baseline: the oracle on the first cpu_cycles cycles
@@ -286,7 +288,7 @@ def euler_section(amr, rank=3, cpu_cycles=6, threads=16):
     o.set_params(1.4, 0.3, threads=threads)
     for _ in range(cfg["initial"]):
         o.refine_all()
-    prim = workloads.euler_pulse(rank)(o.cell_centers())
+    prim = workloads.euler_ic(cfg)(o.cell_centers())
     cons = euler_prim_to_cons(rank, prim.reshape(-1, rank + 2), 1.4).reshape(prim.shape)
     o.set_interior(np.moveaxis(cons, -1, 0))
     o.halo_exchange()
@@ -295,22 +297,25 @@ def euler_section(amr, rank=3, cpu_cycles=6, threads=16):
     for _ in range(cpu_cycles):
         before = o.size()
         dt, ex = o.advance_batch(cfg["every"], cfg["tmax"] - ot)
-        o.reconstruct(o.criterion(0, 0.53, 0.49, 1, cfg["max_level"]))
+        o.reconstruct(o.criterion(0, cfg["refine"], cfg["coarsen"], cfg["min_level"], cfg["max_level"]))
         o.halo_exchange()
         ot += dt
         oupd += ex * before * ni
         osteps += ex
     cwall = time.perf_counter() - t0
-    log(f"euler {rank}D cpu: {osteps} steps in {cwall:.1f} s, {oupd / cwall:.3e} cell-updates/s on {threads} threads")
+    log(f"euler {name} cpu: {osteps} steps in {cwall:.1f} s, {oupd / cwall:.3e} cell-updates/s on {threads} threads")
+    ic = "two pulses" if cfg["ic"] == "two_pulses" else "acoustic pulse"
     return {
-        "workload": f"reference Euler benchmark {rank}D: acoustic pulse, {cfg['patch']}^{rank} FP64 patches, "
-                    f"domain 1000^{rank}, gamma 1.4, cfl 0.3, regrid every {cfg['every']} steps by density "
-                    f"(0.53 / 0.49, levels 1..{cfg['max_level']}), tmax {cfg['tmax']:g}",
+        "workload": f"reference Euler benchmark {cfg['source']}: {ic}, {cfg['patch']}^{rank} FP64 patches, "
+                    f"domain 1000^{rank}, gamma 1.4, cfl 0.3, {cfg['initial']} refine-all, regrid every "
+                    f"{cfg['every']} steps by density ({cfg['refine']:g} / {cfg['coarsen']:g}, levels "
+                    f"{cfg['min_level']}..{cfg['max_level']}), tmax {cfg['tmax']:g}",
         "metric": "cell-updates/s including regrids",
         "value": rate,
         "t_end": t,
         "steps": steps,
         "regrids": cycles,
+        "regrids_changing_topology": changed,
         "leaves_min": lo,
         "leaves_max": hi,
         "wall_ms": wall * 1e3,
@@ -333,8 +338,8 @@ def main():
     ap.add_argument("--variant", default="", help="step-kernel variant to run instead of autotuning")
     ap.add_argument("--no-fusion", action="store_true", help="one step per launch (no two-step fusion)")
     ap.add_argument("--no-euler", action="store_true")
-    ap.add_argument("--euler-only", nargs="?", const="both", choices=["both", "3d", "2d"],
-                    help="only the Euler benchmarks (profiling): both, or the 3D or 2D one")
+    ap.add_argument("--euler-only", nargs="?", const="all", choices=["all", "3d", "2d", "2d_active"],
+                    help="only the Euler benchmarks (profiling): all, or one of them")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the leaves would shard by contiguous "
@@ -348,12 +353,8 @@ def main():
     from gpu_amr_amd import workloads
 
     if args.euler_only:
-        out = {}
-        if args.euler_only in ("both", "3d"):
-            out["euler_3d"] = euler_section(amr, 3, cpu_cycles=1)
-        if args.euler_only in ("both", "2d"):
-            out["euler_2d"] = euler_section(amr, 2, cpu_cycles=1)
-        print(json.dumps(out), flush=True)
+        names = ["3d", "2d", "2d_active"] if args.euler_only == "all" else [args.euler_only]
+        print(json.dumps({f"euler_{n}": euler_section(amr, n, cpu_cycles=1) for n in names}), flush=True)
         return
 
     g, s = workloads.build_c3(amr)
@@ -427,7 +428,8 @@ def main():
 
     euler = None
     if not args.no_euler:
-        euler = {"3d": euler_section(amr, 3), "2d": euler_section(amr, 2, cpu_cycles=2)}
+        euler = {"3d": euler_section(amr, "3d"), "2d": euler_section(amr, "2d", cpu_cycles=2),
+                 "2d_active": euler_section(amr, "2d_active", cpu_cycles=4)}
 
     cpu = None
     if not args.no_cpu:

@@ -134,16 +134,29 @@ def c5_setup(amr, max_level=6, capacity=4096 + 64, max_adapt=8):
 
 
 # ---------------------------------------------------------------------------
-# Euler: the reference's acoustic-pulse benchmarks
-# (benchmark/bench_fvm_solver_integration.b.cpp, 2D: 64^2 patches, depth 7,
-# three refine-all, regrid every 10 steps, levels 1..6, tmax 50;
-# benchmark/bench_fvm_solver_integration3D.b.cpp, 3D: 8^3 patches, depth 5,
-# two refine-all, regrid every 5 steps, levels 1..5, tmax 200).  Domain
-# 1000^R, gamma 1.4, cfl 0.3, density criterion refine > 0.53 / coarsen < 0.49.
+# Euler: the reference's three benchmarks.  Domain 1000^R, gamma 1.4, cfl 0.3,
+# regrid by the maximum density over the padded patch (refine above / coarsen
+# below the thresholds, within [min_level, max_level]).
+#  - "3d": benchmark/bench_fvm_solver_integration3D.b.cpp: 8^3 patches, depth
+#    5, two refine-all, acoustic pulse, regrid every 5 steps, 0.53 / 0.49,
+#    levels 1..5, tmax 200;
+#  - "2d": benchmark/bench_fvm_solver_integration.b.cpp: 64^2 patches, depth
+#    7, three refine-all, acoustic pulse, regrid every 10 steps, 0.53 / 0.49,
+#    levels 1..6, tmax 50;
+#  - "2d_active": benchmark/bench_fvm_solver_integration_active_amr.b.cpp:
+#    64^2 patches, depth 7, two refine-all, two pulses, regrid every 2 steps,
+#    0.512 / 0.506, levels 1..7, tmax 30.
 # ---------------------------------------------------------------------------
 EULER_CONFIGS = {
-    2: dict(patch=64, depth=7, initial=3, every=10, max_level=6, tmax=50.0, capacity=10000),
-    3: dict(patch=8, depth=5, initial=2, every=5, max_level=5, tmax=200.0, capacity=50000),
+    "3d": dict(rank=3, patch=8, depth=5, initial=2, every=5, refine=0.53, coarsen=0.49, min_level=1,
+               max_level=5, tmax=200.0, capacity=50000, ic="pulse",
+               source="benchmark/bench_fvm_solver_integration3D.b.cpp"),
+    "2d": dict(rank=2, patch=64, depth=7, initial=3, every=10, refine=0.53, coarsen=0.49, min_level=1,
+               max_level=6, tmax=50.0, capacity=10000, ic="pulse",
+               source="benchmark/bench_fvm_solver_integration.b.cpp"),
+    "2d_active": dict(rank=2, patch=64, depth=7, initial=2, every=2, refine=0.512, coarsen=0.506, min_level=1,
+                      max_level=7, tmax=30.0, capacity=10000, ic="two_pulses",
+                      source="benchmark/bench_fvm_solver_integration_active_amr.b.cpp"),
 }
 
 
@@ -164,14 +177,45 @@ def euler_pulse(rank, length=1000.0):
     return ic
 
 
-def euler_benchmark(amr, rank):
-    """Solver (tree refined and initialised) and criterion of the reference's
-    Euler benchmark in `rank` dimensions."""
-    cfg = EULER_CONFIGS[rank]
+def euler_two_pulses(length=1000.0):
+    """The active-AMR benchmark's 2D initial state
+    (bench_fvm_solver_integration_active_amr.b.cpp:145-180): two pulses,
+    a = 12 exp(-|x - (0.35, 0.45) L|^2 / w) + 9 exp(-|x - (0.68, 0.58) L|^2 / w),
+    w = 0.004 L^2; rho = 0.5 + 0.2 a, u = v = 0, p = 1 + a."""
+    w = 0.004 * length * length
+
+    def ic(c):
+        dxa, dya = c[..., 0] - 0.35 * length, c[..., 1] - 0.45 * length
+        dxb, dyb = c[..., 0] - 0.68 * length, c[..., 1] - 0.58 * length
+        pert = 12.0 * np.exp(-(dxa * dxa + dya * dya) / w) + 9.0 * np.exp(-(dxb * dxb + dyb * dyb) / w)
+        prim = np.zeros(c.shape[:-1] + (4,))
+        prim[..., 0] = 0.5 + pert * 0.2
+        prim[..., 3] = 1.0 + pert
+        return prim
+
+    return ic
+
+
+def euler_ic(cfg):
+    """Primitive initial state of an Euler benchmark configuration."""
+    if cfg["ic"] == "two_pulses":
+        return euler_two_pulses()
+    return euler_pulse(cfg["rank"])
+
+
+def euler_benchmark(amr, name):
+    """Solver (tree refined and initialised), criterion and configuration of
+    one of the reference's Euler benchmarks ("3d", "2d", "2d_active"; a rank
+    selects "3d" / "2d")."""
+    if not isinstance(name, str):
+        name = f"{int(name)}d"
+    cfg = EULER_CONFIGS[name]
+    rank = cfg["rank"]
     L = (1000.0,) * rank
     solver = amr.EulerSolver(rank, cfg["depth"], cfg["patch"], 1, cfg["capacity"], L, 1.4, 0.3)
     for _ in range(cfg["initial"]):
         solver.tree.refine_all()
-    solver.initialize(euler_pulse(rank))
-    crit = amr.make_criterion(amr.CRITERION_MAX_VALUE, 0.53, 0.49, 1, cfg["max_level"])
+    solver.initialize(euler_ic(cfg))
+    crit = amr.make_criterion(amr.CRITERION_MAX_VALUE, cfg["refine"], cfg["coarsen"], cfg["min_level"],
+                              cfg["max_level"])
     return solver, crit, cfg

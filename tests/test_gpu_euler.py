@@ -175,6 +175,44 @@ def test_euler_reference_3d_benchmark_start(amr):
     say("leaves:", s.tree.size(), "t =", t)
 
 
+def test_euler_reference_active_amr_start(amr):
+    """The reference's active-AMR Euler benchmark
+    (bench_fvm_solver_integration_active_amr.b.cpp: 64^2 patches, depth 7,
+    domain 1000^2, two refine-all, two pulses, regrid every 2 steps by max
+    density 0.512 / 0.506 on levels 1..7): its first 8 cycles, with the
+    identity / topology-changing regrid count."""
+    from gpu_amr_amd import workloads
+
+    say("case: Euler 2D active-AMR reference benchmark, 8 cycles")
+    s, crit, cfg = workloads.euler_benchmark(amr, "2d_active")
+    o = OracleEuler(2, 64, 1, 7, 2 * cfg["capacity"] + 64, (1000.0, 1000.0))
+    o.set_params(1.4, 0.3, threads=THREADS)
+    for _ in range(cfg["initial"]):
+        o.refine_all()
+    assert_topology(s, o, "initial")
+    prim = workloads.euler_two_pulses()(o.cell_centers())
+    cons = euler_prim_to_cons(2, prim.reshape(-1, 4), 1.4).reshape(prim.shape)
+    o.set_interior(np.moveaxis(cons, -1, 0))
+    o.halo_exchange()
+    assert_fields(s.tree.get_interior(), o.get_interior(), "initial state")
+    t, changed = 0.0, 0
+    for it in range(8):
+        s.advance_batch_async(cfg["every"], cfg["tmax"] - t)
+        got = s.finish_advance_batch()
+        exp = o.advance_batch(cfg["every"], cfg["tmax"] - t)
+        assert got == exp, (it, got, exp)
+        t += got[0]
+        fo = o.criterion(0, cfg["refine"], cfg["coarsen"], cfg["min_level"], cfg["max_level"])
+        np.testing.assert_array_equal(s.tree.compute_flags(crit), fo)
+        got_changed = s.tree.reconstruct_tree(crit)
+        assert got_changed == o.reconstruct(fo), it
+        o.halo_exchange()
+        changed += got_changed
+        assert_topology(s, o, f"cycle {it}")
+        assert_fields(s.tree.get_interior(), o.get_interior(), f"cycle {it}")
+    say("leaves:", s.tree.size(), "t =", t, "topology-changing regrids:", changed)
+
+
 def test_euler_rejects_advection_only_paths(amr):
     say("case: Euler context on advection-only entry points")
     s = amr.EulerSolver(2, 7, 16, 1, 32)
