@@ -241,6 +241,81 @@ def adjoint_section(amr, steps=300, checkpoint_every=20, reps=3):
     }
 
 
+def c2_section(amr, cycles=50, cpu_cycles=50, threads=16):
+    """C2, 2D linear advection with active AMR: 16^2 FP64 patches from
+    uniform depth 3, the 8-bump field (seed 2026), v = (1, 0.5), CFL 0.4, a
+    regrid by the undivided gradient (refine > 0.05, coarsen < 0.01, levels
+    3..7, 2:1) then 10 steps, 50 times (500 steps).  Cell-updates/s over the
+    loop's wall-clock (regrids included) and over the step launches alone
+    (HIP events around each batch of 10 steps on the context's stream); the
+    oracle on the first `cpu_cycles` cycles as the CPU baseline."""
+    from gpu_amr_amd import workloads
+
+    g = amr.NdTree(2, 7, 16, 1, 16384 + 64)
+    s = amr.AmrSolver(g, (1.0, 0.5), 0.4)
+    for _ in range(3):
+        g.refine_all()
+    ic = workloads.c2_mixture()
+    s.initialize(ic)
+    crit = amr.make_criterion(amr.CRITERION_GRADIENT, 0.05, 0.01, 3, 7)
+    ev = HipEvents()
+    marks = [(ev.create(), ev.create()) for _ in range(cycles)]
+    st = g.stream()
+    g.synchronize()
+    upd, lo, hi = 0, g.size(), g.size()
+    t0 = time.perf_counter()
+    for it in range(cycles):
+        g.reconstruct_tree(crit)
+        n = g.size()
+        ev.record(marks[it][0], st)
+        s.advance_batch_async(10)
+        ev.record(marks[it][1], st)
+        _, ex = s.finish_advance_batch()
+        upd += ex * n * 256
+        lo, hi = min(lo, n), max(hi, n)
+    wall = time.perf_counter() - t0
+    step_s = sum(ev.elapsed_ms(a, b) for a, b in marks) * 1e-3
+    log(f"c2: {cycles * 10} steps / {cycles} regrids, leaves {lo}..{hi}, {wall * 1e3:.1f} ms, "
+        f"{upd / wall:.3e} cell-updates/s incl. regrid, {upd / step_s:.3e} in the steps ({step_s * 1e3:.1f} ms)")
+    g.close()
+
+    if str(ROOT) not in sys.path:
+        sys.path.insert(0, str(ROOT))
+    from oracle import OracleTree
+
+    o = OracleTree(2, 16, 1, 7, 2 * 16384 + 128)
+    o.set_params((1.0, 0.5), 0.4, threads=threads)
+    for _ in range(3):
+        o.refine_all()
+    o.set_interior(ic(o.cell_centers()))
+    o.halo_exchange()
+    oupd = 0
+    t0 = time.perf_counter()
+    for _ in range(cpu_cycles):
+        o.reconstruct(o.criterion(1, 0.05, 0.01, 3, 7))
+        o.halo_exchange()
+        n = o.size()
+        o.advance_batch(10)
+        oupd += 10 * n * 256
+    cwall = time.perf_counter() - t0
+    log(f"c2 cpu: {cpu_cycles * 10} steps in {cwall:.2f} s, {oupd / cwall:.3e} cell-updates/s on {threads} threads")
+    return {
+        "workload": "C2: 2D linear advection with active AMR, 16^2 FP64 patches, depth 3..7, 8 Gaussian bumps "
+                    "(seed 2026), v = (1, 0.5), CFL 0.4, regrid every 10 steps by the undivided gradient "
+                    "(0.05 / 0.01), 500 steps",
+        "metric": "cell-updates/s",
+        "value": upd / wall,
+        "value_steps_only": upd / step_s,
+        "wall_ms": wall * 1e3,
+        "steps_ms": step_s * 1e3,
+        "leaves_min": lo,
+        "leaves_max": hi,
+        "cpu_baseline": {"value": oupd / cwall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                         "sample": f"first {cpu_cycles} regrid cycles ({cpu_cycles * 10} steps) of the same loop, "
+                                   f"{cwall:.2f} s"},
+    }
+
+
 def euler_section(amr, name="3d", cpu_cycles=6, threads=16):
     """One of the reference's Euler benchmarks (`name`: "3d", "2d",
     "2d_active"; bench_fvm_solver_integration{3D,,_active_amr}.b.cpp): the
@@ -338,6 +413,7 @@ def main():
     ap.add_argument("--variant", default="", help="step-kernel variant to run instead of autotuning")
     ap.add_argument("--no-fusion", action="store_true", help="one step per launch (no two-step fusion)")
     ap.add_argument("--no-euler", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--euler-only", nargs="?", const="all", choices=["all", "3d", "2d", "2d_active"],
                     help="only the Euler benchmarks (profiling): all, or one of them")
     args = ap.parse_args()
@@ -426,6 +502,10 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_section(amr)
 
+    c2 = None
+    if not args.no_c2:
+        c2 = c2_section(amr)
+
     euler = None
     if not args.no_euler:
         euler = {"3d": euler_section(amr, "3d"), "2d": euler_section(amr, "2d", cpu_cycles=2),
@@ -487,6 +567,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "c2": c2,
         "euler": euler,
     }
     print(json.dumps(out), flush=True)

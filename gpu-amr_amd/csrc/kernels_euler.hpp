@@ -485,6 +485,111 @@ struct strip_shape
     __host__ __device__ static constexpr int stride(int j) { return j == 0 ? PL : (j == R - 1 ? 1 : TP); }
 };
 
+// Cell q of a strip tile (strip_shape<R, N, B>, first layer r0): its patch
+// coordinates in c; returns -1 for a cell of the patch, the face direction
+// for a face-ghost cell and -2 for an edge or corner cell (never read).
+template <int R, int N, int B>
+__device__ inline int strip_cell(int q, int r0, int* c)
+{
+    using T = strip_shape<R, N, B>;
+    int r   = q;
+    for (int d = R - 1; d >= 0; --d)
+    {
+        const int ext = (d == 0) ? B + 2 : T::TP;
+        c[d]          = r % ext - 1;
+        r /= ext;
+    }
+    c[0] += r0;
+    int outside = 0, dir = 0;
+    for (int d = 0; d < R; ++d)
+    {
+        if (c[d] < 0)
+        {
+            ++outside;
+            dir = 2 * d;
+        }
+        else if (c[d] >= N)
+        {
+            ++outside;
+            dir = 2 * d + 1;
+        }
+    }
+    return outside == 0 ? -1 : (outside == 1 ? dir : -2);
+}
+
+// The update of one staged strip (tile and LDS states formed): every cell's
+// Rusanov update as in k_euler_step_tile, stored to the next state, and the
+// next step's dt minimum over the new values folded into dtn.
+template <int R, int N, int B, int NT>
+__device__ inline void strip_update(const double (*tile)[strip_shape<R, N, B>::TS], const double* st_inv,
+                                    const double* st_p, const double* st_a, int r0, size_t first, int lv, double dt,
+                                    euler_params const& prm, double* __restrict__ un, size_t fstride, double& dtn)
+{
+    using T          = strip_shape<R, N, B>;
+    using F          = euler_flux<R>;
+    constexpr int NV = R + 2;
+    double        dtdx[R];
+    for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
+    for (int i = threadIdx.x; i < B * T::CL; i += NT)
+    {
+        int c[R], r = i % T::CL;
+        c[0] = r0 + i / T::CL;
+        for (int d = R - 1; d >= 1; --d)
+        {
+            c[d] = r % N;
+            r /= N;
+        }
+        const int t = T::idx(c, r0);
+        double    Uc[NV], total[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            Uc[v]    = tile[v][t];
+            total[v] = 0.0;
+        }
+        const typename F::state stc{st_inv[t], st_p[t], st_a[t]};
+        for (int d = 0; d < R; ++d)
+        {
+            const int s = T::stride(R - 1 - d); // physical axis d
+            double    UL[NV], UR[NV], fL[NV], fR[NV];
+            for (int v = 0; v < NV; ++v)
+            {
+                UL[v] = tile[v][t - s];
+                UR[v] = tile[v][t + s];
+            }
+            const typename F::state stl{st_inv[t - s], st_p[t - s], st_a[t - s]};
+            const typename F::state str{st_inv[t + s], st_p[t + s], st_a[t + s]};
+            F::rusanov(UL, stl, Uc, stc, fL, d);
+            F::rusanov(Uc, stc, UR, str, fR, d);
+            for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
+        }
+        double o[NV];
+        for (int v = 0; v < NV; ++v)
+        {
+            o[v]                     = Uc[v] + total[v];
+            un[v * fstride + first + i] = o[v];
+        }
+        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
+    }
+}
+
+// Every tile cell's state (1/rho, pressure, sound speed) into LDS (edge /
+// corner cells hold stale values; their states are never read).
+template <int R, int N, int B, int NT>
+__device__ inline void strip_states(const double (*tile)[strip_shape<R, N, B>::TS], double* st_inv, double* st_p,
+                                    double* st_a, double gamma)
+{
+    constexpr int NV = R + 2;
+    for (int q = threadIdx.x; q < strip_shape<R, N, B>::TS; q += NT)
+    {
+        double U[NV];
+        for (int v = 0; v < NV; ++v) U[v] = tile[v][q];
+        const auto st = euler_flux<R>::state_of(U, gamma);
+        st_inv[q]     = st.inv_rho;
+        st_p[q]       = st.p;
+        st_a[q]       = st.a;
+    }
+}
+
 // k_euler_step_tile for patches whose whole tile does not fit in LDS (2D
 // 64^2: 139 KB): one workgroup per strip of B layers, so that a leaf is n / B
 // workgroups.  The strip tile holds the strip's cells, the neighbouring
@@ -520,99 +625,27 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
     }
     for (int q = threadIdx.x; q < T::TS; q += NT)
     {
-        int c[R], r = q;
-        for (int d = R - 1; d >= 0; --d)
-        {
-            const int ext = (d == 0) ? B + 2 : T::TP;
-            c[d]          = r % ext - 1;
-            r /= ext;
-        }
-        c[0] += r0;
-        int outside = 0, dir = 0;
-        for (int d = 0; d < R; ++d)
-        {
-            if (c[d] < 0)
-            {
-                ++outside;
-                dir = 2 * d;
-            }
-            else if (c[d] >= N)
-            {
-                ++outside;
-                dir = 2 * d + 1;
-            }
-        }
-        if (outside > 1) continue; // edge / corner cells are never read
+        int       c[R];
+        const int kind = strip_cell<R, N, B>(q, r0, c);
+        if (kind == -2) continue; // edge / corner cells are never read
         double w[NV];
-        if (outside == 0)
+        if (kind == -1)
         {
             const size_t at = base + S::lin(c);
             for (int v = 0; v < NV; ++v) w[v] = u[v * fstride + at];
         }
         else
         {
-            const size_t e = static_cast<size_t>(leaf) * S::NDIR + dir;
-            ghost_values<R, N, NV>(u, fstride, links[e], metas[e], dir, c, w);
+            const size_t e = static_cast<size_t>(leaf) * S::NDIR + kind;
+            ghost_values<R, N, NV>(u, fstride, links[e], metas[e], kind, c, w);
         }
         for (int v = 0; v < NV; ++v) tile[v][q] = w[v];
     }
     __syncthreads();
-    // every tile cell's state once (edge / corner cells hold stale LDS
-    // values; their states are never read)
-    for (int q = threadIdx.x; q < T::TS; q += NT)
-    {
-        double U[NV];
-        for (int v = 0; v < NV; ++v) U[v] = tile[v][q];
-        const auto st = euler_flux<R>::state_of(U, prm.gamma);
-        st_inv[q]     = st.inv_rho;
-        st_p[q]       = st.p;
-        st_a[q]       = st.a;
-    }
+    strip_states<R, N, B, NT>(tile, st_inv, st_p, st_a, prm.gamma);
     __syncthreads();
-    const int lv = key_level(keys[leaf]);
-    double    dtdx[R];
-    for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
     double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
-    for (int i = threadIdx.x; i < B * T::CL; i += NT)
-    {
-        int c[R], r = i % T::CL;
-        c[0] = r0 + i / T::CL;
-        for (int d = R - 1; d >= 1; --d)
-        {
-            c[d] = r % N;
-            r /= N;
-        }
-        const int t = T::idx(c, r0);
-        double    Uc[NV], total[NV];
-        for (int v = 0; v < NV; ++v)
-        {
-            Uc[v]    = tile[v][t];
-            total[v] = 0.0;
-        }
-        const typename euler_flux<R>::state stc{st_inv[t], st_p[t], st_a[t]};
-        for (int d = 0; d < R; ++d)
-        {
-            const int s = T::stride(R - 1 - d); // physical axis d
-            double    UL[NV], UR[NV], fL[NV], fR[NV];
-            for (int v = 0; v < NV; ++v)
-            {
-                UL[v] = tile[v][t - s];
-                UR[v] = tile[v][t + s];
-            }
-            const typename euler_flux<R>::state stl{st_inv[t - s], st_p[t - s], st_a[t - s]};
-            const typename euler_flux<R>::state str{st_inv[t + s], st_p[t + s], st_a[t + s]};
-            euler_flux<R>::rusanov(UL, stl, Uc, stc, fL, d);
-            euler_flux<R>::rusanov(Uc, stc, UR, str, fR, d);
-            for (int k = 0; k < NV; ++k) total[k] -= dtdx[d] * (fR[k] - fL[k]);
-        }
-        double o[NV];
-        for (int v = 0; v < NV; ++v)
-        {
-            o[v]                              = Uc[v] + total[v];
-            un[v * fstride + first + i] = o[v];
-        }
-        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
-    }
+    strip_update<R, N, B, NT>(tile, st_inv, st_p, st_a, r0, first, key_level(keys[leaf]), dt, prm, un, fstride, dtn);
     if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
 }
 
