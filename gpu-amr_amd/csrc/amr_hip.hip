@@ -575,9 +575,12 @@ struct amr_ctx
 
     void sync() { AMR_HIP_CHECK(hipStreamSynchronize(stream)); }
 
-    void build_table()
+    // Neighbour records of the current leaves.  After a regrid the summary
+    // still holds the emit's deepest level (and no error bits: the regrid
+    // stops before the emit otherwise), so one read-back serves both.
+    void build_table(bool after_regrid = false)
     {
-        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
+        if (!after_regrid) AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
         const uint64_t items = n * static_cast<uint64_t>(ndir);
         if (rank == 2)
             hipLaunchKernelGGL(k_build_table<2>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[kc],
@@ -594,6 +597,7 @@ struct amr_ctx
             broken = true;
             throw amr_error(AMR_ERR_TOPOLOGY, "leaves do not tile the domain or violate 2:1 face balance");
         }
+        if (after_regrid) max_level = static_cast<int>(h_summary->max_level);
         runs.build(rank, keys[kc], static_cast<uint32_t>(n), depth, links, metas, stream);
     }
 
@@ -776,10 +780,7 @@ struct amr_ctx
         cur = 1 - cur;
         kc  = 1 - kc;
         n   = s.total;
-        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
-        sync();
-        max_level = static_cast<int>(h_summary->max_level);
-        build_table();
+        build_table(true);
         return true;
     }
 };
