@@ -734,13 +734,11 @@ struct amr_ctx
 
     void scan(uint32_t count) { exclusive_scan(counts, pos, count, block_sums, summary, stream); }
 
-    // ndtree::reconstruct_tree (ndtree.hpp:1249-1271) with the flags already
-    // in device memory.
-    bool regrid()
+    // The regrid's planning passes as one launch per pass (any tree size):
+    // refine marks, the closure per level, coarsening families, output
+    // counts, their scan.
+    void plan_regrid(uint32_t nn)
     {
-        ensure_usable();
-        const uint32_t nn = static_cast<uint32_t>(n);
-        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
         hipLaunchKernelGGL(k_mark_refine, dim3(grid1d(nn, 256)), dim3(256), 0, stream, flags, keys[kc], nn, depth, mark);
         check_launch("k_mark_refine");
         for (int l = std::min(max_level, depth); l >= 1; --l)
@@ -769,6 +767,29 @@ struct amr_ctx
         }
         check_launch("k_coarsen_families / k_emit_counts");
         scan(nn);
+    }
+
+    // ndtree::reconstruct_tree (ndtree.hpp:1249-1271) with the flags already
+    // in device memory.
+    bool regrid()
+    {
+        ensure_usable();
+        const uint32_t nn = static_cast<uint32_t>(n);
+        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
+        if (nn <= static_cast<uint32_t>(kPlanMaxLeaves))
+        {
+            // small tree: every planning pass in one workgroup
+            const int top = std::min(max_level, depth);
+            if (rank == 2)
+                hipLaunchKernelGGL(k_regrid_plan<2>, dim3(1), dim3(kPlanThreads), 0, stream, flags, keys[kc], nn,
+                                   depth, top, links, metas, mark, head, counts, pos, summary);
+            else
+                hipLaunchKernelGGL(k_regrid_plan<3>, dim3(1), dim3(kPlanThreads), 0, stream, flags, keys[kc], nn,
+                                   depth, top, links, metas, mark, head, counts, pos, summary);
+            check_launch("k_regrid_plan");
+        }
+        else
+            plan_regrid(nn);
         AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
         sync();
         const regrid_summary s = *h_summary;

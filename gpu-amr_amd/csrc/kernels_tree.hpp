@@ -403,9 +403,10 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems   = 4;
 constexpr int kScanBlock   = kScanThreads * kScanItems;
 
+template <int NT = kScanThreads>
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
 {
-    __shared__ uint32_t wsum[kScanThreads / 64];
+    __shared__ uint32_t wsum[NT / 64];
     const int           lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t            x    = v;
     for (int d = 1; d < 64; d <<= 1)
@@ -416,7 +417,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
     if (lane == 63) wsum[w] = x;
     __syncthreads();
     uint32_t off = 0, tot = 0;
-    for (int j = 0; j < kScanThreads / 64; ++j)
+    for (int j = 0; j < NT / 64; ++j)
     {
         if (j < w) off += wsum[j];
         tot += wsum[j];
@@ -424,6 +425,66 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
     __syncthreads();
     *total = tot;
     return off + x - v;
+}
+
+// The regrid's planning passes for a tree of at most kPlanMaxLeaves leaves in
+// one workgroup: refine marks, the 2:1 closure level by level from
+// `top_level` down, the coarsening families, the output counts and their
+// exclusive scan.  The per-leaf functions are those of k_mark_refine,
+// k_close_level, k_coarsen_families, k_emit_counts and the scan; workgroup
+// barriers take the place of the launches between the passes, so a small
+// tree's regrid costs one launch instead of about a dozen.
+constexpr int kPlanThreads   = 1024;
+constexpr int kPlanMaxLeaves = 4 * kPlanThreads;
+
+template <int R>
+__global__ __launch_bounds__(kPlanThreads) void k_regrid_plan(
+    const int8_t* __restrict__ flags, const uint64_t* __restrict__ keys, uint32_t n, int depth, int top_level,
+    const face_link* __restrict__ links, const uint32_t* __restrict__ metas, uint8_t* mark, uint8_t* head,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ pos, regrid_summary* __restrict__ summary)
+{
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < n; i += kPlanThreads) mark[i] = refine_mark(flags[i], keys[i], depth);
+    __syncthreads();
+    for (int l = top_level; l >= 1; --l)
+    {
+        for (uint32_t i = t; i < n; i += kPlanThreads) close_item<R>(keys, l, links, metas, mark, i);
+        __syncthreads();
+    }
+    uint32_t n_coarsen = 0, n_refine = 0;
+    bool     root_any  = false;
+    for (uint32_t i = t; i < n; i += kPlanThreads)
+    {
+        bool       root = false;
+        const bool h    = coarsen_family_head<R>(flags, keys, n, depth, links, metas, mark, i, &root);
+        head[i]         = h ? 1 : 0;
+        n_coarsen += h ? 1u : 0u;
+        root_any = root_any || root;
+    }
+    __syncthreads();
+    // counts over one contiguous chunk per thread, then the exclusive scan
+    const uint32_t per = (n + kPlanThreads - 1) / kPlanThreads;
+    const uint32_t lo  = t * per < n ? t * per : n;
+    const uint32_t hi  = lo + per < n ? lo + per : n;
+    uint32_t       sum = 0;
+    for (uint32_t i = lo; i < hi; ++i)
+    {
+        const uint32_t c = emit_count<R>(keys, depth, mark, head, i);
+        counts[i]        = c;
+        sum += c;
+        n_refine += mark[i] ? 1u : 0u;
+    }
+    uint32_t total;
+    uint32_t off = block_exclusive_scan<kPlanThreads>(sum, &total);
+    for (uint32_t i = lo; i < hi; ++i)
+    {
+        pos[i] = off;
+        off += counts[i];
+    }
+    if (n_coarsen) atomicAdd(&summary->n_coarsen, n_coarsen);
+    if (n_refine) atomicAdd(&summary->n_refine, n_refine);
+    if (root_any) atomicOr(&summary->err, 2u);
+    if (t == 0) summary->total = total;
 }
 
 __global__ void k_scan_blocks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,

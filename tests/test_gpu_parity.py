@@ -149,6 +149,41 @@ def test_regrid_sequences(amr, rank, n, h, depth, start, rounds, seed):
         assert_fields(g.get_interior(), o.get_interior(), f"round {rd}: 3 steps")
 
 
+def test_regrid_above_one_workgroup_planning(amr):
+    """Regrids of trees above 4096 leaves plan with one launch per pass (the
+    smaller ones in one workgroup): keys, transfer and steps against the
+    oracle on a 2D depth-7 tree of 4^2 patches that grows past the limit."""
+    say("case: regrids above the one-workgroup planning limit")
+    rank, n, depth = 2, 4, 7
+    cap = (1 << (rank * depth)) + 64
+    g, s, o = pair(amr, rank, n, 1, depth, cap, (1.0, 0.5), 0.4)
+    for _ in range(6):
+        g.refine_all()
+        o.refine_all()
+    rng = np.random.default_rng(11)
+    large = 0
+    for rd in range(6):
+        keys = o.keys()
+        v = rng.standard_normal(o.size() * n**rank)
+        g.set_interior(v)
+        o.set_interior(v)
+        if rd == 0:
+            flags = (rng.random(len(keys)) < 0.3).astype(np.int8)  # grow past 4096 leaves
+        else:
+            flags = random_flags(rng, keys, rank, depth)
+        large += g.size() > 4096
+        assert g.reconstruct_tree(flags) == o.reconstruct(flags), f"round {rd}: changed flag"
+        assert_topology(g, o, f"round {rd}")
+        assert_fields(g.get_interior(), o.get_interior(), f"round {rd}: transfer")
+        o.halo_exchange()
+        s.advance_batch_async(2)
+        s.finish_advance_batch()
+        o.advance_batch(2)
+        assert_fields(g.get_interior(), o.get_interior(), f"round {rd}: 2 steps")
+    say("regrids above 4096 leaves:", large, "final leaves:", g.size())
+    assert large >= 2
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_criterion_flags(amr, kind):
     say(f"case: criterion kind={kind}")
