@@ -862,17 +862,30 @@ __global__ __launch_bounds__(NT) void k_criterion_max(const double* __restrict__
                                                       int min_level, int max_level, int8_t* __restrict__ flags,
                                                       uint32_t nleaves)
 {
-    constexpr int  P    = N + 2 * H;
-    constexpr int  FLAT = (R == 2) ? P * P : P * P * P;
+    using S             = patch_shape<R, N>;
+    constexpr int     NH = S::NDIR * H * S::FACE; // face-halo cells
     __shared__ double red[NT];
-    const uint32_t leaf = blockIdx.x;
+    const uint32_t    leaf = blockIdx.x;
     if (leaf >= nleaves) return;
-    double m = 0.0;
-    for (int i = threadIdx.x; i < FLAT; i += NT)
+    // the interior with coalesced loads, then the face halos by the ghost
+    // rules (padded_value's cells of kind 0 and 1; the maximum is
+    // order-free)
+    double        m  = 0.0;
+    const double* up = u + static_cast<size_t>(leaf) * S::NI;
+    for (int i = threadIdx.x; i < S::NI; i += NT) m = fmax(m, up[i]);
+    for (int g = threadIdx.x; g < NH; g += NT)
     {
-        int          kind;
-        const double v = padded_value<R, N, H>(u, leaf, links, metas, i, &kind);
-        if (kind < 2) m = fmax(m, v);
+        const int dir = g / (H * S::FACE), layer = (g / S::FACE) % H, dim = dir >> 1;
+        int       c[R], r = g % S::FACE;
+        for (int d = R - 1; d >= 0; --d)
+        {
+            if (d == dim) continue;
+            c[d] = r % N;
+            r /= N;
+        }
+        c[dim]         = (dir & 1) ? N + layer : -1 - layer;
+        const size_t e = static_cast<size_t>(leaf) * S::NDIR + dir;
+        m              = fmax(m, ghost_value<R, N>(u, links[e], metas[e], dir, c));
     }
     m = block_max<NT>(m, red);
     if (threadIdx.x == 0) flags[leaf] = decide(m, key_level(keys[leaf]), rthr, cthr, min_level, max_level);

@@ -185,24 +185,27 @@ def test_regrid_above_one_workgroup_planning(amr):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_criterion_flags(amr, kind):
-    say(f"case: criterion kind={kind}")
-    g, s, o = pair(amr, 2, 16, 1, 7, 4200, (1.0, 0.5), 0.4)
-    for _ in range(3):
+@pytest.mark.parametrize("rank,n,h", [(2, 16, 1), (2, 10, 2), (3, 8, 1)])
+def test_criterion_flags(amr, kind, rank, n, h):
+    say(f"case: criterion kind={kind} rank={rank} n={n} h={h}")
+    depth = 7 if rank == 2 else 5
+    g, s, o = pair(amr, rank, n, h, depth, 4200, (1.0, 0.5, 0.25)[:rank], 0.4)
+    for _ in range(3 if rank == 2 else 2):
         g.refine_all()
         o.refine_all()
     rng = np.random.default_rng(11)
     for rd in range(4):
         cen = o.cell_centers()
-        c0 = rng.random(2)
-        u = np.exp(-((cen[..., 0] - c0[0]) ** 2 + (cen[..., 1] - c0[1]) ** 2) / (2 * 0.05**2))
+        c0 = rng.random(rank)
+        r2 = sum((cen[..., d] - c0[d]) ** 2 for d in range(rank))
+        u = np.exp(-r2 / (2 * 0.05**2))
         g.set_interior(u)
         o.set_interior(u)
         o.halo_exchange()
         thr = (0.5, 0.05) if kind == 0 else (0.05, 0.01)
-        crit = amr.make_criterion(kind, thr[0], thr[1], 2, 6)
+        crit = amr.make_criterion(kind, thr[0], thr[1], 2 if rank == 2 else 1, 6 if rank == 2 else 4)
         fg = g.compute_flags(crit)
-        fo = o.criterion(kind, thr[0], thr[1], 2, 6)
+        fo = o.criterion(kind, thr[0], thr[1], 2 if rank == 2 else 1, 6 if rank == 2 else 4)
         np.testing.assert_array_equal(fg, fo)
         assert g.reconstruct_tree(crit) == o.reconstruct(fo)
         assert_topology(g, o, f"criterion round {rd}")
