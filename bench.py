@@ -526,7 +526,8 @@ def main():
 
     if fused:
         kernel = "k_step2_3d8_uniform"
-        kernel_desc = "k_step2_3d8_uniform: two steps per launch (u^n read once, u^(n+2) written once)"
+        kernel_desc = ("k_step2_3d8_uniform: two steps per launch (u^n read once, u^(n+2) written once), "
+                       "one workgroup per 2x2x1 block of leaves")
     else:
         kernel = "k_step_3d8"
         kernel_desc = f"single-step kernel variant '{names[best]}' ({how})"

@@ -222,7 +222,16 @@ struct instance
     {
         if (n == 0) return;
         if constexpr (R == 3 && N == 8)
-            hipLaunchKernelGGL(k_step2_3d8_uniform<false>, dim3(xcd_grid(n)), dim3(64), 0, s, u, un, prm, n);
+        {
+            // levels >= 1: a workgroup per 2 x 2 x 1 block of sibling leaves
+            // (n = 8^level, so the blocks tile the slots); the root alone:
+            // one wave
+            if (n % 4 == 0)
+                hipLaunchKernelGGL((k_step2_3d8_uniform<false, 4>), dim3(xcd_grid(n / 4)), dim3(256), 0, s, u, un, prm,
+                                   n);
+            else
+                hipLaunchKernelGGL((k_step2_3d8_uniform<false, 1>), dim3(xcd_grid(n)), dim3(64), 0, s, u, un, prm, n);
+        }
         check_launch("k_step2");
     }
     static void euler_dt(const double* u, size_t fs, const uint64_t* keys, euler_params const& prm, euler_scalars* sc,

@@ -68,6 +68,23 @@ struct morton_masks3
 // hs = 0.5 |v|: upwind_flux(lo, hi, ...), for the shared-face update below.
 __device__ inline double face_flux(double lo, double hi, double v, double hs) { return upwind_flux(lo, hi, v, hs); }
 
+// Orders a wave's own LDS writes before its later reads.  With one wave per
+// workgroup this is the workgroup barrier; with several, each wave only
+// waits for itself (a wave's LDS operations complete in issue order), so
+// the waves of a workgroup meet only where they exchange data.
+template <int W>
+__device__ inline void leaf_sync()
+{
+    if constexpr (W == 1)
+        __syncthreads();
+    else
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    }
+}
+
 // Two steps of a single-level tree of 8^3 patches, one wave per leaf.
 //
 // Layouts.  Core: lane l = 8 y + x owns the z-column (y, x) in registers, as
@@ -83,17 +100,35 @@ __device__ inline double face_flux(double lo, double hi, double v, double hs) { 
 // (the ring cell's flux toward the core), step two's from a pass over the
 // new ring and core boundary values.  A face flux is the same bits whichever
 // cell evaluates it, so every value equals the single-step kernel's.
-template <bool NT>
-__global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restrict__ u, double* __restrict__ un,
-                                                          step2_params prm, uint32_t nleaves)
+//
+// W = 4: one workgroup of four waves per 2 x 2 x 1 block of leaves (Morton
+// slots 4b .. 4b + 3: x and y siblings).  A wave skips the ring passes of
+// its two faces inside the block: the step-one flux through such a face is
+// evaluated from the same two cells as the ring pass would (the ring cell's
+// flux toward the core), and step two takes the ring's u^{n+1} from the
+// neighbouring wave's core boundary layer in LDS, the value the ring pass
+// would have computed.  Every value is therefore the same bits as with W = 1.
+template <bool NT, int W = 1>
+__global__ __launch_bounds__(64 * W) void k_step2_3d8_uniform(const double* __restrict__ u, double* __restrict__ un,
+                                                              step2_params prm, uint32_t nleaves)
 {
+    static_assert(W == 1 || W == 4, "one leaf or a 2 x 2 x 1 block per workgroup");
     constexpr int NI = 512;
-    __shared__ double fbound[4][64]; // core boundary-face fluxes -y, +y, -x, +x (face layout)
-    __shared__ double bring[4][64];  // u^{n+1} of the ghost ring (face layout)
-    __shared__ double bcore[4][64];  // u^{n+1} of the core's boundary layers (face layout)
-    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
-    if (leaf >= nleaves) return;
-    const int           lane = threadIdx.x, hi = lane >> 3, lo = lane & 7;
+    __shared__ double fbound_w[W][4][64]; // core boundary-face fluxes -y, +y, -x, +x (face layout)
+    __shared__ double bring_w[W][4][64];  // u^{n+1} of the ghost ring (face layout)
+    __shared__ double bcore_w[W][4][64];  // u^{n+1} of the core's boundary layers (face layout)
+    const uint32_t item = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    if (item * W >= nleaves) return;
+    // wave within the workgroup, as a wave-uniform (scalar) value
+    const int      wv   = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const uint32_t leaf = item * W + wv;
+    double(*fbound)[64] = fbound_w[wv];
+    double(*bring)[64]  = bring_w[wv];
+    double(*bcore)[64]  = bcore_w[wv];
+    // faces inside the 2 x 2 x 1 block (W = 4): +x or -x, +y or -y
+    const int           fxi  = (W == 4) ? ((wv & 1) ? 2 : 3) : -1;
+    const int           fyi  = (W == 4) ? ((wv & 2) ? 0 : 1) : -1;
+    const int           lane = threadIdx.x & 63, hi = lane >> 3, lo = lane & 7;
     const bool          hlo = hi < 4, llo = lo < 4;
     const morton_masks3 mk(prm.level);
     auto                P = [&](uint32_t slot) { return u + static_cast<size_t>(slot) * NI; };
@@ -120,18 +155,25 @@ __global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restri
     double r[4], o[4], c[4], tq[4], tp[4];
     {
         const int zq = hi * 64 + lo; // (z, _, x) of a y face, without the row
-        r[0]  = cell(s_ym, zq + 7 * 8);
-        o[0]  = cell(s_ym, zq + 6 * 8);
-        c[0]  = own[zq];
-        r[1]  = cell(s_yp, zq);
-        o[1]  = cell(s_yp, zq + 8);
-        c[1]  = own[zq + 7 * 8];
-        // y faces: x-side edge in leaf (-1 or +1, -+1, 0) at column 7 or 0,
-        // z-side edge in leaf (0, -+1, -1 or +1) at plane 7 or 0
-        tq[0] = cell(llo ? s_xmym : s_xpym, hi * 64 + 7 * 8 + ecol);
-        tq[1] = cell(llo ? s_xmyp : s_xpyp, hi * 64 + ecol);
-        tp[0] = cell(hlo ? s_ymzm : s_ymzp, epl + 7 * 8 + lo);
-        tp[1] = cell(hlo ? s_ypzm : s_ypzp, epl + lo);
+        r[0] = cell(s_ym, zq + 7 * 8);
+        c[0] = own[zq];
+        r[1] = cell(s_yp, zq);
+        c[1] = own[zq + 7 * 8];
+        // y faces: second layer; x-side edge in leaf (-1 or +1, -+1, 0) at
+        // column 7 or 0, z-side edge in leaf (0, -+1, -1 or +1) at plane 7
+        // or 0 (only the ring passes read them: none inside a block)
+        if (fyi != 0)
+        {
+            o[0]  = cell(s_ym, zq + 6 * 8);
+            tq[0] = cell(llo ? s_xmym : s_xpym, hi * 64 + 7 * 8 + ecol);
+            tp[0] = cell(hlo ? s_ymzm : s_ymzp, epl + 7 * 8 + lo);
+        }
+        if (fyi != 1)
+        {
+            o[1]  = cell(s_yp, zq + 8);
+            tq[1] = cell(llo ? s_xmyp : s_xpyp, hi * 64 + ecol);
+            tp[1] = cell(hlo ? s_ypzm : s_ypzp, epl + lo);
+        }
         const int     zy = hi * 64 + lo * 8; // (z, y, _) of an x face
         const double2 pm = *reinterpret_cast<const double2*>(u + static_cast<size_t>(s_xm) * NI + zy + 6);
         const double2 pp = *reinterpret_cast<const double2*>(u + static_cast<size_t>(s_xp) * NI + zy);
@@ -143,10 +185,16 @@ __global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restri
         c[3]  = own[zy + 7];
         // x faces: y-side edge in leaf (-+1, -1 or +1, 0) at row 7 or 0,
         // z-side edge in leaf (-+1, 0, -1 or +1) at plane 7 or 0
-        tq[2] = cell(llo ? s_xmym : s_xmyp, hi * 64 + erow + 7);
-        tq[3] = cell(llo ? s_xpym : s_xpyp, hi * 64 + erow);
-        tp[2] = cell(hlo ? s_xmzm : s_xmzp, epl + lo * 8 + 7);
-        tp[3] = cell(hlo ? s_xpzm : s_xpzp, epl + lo * 8);
+        if (fxi != 2)
+        {
+            tq[2] = cell(llo ? s_xmym : s_xmyp, hi * 64 + erow + 7);
+            tp[2] = cell(hlo ? s_xmzm : s_xmzp, epl + lo * 8 + 7);
+        }
+        if (fxi != 3)
+        {
+            tq[3] = cell(llo ? s_xpym : s_xpyp, hi * 64 + erow);
+            tp[3] = cell(hlo ? s_xpzm : s_xpzp, epl + lo * 8);
+        }
     }
     // core column at z = -2..9, core layout (y, x) = (hi, lo)
     double a[12];
@@ -173,6 +221,12 @@ __global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restri
         const bool   neg = (f & 1) == 0;    // ring at -1 (else at 8)
         const int    nd  = (f < 2) ? 1 : 0; // physical axis of the face normal
         const int    qd  = 1 - nd;          // physical axis of q
+        if (f == fxi || f == fyi)
+        {
+            // inside the block: only the flux through the core's face
+            fbound[f][lane] = neg ? face_flux(r[f], c[f], v[nd], h[nd]) : face_flux(c[f], r[f], v[nd], h[nd]);
+            continue;
+        }
         const double uc  = r[f];
         // cross-lane reads first, with every lane active
         const double ql = lane_minus1(uc), qh = lane_plus1(uc);
@@ -194,7 +248,7 @@ __global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restri
         bring[f][lane]  = update_from_fluxes<3>(uc, fL, fR, prm.dtdx[0]);
         fbound[f][lane] = neg ? fR[nd] : fL[nd]; // through the core's boundary face
     }
-    __syncthreads();
+    leaf_sync<W>(); // this wave's ring values and fluxes
 
     // ---- step one: core column at z = -1..8 ---------------------------------
     const int y = hi, x = lo;
@@ -248,7 +302,7 @@ __global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restri
 #pragma unroll
         for (int z = 0; z < 8; ++z) bcore[y == 0 ? 0 : 1][z * 8 + x] = b[1 + z];
     }
-    __syncthreads();
+    __syncthreads(); // every wave's core boundary layers
 
     // ---- step two: boundary-face fluxes, face layout ------------------------
 #pragma unroll
@@ -256,10 +310,15 @@ __global__ __launch_bounds__(64) void k_step2_3d8_uniform(const double* __restri
     {
         const bool   neg = (f & 1) == 0;
         const int    nd  = (f < 2) ? 1 : 0;
-        const double rv = bring[f][lane], cv = bcore[f][lane];
+        // inside the block, the ring's u^{n+1} is the neighbouring wave's
+        // core boundary layer on the opposite face
+        const double rv = (f == fxi)   ? bcore_w[wv ^ 1][f ^ 1][lane]
+                          : (f == fyi) ? bcore_w[wv ^ 2][f ^ 1][lane]
+                                       : bring[f][lane];
+        const double cv = bcore[f][lane];
         fbound[f][lane] = neg ? face_flux(rv, cv, v[nd], h[nd]) : face_flux(cv, rv, v[nd], h[nd]);
     }
-    __syncthreads();
+    leaf_sync<W>(); // this wave's step-two boundary fluxes
 
     // ---- step two: core column at z = 0..7 ----------------------------------
     double* out = un + static_cast<size_t>(leaf) * NI;
