@@ -301,7 +301,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(const double* __restrict__ lam_
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1)
     {
-        if (threadIdx.x < s)
+        if (static_cast<int>(threadIdx.x) < s)
             for (int d = 0; d < R; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + s];
         __syncthreads();
     }
@@ -352,7 +352,7 @@ __device__ inline void block_sum3(double* s, double (*red)[256])
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1)
     {
-        if (threadIdx.x < k)
+        if (static_cast<int>(threadIdx.x) < k)
             for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + k];
         __syncthreads();
     }

@@ -824,7 +824,7 @@ __device__ inline double block_max(double v, double* red)
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1)
     {
-        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
     return red[0];
