@@ -153,6 +153,33 @@ __global__ __launch_bounds__(64 * W) void k_step2_3d8_uniform(const double* __re
     // Ring cells, face layout (p, q) = (hi, lo): value r, second layer o,
     // adjacent core cell c, edge cells tq (q = -1 / 8 side) and tp (z = -1 / 8).
     double r[4], o[4], c[4], tq[4], tp[4];
+    // W = 4: each wave has one outer y face and one outer x face (index e = 0
+    // for y, 1 for x: value, second layer, core cell and the two edge cells,
+    // as above) and one inner y and x face (value and core cell only)
+    double re[2], oe[2], ce[2], tqe[2], tpe[2], ri[2], ci[2];
+    if constexpr (W == 4)
+    {
+        const int  zq  = hi * 64 + lo, zy = hi * 64 + lo * 8;
+        const bool yme = fyi == 1; // the outer y face is -y
+        const bool xme = fxi == 3; // the outer x face is -x
+        re[0]  = cell(yme ? s_ym : s_yp, zq + (yme ? 7 * 8 : 0));
+        oe[0]  = cell(yme ? s_ym : s_yp, zq + (yme ? 6 * 8 : 8));
+        ce[0]  = own[zq + (yme ? 0 : 7 * 8)];
+        tqe[0] = cell(yme ? (llo ? s_xmym : s_xpym) : (llo ? s_xmyp : s_xpyp), hi * 64 + (yme ? 7 * 8 : 0) + ecol);
+        tpe[0] = cell(yme ? (hlo ? s_ymzm : s_ymzp) : (hlo ? s_ypzm : s_ypzp), epl + (yme ? 7 * 8 : 0) + lo);
+        ri[0]  = cell(yme ? s_yp : s_ym, zq + (yme ? 0 : 7 * 8));
+        ci[0]  = own[zq + (yme ? 7 * 8 : 0)];
+        const double2 px = *reinterpret_cast<const double2*>(u + static_cast<size_t>(xme ? s_xm : s_xp) * NI + zy +
+                                                             (xme ? 6 : 0));
+        re[1]  = xme ? px.y : px.x;
+        oe[1]  = xme ? px.x : px.y;
+        ce[1]  = own[zy + (xme ? 0 : 7)];
+        tqe[1] = cell(xme ? (llo ? s_xmym : s_xmyp) : (llo ? s_xpym : s_xpyp), hi * 64 + erow + (xme ? 7 : 0));
+        tpe[1] = cell(xme ? (hlo ? s_xmzm : s_xmzp) : (hlo ? s_xpzm : s_xpzp), epl + lo * 8 + (xme ? 7 : 0));
+        ri[1]  = cell(xme ? s_xp : s_xm, zy + (xme ? 0 : 7));
+        ci[1]  = own[zy + (xme ? 7 : 0)];
+    }
+    else
     {
         const int zq = hi * 64 + lo; // (z, _, x) of a y face, without the row
         r[0] = cell(s_ym, zq + 7 * 8);
@@ -161,19 +188,13 @@ __global__ __launch_bounds__(64 * W) void k_step2_3d8_uniform(const double* __re
         c[1] = own[zq + 7 * 8];
         // y faces: second layer; x-side edge in leaf (-1 or +1, -+1, 0) at
         // column 7 or 0, z-side edge in leaf (0, -+1, -1 or +1) at plane 7
-        // or 0 (only the ring passes read them: none inside a block)
-        if (fyi != 0)
-        {
-            o[0]  = cell(s_ym, zq + 6 * 8);
-            tq[0] = cell(llo ? s_xmym : s_xpym, hi * 64 + 7 * 8 + ecol);
-            tp[0] = cell(hlo ? s_ymzm : s_ymzp, epl + 7 * 8 + lo);
-        }
-        if (fyi != 1)
-        {
-            o[1]  = cell(s_yp, zq + 8);
-            tq[1] = cell(llo ? s_xmyp : s_xpyp, hi * 64 + ecol);
-            tp[1] = cell(hlo ? s_ypzm : s_ypzp, epl + lo);
-        }
+        // or 0
+        o[0]  = cell(s_ym, zq + 6 * 8);
+        tq[0] = cell(llo ? s_xmym : s_xpym, hi * 64 + 7 * 8 + ecol);
+        tp[0] = cell(hlo ? s_ymzm : s_ymzp, epl + 7 * 8 + lo);
+        o[1]  = cell(s_yp, zq + 8);
+        tq[1] = cell(llo ? s_xmyp : s_xpyp, hi * 64 + ecol);
+        tp[1] = cell(hlo ? s_ypzm : s_ypzp, epl + lo);
         const int     zy = hi * 64 + lo * 8; // (z, y, _) of an x face
         const double2 pm = *reinterpret_cast<const double2*>(u + static_cast<size_t>(s_xm) * NI + zy + 6);
         const double2 pp = *reinterpret_cast<const double2*>(u + static_cast<size_t>(s_xp) * NI + zy);
@@ -185,16 +206,10 @@ __global__ __launch_bounds__(64 * W) void k_step2_3d8_uniform(const double* __re
         c[3]  = own[zy + 7];
         // x faces: y-side edge in leaf (-+1, -1 or +1, 0) at row 7 or 0,
         // z-side edge in leaf (-+1, 0, -1 or +1) at plane 7 or 0
-        if (fxi != 2)
-        {
-            tq[2] = cell(llo ? s_xmym : s_xmyp, hi * 64 + erow + 7);
-            tp[2] = cell(hlo ? s_xmzm : s_xmzp, epl + lo * 8 + 7);
-        }
-        if (fxi != 3)
-        {
-            tq[3] = cell(llo ? s_xpym : s_xpyp, hi * 64 + erow);
-            tp[3] = cell(hlo ? s_xpzm : s_xpzp, epl + lo * 8);
-        }
+        tq[2] = cell(llo ? s_xmym : s_xmyp, hi * 64 + erow + 7);
+        tp[2] = cell(hlo ? s_xmzm : s_xmzp, epl + lo * 8 + 7);
+        tq[3] = cell(llo ? s_xpym : s_xpyp, hi * 64 + erow);
+        tp[3] = cell(hlo ? s_xpzm : s_xpzp, epl + lo * 8);
     }
     // core column at z = -2..9, core layout (y, x) = (hi, lo)
     double a[12];
@@ -215,38 +230,72 @@ __global__ __launch_bounds__(64 * W) void k_step2_3d8_uniform(const double* __re
     const double h[3] = { prm.hs[0], prm.hs[1], prm.hs[2] };
 
     // ---- step one: ghost ring, face layout ----------------------------------
-#pragma unroll
-    for (int f = 0; f < 4; ++f)
+    if constexpr (W == 4)
     {
-        const bool   neg = (f & 1) == 0;    // ring at -1 (else at 8)
-        const int    nd  = (f < 2) ? 1 : 0; // physical axis of the face normal
-        const int    qd  = 1 - nd;          // physical axis of q
-        if (f == fxi || f == fyi)
-        {
-            // inside the block: only the flux through the core's face
-            fbound[f][lane] = neg ? face_flux(r[f], c[f], v[nd], h[nd]) : face_flux(c[f], r[f], v[nd], h[nd]);
-            continue;
-        }
-        const double uc  = r[f];
-        // cross-lane reads first, with every lane active
-        const double ql = lane_minus1(uc), qh = lane_plus1(uc);
-        const double pl = __shfl_up(uc, 8, 64), ph = __shfl_down(uc, 8, 64);
-        double       lo3[3], hi3[3];
-        lo3[qd] = (lo == 0) ? tq[f] : ql;
-        hi3[qd] = (lo == 7) ? tq[f] : qh;
-        lo3[nd] = neg ? o[f] : c[f];
-        hi3[nd] = neg ? c[f] : o[f];
-        lo3[2]  = (hi == 0) ? tp[f] : pl;
-        hi3[2]  = (hi == 7) ? tp[f] : ph;
-        double fL[3], fR[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < 2; ++e)
         {
-            fL[d] = face_flux(lo3[d], uc, v[d], h[d]);
-            fR[d] = face_flux(uc, hi3[d], v[d], h[d]);
+            const int nd = e == 0 ? 1 : 0; // physical axis of the face normal
+            const int qd = 1 - nd;         // physical axis of q
+            {
+                // inner face: only the flux through the core's face
+                const int  f   = e == 0 ? fyi : fxi;
+                const bool neg = (f & 1) == 0;
+                fbound[f][lane] = face_flux(neg ? ri[e] : ci[e], neg ? ci[e] : ri[e], v[nd], h[nd]);
+            }
+            const int    f   = e == 0 ? (fyi ^ 1) : (fxi ^ 1); // the outer face
+            const bool   neg = (f & 1) == 0;                    // ring at -1 (else at 8)
+            const double uc  = re[e];
+            // cross-lane reads first, with every lane active
+            const double ql = lane_minus1(uc), qh = lane_plus1(uc);
+            const double pl = __shfl_up(uc, 8, 64), ph = __shfl_down(uc, 8, 64);
+            double       lo3[3], hi3[3];
+            lo3[qd] = (lo == 0) ? tqe[e] : ql;
+            hi3[qd] = (lo == 7) ? tqe[e] : qh;
+            lo3[nd] = neg ? oe[e] : ce[e];
+            hi3[nd] = neg ? ce[e] : oe[e];
+            lo3[2]  = (hi == 0) ? tpe[e] : pl;
+            hi3[2]  = (hi == 7) ? tpe[e] : ph;
+            double fL[3], fR[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                fL[d] = face_flux(lo3[d], uc, v[d], h[d]);
+                fR[d] = face_flux(uc, hi3[d], v[d], h[d]);
+            }
+            bring[f][lane]  = update_from_fluxes<3>(uc, fL, fR, prm.dtdx[0]);
+            fbound[f][lane] = neg ? fR[nd] : fL[nd]; // through the core's boundary face
         }
-        bring[f][lane]  = update_from_fluxes<3>(uc, fL, fR, prm.dtdx[0]);
-        fbound[f][lane] = neg ? fR[nd] : fL[nd]; // through the core's boundary face
+    }
+    else
+    {
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+        {
+            const bool   neg = (f & 1) == 0;    // ring at -1 (else at 8)
+            const int    nd  = (f < 2) ? 1 : 0; // physical axis of the face normal
+            const int    qd  = 1 - nd;          // physical axis of q
+            const double uc  = r[f];
+            // cross-lane reads first, with every lane active
+            const double ql = lane_minus1(uc), qh = lane_plus1(uc);
+            const double pl = __shfl_up(uc, 8, 64), ph = __shfl_down(uc, 8, 64);
+            double       lo3[3], hi3[3];
+            lo3[qd] = (lo == 0) ? tq[f] : ql;
+            hi3[qd] = (lo == 7) ? tq[f] : qh;
+            lo3[nd] = neg ? o[f] : c[f];
+            hi3[nd] = neg ? c[f] : o[f];
+            lo3[2]  = (hi == 0) ? tp[f] : pl;
+            hi3[2]  = (hi == 7) ? tp[f] : ph;
+            double fL[3], fR[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                fL[d] = face_flux(lo3[d], uc, v[d], h[d]);
+                fR[d] = face_flux(uc, hi3[d], v[d], h[d]);
+            }
+            bring[f][lane]  = update_from_fluxes<3>(uc, fL, fR, prm.dtdx[0]);
+            fbound[f][lane] = neg ? fR[nd] : fL[nd]; // through the core's boundary face
+        }
     }
     leaf_sync<W>(); // this wave's ring values and fluxes
 
