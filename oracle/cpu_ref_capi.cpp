@@ -216,6 +216,8 @@ eiface* make_euler(int rank, int n, int h, int depth, std::size_t cap, std::arra
     ORACLE_CASE(2, 8, 1)
     ORACLE_CASE(2, 16, 1)
     ORACLE_CASE(2, 64, 1)
+    ORACLE_CASE(2, 10, 2)
+    ORACLE_CASE(2, 50, 2)
     ORACLE_CASE(3, 4, 1)
     ORACLE_CASE(3, 8, 1)
 #undef ORACLE_CASE

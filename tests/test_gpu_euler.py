@@ -28,9 +28,9 @@ def amr():
     return load_package()
 
 
-def pair(amr, rank, n, depth, cap, lengths=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3):
-    s = amr.EulerSolver(rank, depth, n, 1, cap, lengths, gamma, cfl)
-    o = OracleEuler(rank, n, 1, depth, 2 * cap + 64, lengths)
+def pair(amr, rank, n, depth, cap, lengths=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3, h=1):
+    s = amr.EulerSolver(rank, depth, n, h, cap, lengths, gamma, cfl)
+    o = OracleEuler(rank, n, h, depth, 2 * cap + 64, lengths)
     o.set_params(gamma, cfl, threads=THREADS)
     return s, o
 
@@ -84,6 +84,34 @@ def test_euler_uniform(amr, rank, n, levels, steps):
     exp = o.advance_batch(steps)
     assert got == exp, (got, exp)
     assert_fields(s.tree.get_interior(), o.get_interior(), "after steps")
+
+
+@pytest.mark.parametrize("n,steps", [(10, 6), (50, 4)])
+def test_euler_halo2_shapes(amr, n, steps):
+    """The reference test's halo-2 patch shapes (10^2 through the LDS tile,
+    50^2 through strips of 10 rows) on a uniform tree, then a density
+    regrid (same / finer / coarser faces; the criterion over the 2-deep
+    padded patch) and more steps."""
+    say(f"case: Euler h=2 n={n} steps={steps}")
+    s, o = pair(amr, 2, n, 7, 600, h=2)
+    for _ in range(2):
+        s.tree.refine_all()
+        o.refine_all()
+    init_both(s, o, pulse(2, (1.0, 1.0, 1.0), amp=2.0))
+    for rd in range(2):
+        s.advance_batch_async(steps)
+        got = s.finish_advance_batch()
+        exp = o.advance_batch(steps)
+        assert got == exp, (rd, got, exp)
+        assert_fields(s.tree.get_interior(), o.get_interior(), f"round {rd}: steps")
+        crit = amr.make_criterion(amr.CRITERION_MAX_VALUE, 0.62, 0.55, 1, 4)
+        fo = o.criterion(0, 0.62, 0.55, 1, 4)
+        np.testing.assert_array_equal(s.tree.compute_flags(crit), fo)
+        assert s.tree.reconstruct_tree(crit) == o.reconstruct(fo)
+        o.halo_exchange()
+        assert_topology(s, o, f"round {rd}: regrid")
+        assert_fields(s.tree.get_interior(), o.get_interior(), f"round {rd}: transfer")
+    say("leaves:", s.tree.size())
 
 
 def test_euler_remaining_time(amr):
