@@ -3,7 +3,9 @@
 The oracle (cpu_ref.hpp) restates the reference's CPU algorithm for the
 patch-AMR advection step.  Only tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py may import this module; the product path under
-gpu-amr_amd/ never does.
+gpu-amr_amd/ never does.  load_ref() opens the reference's own physics
+functions (_ref/libref_physics.so, compiled from the reference headers where
+they are present), against which the tests check the restatement.
 """
 from __future__ import annotations
 
@@ -334,6 +336,49 @@ def euler_max_speed(rank, U, direction, gamma):
     """getMaxSpeed of one conservative state."""
     U = np.ascontiguousarray(U, dtype=np.float64)
     return float(load().oracle_euler_max_speed(rank, _ptr(U, ctypes.c_double), int(direction), float(gamma)))
+
+
+def advection_flux(ul, ur, v):
+    """rusanovFlux of AdvectionPhysics for velocity component v."""
+    lib = load()
+    lib.oracle_advection_flux.restype = ctypes.c_double
+    lib.oracle_advection_flux.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    return float(lib.oracle_advection_flux(float(ul), float(ur), float(v)))
+
+
+# ---------------------------------------------------------------------------
+# The reference's own physics functions (oracle/_ref/libref_physics.so, built
+# from the reference headers by `make -C oracle ref` where the reference
+# sources are present): the checker of the restatement above.
+# ---------------------------------------------------------------------------
+_REF_PATH = _HERE / "_ref" / "libref_physics.so"
+
+
+def load_ref():
+    """The reference-physics library, or None where it is not built."""
+    if not _REF_PATH.exists():
+        return None
+    lib = ctypes.CDLL(str(_REF_PATH))
+    d = ctypes.POINTER(ctypes.c_double)
+    lib.ref_euler_prim_to_cons.argtypes = [ctypes.c_int, d, d, ctypes.c_double]
+    lib.ref_euler_flux.argtypes = [ctypes.c_int, d, d, d, ctypes.c_int, ctypes.c_double]
+    lib.ref_euler_max_speed.argtypes = [ctypes.c_int, d, ctypes.c_int, ctypes.c_double]
+    lib.ref_euler_max_speed.restype = ctypes.c_double
+    lib.ref_advection_flux.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.ref_advection_flux.restype = ctypes.c_double
+    return lib
+
+
+def build_ref(reference="/root/reference", verbose=False):
+    """Compile oracle/_ref/libref_physics.so from the reference headers (only
+    where the reference sources are present)."""
+    import subprocess
+
+    r = subprocess.run(["make", "-s", "-C", str(_HERE), "ref", f"REFERENCE={reference}"], capture_output=not verbose,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference physics build failed:\n{r.stdout}\n{r.stderr}")
+    return str(_REF_PATH)
 
 
 def build(verbose=False):

@@ -206,6 +206,16 @@ inline void parallel_for(std::size_t n, int threads, Fn&& fn)
 // index fastest (include/containers/static_layout.hpp:28-37); physical x is the
 // last layout index (include/solver/physics_system.hpp:126-135,215-224).
 // ---------------------------------------------------------------------------
+// rusanovFlux of AdvectionPhysics (AdvectionPhysics.hpp:45-66) for velocity
+// component v along the face normal: 0.5 (ul v + ur v) - 0.5 |v| (ur - ul).
+inline double advection_flux(double ul, double ur, double v)
+{
+    const double fl   = ul * v;
+    const double fr   = ur * v;
+    const double smax = std::abs(v);
+    return 0.5 * (fl + fr) - 0.5 * smax * (ur - ul);
+}
+
 template <int R, int N, int H>
 class tree
 {
@@ -1024,13 +1034,7 @@ public:
     // velocity as a run-time parameter.
     // ---------------------------------------------------------------------
     // rusanovFlux (AdvectionPhysics.hpp:45-66)
-    double flux(double ul, double ur, int d) const
-    {
-        const double fl   = ul * m_vel[d];
-        const double fr   = ur * m_vel[d];
-        const double smax = std::abs(m_vel[d]);
-        return 0.5 * (fl + fr) - 0.5 * smax * (ur - ul);
-    }
+    double flux(double ul, double ur, int d) const { return advection_flux(ul, ur, m_vel[d]); }
 
     // compute_time_step_cpu (amr_solver.hpp:355-413)
     double compute_time_step() const

@@ -429,6 +429,8 @@ int oracle_euler_flux(int rank, double const* UL, double const* UR, double* flux
         }
     );
 }
+double oracle_advection_flux(double ul, double ur, double v) { return oracle::advection_flux(ul, ur, v); }
+
 double oracle_euler_max_speed(int rank, double const* U, int dir, double gamma)
 {
     return rank == 2 ? oracle::euler_physics<2>::max_speed(U, dir, gamma) : oracle::euler_physics<3>::max_speed(U, dir, gamma);
