@@ -369,9 +369,21 @@ def load_ref():
     return lib
 
 
+def load_ref_morton():
+    """ref_morton_key(rank, x, y, z, level) from the reference's libmorton,
+    or None where it is not built."""
+    path = _HERE / "_ref" / "libref_morton.so"
+    if not path.exists():
+        return None
+    lib = ctypes.CDLL(str(path))
+    lib.ref_morton_key.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    lib.ref_morton_key.restype = ctypes.c_uint64
+    return lib
+
+
 def build_ref(reference="/root/reference", verbose=False):
-    """Compile oracle/_ref/libref_physics.so from the reference headers (only
-    where the reference sources are present)."""
+    """Compile oracle/_ref/ (the reference's physics and Morton encoding) from
+    the reference headers (only where the reference sources are present)."""
     import subprocess
 
     r = subprocess.run(["make", "-s", "-C", str(_HERE), "ref", f"REFERENCE={reference}"], capture_output=not verbose,

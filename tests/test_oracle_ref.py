@@ -99,3 +99,26 @@ def test_advection_flux_matches_reference(rank):
         for d in range(rank):
             ours = oracle.advection_flux(vals[k], vals[k + 1], vel[d])
             assert same_bits(ours, REF.ref_advection_flux(rank, vals[k], vals[k + 1], d)), (k, d)
+
+
+REFM = oracle.load_ref_morton()
+
+
+@pytest.mark.skipif(REFM is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("rank,depth,levels", [(2, 7, 3), (2, 7, 5), (3, 5, 2), (3, 5, 3)])
+def test_uniform_tree_keys_match_reference_encoding(rank, depth, levels):
+    """Every key of a uniform tree equals morton_id<depth, rank>::encode of
+    its cell: libmorton's encode of the coordinates at the 2^depth
+    resolution, << 6, | level (the reference's dependency, not a retyped
+    interleave)."""
+    o = oracle.OracleTree(rank, 4, 1, depth, (1 << (rank * levels)) * 2 + 64)
+    for _ in range(levels):
+        o.refine_all()
+    step = 1 << (depth - levels)
+    n = 1 << levels
+    exp = []
+    for z in range(n if rank == 3 else 1):
+        for y in range(n):
+            for x in range(n):
+                exp.append(REFM.ref_morton_key(rank, x * step, y * step, z * step, levels))
+    np.testing.assert_array_equal(o.keys(), np.sort(np.array(exp, dtype=np.uint64)))
