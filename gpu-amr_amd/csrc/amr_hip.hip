@@ -962,16 +962,30 @@ struct amr_tape
     // work buffers (max_n leaves each)
     double*                    lam[2]{};
     double*                    work  = nullptr;
-    double*                    vpart = nullptr; // ckpt_every * max_n triples
+    double*                    vpart = nullptr; // per-leaf v-gradient triples of every step
+    std::vector<uint64_t>      vbase;           // first triple of step k in vpart
     double*                    chunk = nullptr;
     double*                    acc   = nullptr;
     uint64_t*                  scratch_keys    = nullptr;
     regrid_summary*            scratch_summary = nullptr;
     std::vector<double*>       seg; // recomputed states of one segment
+    // The launch sequence of a recorded tape is fixed (steps, regrids,
+    // buffers), so the forward replay and the adjoint pass are each captured
+    // once into a hipGraph and replayed.  The velocity is part of the kernels'
+    // arguments: a call with another velocity captures the pass again.
+    struct pass_graph
+    {
+        hipGraphExec_t exec = nullptr;
+        double         v[3]{};
+        double*        result = nullptr; // buffer that holds the pass's output
+    };
+    pass_graph g_forward, g_adjoint;
 
     ~amr_tape()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (pass_graph* g : { &g_forward, &g_adjoint })
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
         std::vector<void*> ptrs = { lam[0], lam[1], work, vpart, chunk, acc, scratch_keys, scratch_summary };
         for (auto& t : topos) ptrs.insert(ptrs.end(), { t.keys, t.links, t.metas });
         for (auto& r : regrids) ptrs.insert(ptrs.end(), { r.mark, r.head, r.counts, r.pos });
@@ -1086,7 +1100,12 @@ struct amr_tape
         // work buffers
         for (int b = 0; b < 2; ++b) lam[b] = dev_alloc<double>(max_n * ni);
         work            = dev_alloc<double>(max_n * ni);
-        vpart           = dev_alloc<double>(ckpt_every * max_n * 3);
+        // Step k's partials sit at vbase[k] whatever the checkpoint interval,
+        // and are summed once after the backward sweep: grad_v is the same
+        // bits for every interval.
+        vbase.assign(steps + 1, 0);
+        for (uint64_t k = 0; k < steps; ++k) vbase[k + 1] = vbase[k] + topo(k).n;
+        vpart           = dev_alloc<double>(vbase[steps] * 3);
         chunk           = dev_alloc<double>(kReduceBlocks * 3);
         acc             = dev_alloc<double>(3);
         scratch_keys    = dev_alloc<uint64_t>(max_n);
@@ -1120,22 +1139,27 @@ struct amr_tape
     {
         AMR_HIP_CHECK(hipDeviceSynchronize()); // inputs written on other streams
         for (int d = 0; d < 3; ++d) vel[d] = d < rank ? v[d] : 0.0;
-        double* a = lam[0];
-        double* b = lam[1];
-        AMR_HIP_CHECK(hipMemcpyAsync(a, u0, field_bytes(topos[0].n), hipMemcpyDeviceToDevice, stream));
-        for (uint64_t k = 0; k < steps; ++k)
-        {
-            if (auto const* r = regrid_before(k))
+        AMR_HIP_CHECK(hipMemcpyAsync(lam[0], u0, field_bytes(topos[0].n), hipMemcpyDeviceToDevice, stream));
+        run_pass(g_forward, [this]() {
+            double* a = lam[0];
+            double* b = lam[1];
+            for (uint64_t k = 0; k < steps; ++k)
             {
-                replay_transfer(*r, a, b);
+                if (auto const* r = regrid_before(k))
+                {
+                    replay_transfer(*r, a, b);
+                    std::swap(a, b);
+                }
+                if (k % ckpt_every == 0)
+                    AMR_HIP_CHECK(hipMemcpyAsync(ckpts[k / ckpt_every], a, field_bytes(topo(k).n),
+                                                 hipMemcpyDeviceToDevice, stream));
+                forward_step(k, a, b);
                 std::swap(a, b);
             }
-            if (k % ckpt_every == 0)
-                AMR_HIP_CHECK(hipMemcpyAsync(ckpts[k / ckpt_every], a, field_bytes(topo(k).n), hipMemcpyDeviceToDevice, stream));
-            forward_step(k, a, b);
-            std::swap(a, b);
-        }
-        AMR_HIP_CHECK(hipMemcpyAsync(uT, a, field_bytes(topo(steps - 1).n), hipMemcpyDeviceToDevice, stream));
+            return a;
+        });
+        AMR_HIP_CHECK(hipMemcpyAsync(uT, g_forward.result, field_bytes(topo(steps - 1).n), hipMemcpyDeviceToDevice,
+                                     stream));
         AMR_HIP_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -1144,10 +1168,21 @@ struct amr_tape
     void adjoint(const double* lamT, double* grad_u0, double* grad_v)
     {
         AMR_HIP_CHECK(hipDeviceSynchronize());
-        double* l  = lam[0];
-        double* lo = lam[1];
-        AMR_HIP_CHECK(hipMemcpyAsync(l, lamT, field_bytes(topo(steps - 1).n), hipMemcpyDeviceToDevice, stream));
-        AMR_HIP_CHECK(hipMemsetAsync(acc, 0, 3 * sizeof(double), stream));
+        AMR_HIP_CHECK(hipMemcpyAsync(lam[0], lamT, field_bytes(topo(steps - 1).n), hipMemcpyDeviceToDevice, stream));
+        run_pass(g_adjoint, [this]() { return adjoint_sweep(); });
+        AMR_HIP_CHECK(hipMemcpyAsync(grad_u0, g_adjoint.result, field_bytes(topos[0].n), hipMemcpyDeviceToDevice, stream));
+        double h[3];
+        AMR_HIP_CHECK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, stream));
+        AMR_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int d = 0; d < 3; ++d) grad_v[d] = d < rank ? h[d] : 0.0;
+    }
+
+    // The backward sweep from lambda_T in lam[0]: grad_u0 in the returned
+    // buffer, grad_v in acc.
+    double* adjoint_sweep()
+    {
+        double*                    l  = lam[0];
+        double*                    lo = lam[1];
         std::vector<const double*> states(ckpt_every);
         const uint64_t             nseg = (steps + ckpt_every - 1) / ckpt_every;
         for (uint64_t s = nseg; s-- > 0;)
@@ -1166,13 +1201,11 @@ struct amr_tape
                     forward_step(k, states[k - c], next);
                 states[k + 1 - c] = next;
             }
-            uint64_t off = 0;
             for (uint64_t k = e; k-- > c;)
             {
                 topology const& t = topo(k);
-                op->adj_step(l, states[k - c], t.keys, t.links, t.metas, adjoint_consts(dts[k]), lo, vpart + 3 * off,
-                             static_cast<uint32_t>(t.n), stream);
-                off += t.n;
+                op->adj_step(l, states[k - c], t.keys, t.links, t.metas, adjoint_consts(dts[k]), lo,
+                             vpart + 3 * vbase[k], static_cast<uint32_t>(t.n), stream);
                 std::swap(l, lo);
                 if (auto const* r = regrid_before(k))
                 {
@@ -1180,16 +1213,48 @@ struct amr_tape
                     std::swap(l, lo);
                 }
             }
-            hipLaunchKernelGGL(k_sum3_chunks, dim3(kReduceBlocks), dim3(256), 0, stream, vpart, off, chunk);
-            check_launch("k_sum3_chunks");
-            hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(256), 0, stream, chunk, acc);
-            check_launch("k_sum3_final");
         }
-        AMR_HIP_CHECK(hipMemcpyAsync(grad_u0, l, field_bytes(topos[0].n), hipMemcpyDeviceToDevice, stream));
-        double h[3];
-        AMR_HIP_CHECK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, stream));
-        AMR_HIP_CHECK(hipStreamSynchronize(stream));
-        for (int d = 0; d < 3; ++d) grad_v[d] = d < rank ? h[d] : 0.0;
+        hipLaunchKernelGGL(k_sum3_chunks, dim3(kReduceBlocks), dim3(256), 0, stream, vpart, vbase[steps], chunk);
+        check_launch("k_sum3_chunks");
+        hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(256), 0, stream, chunk, acc);
+        check_launch("k_sum3_final");
+        return l;
+    }
+
+    // Replays g, or captures body() into it first (no graph yet, or one made
+    // for another velocity).  body returns the buffer that holds the output.
+    template <class F>
+    void run_pass(pass_graph& g, F&& body)
+    {
+        if (g.exec && std::memcmp(g.v, vel, sizeof(g.v)) == 0)
+        {
+            AMR_HIP_CHECK(hipGraphLaunch(g.exec, stream));
+            return;
+        }
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec           = nullptr;
+        hipGraph_t graph = nullptr;
+        AMR_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try
+        {
+            g.result = body();
+        }
+        catch (...)
+        {
+            (void)hipStreamEndCapture(stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        AMR_HIP_CHECK(hipStreamEndCapture(stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess)
+        {
+            g.exec = nullptr;
+            throw amr_error(AMR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        std::memcpy(g.v, vel, sizeof(g.v));
+        AMR_HIP_CHECK(hipGraphLaunch(g.exec, stream));
     }
 };
 

@@ -264,7 +264,9 @@ __global__ __launch_bounds__(NT) void k_adj_step(const double* __restrict__ lam_
     __shared__ double tile[tile_shape<R, N>::TS];
     __shared__ double lt[S::NI];
     __shared__ double h[S::NDIR * S::FACE];
-    __shared__ double red[R][NT];
+    constexpr int     NW = NT / 64;
+    static_assert(NT % 64 == 0, "whole waves");
+    __shared__ double red[R][NW];
     const uint32_t leaf = blockIdx.x;
     if (leaf >= nleaves) return;
     const face_link* L  = links + static_cast<size_t>(leaf) * S::NDIR;
@@ -296,17 +298,24 @@ __global__ __launch_bounds__(NT) void k_adj_step(const double* __restrict__ lam_
         out[i] = adj_cell<R, N>(lt, h, M, i, dtdx, vp, vm);
         adj_velocity_cell<R, N>(tile, i, lt[i], dtdx, sgn, acc);
     }
-    // fixed-order block reduction of the velocity partials
-    for (int d = 0; d < R; ++d) red[d][threadIdx.x] = acc[d];
+    // fixed-order reduction of the velocity partials: a shuffle tree inside
+    // each wave, then the waves' sums in wave order
+    for (int d = 0; d < R; ++d)
+        for (int o = 32; o > 0; o >>= 1) acc[d] += __shfl_down(acc[d], o, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < R; ++d) red[d][threadIdx.x >> 6] = acc[d];
     __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1)
-    {
-        if (static_cast<int>(threadIdx.x) < s)
-            for (int d = 0; d < R; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + s];
-        __syncthreads();
-    }
     if (threadIdx.x == 0)
-        for (int d = 0; d < 3; ++d) vpart[static_cast<size_t>(leaf) * 3 + d] = d < R ? red[d][0] : 0.0;
+        for (int d = 0; d < 3; ++d)
+        {
+            double sum = 0.0;
+            if (d < R)
+            {
+                sum = red[d][0];
+                for (int w = 1; w < NW; ++w) sum += red[d][w];
+            }
+            vpart[static_cast<size_t>(leaf) * 3 + d] = sum;
+        }
 }
 
 // Transposed regrid transfer, one workgroup per leaf before the regrid
@@ -381,7 +390,7 @@ __global__ __launch_bounds__(256) void k_sum3_final(const double* __restrict__ c
         for (int d = 0; d < 3; ++d) s[d] = chunk_sums[threadIdx.x * 3 + d];
     block_sum3(s, red);
     if (threadIdx.x == 0)
-        for (int d = 0; d < 3; ++d) acc[d] += red[d][0];
+        for (int d = 0; d < 3; ++d) acc[d] = red[d][0];
 }
 
 } // namespace amrhip
