@@ -275,7 +275,10 @@ __global__ __launch_bounds__(NT) void k_adj_step(const double* __restrict__ lam_
     stage_tile<R, N, NT>(tile, u, leaf, links, metas);
     const double* lp = lam_next + static_cast<size_t>(leaf) * S::NI;
     for (int i = threadIdx.x; i < S::NI; i += NT) lt[i] = lp[i];
-    for (int g = threadIdx.x; g < S::NDIR * S::FACE; g += NT)
+    // The face terms start half a workgroup away from the ghost gather of
+    // stage_tile (which starts at thread 0), so that their two dependent
+    // load chains run on different waves.
+    for (int g = (threadIdx.x + NT / 2) % NT; g < S::NDIR * S::FACE; g += NT)
     {
         const int dir = g / S::FACE;
         int       c[R];
