@@ -298,7 +298,10 @@ int amr_tape_get_keys(const amr_tape* tape, uint64_t index, uint64_t* keys, uint
  * and velocity v, over the recorded topologies and dts (dt does not follow
  * v).  Device pointers; the call waits for all prior device work before
  * reading and for its own work before returning.  Refreshes the checkpoints:
- * the next amr_tape_adjoint differentiates this pass. */
+ * the next amr_tape_adjoint differentiates this pass.  The pass runs as a
+ * hipGraph captured on the first call and on each call with a velocity other
+ * than the previous call's (amr_tape_adjoint likewise); results are the same
+ * bits either way. */
 int amr_tape_forward(amr_tape* tape, const double* u0_device, const double velocity[3], double* uT_device);
 /* grad_u0 = (du(T)/du0)^T lambda_T and grad_velocity = (du(T)/dv)^T lambda_T
  * of the most recent forward pass (the recording or amr_tape_forward), with
