@@ -23,7 +23,9 @@ import json
 import os
 import pathlib
 import re
+import shutil
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -80,6 +82,16 @@ def traffic_from_profiles(kernel):
     return float(m.group(1)) if m else None
 
 
+def trace_from_profiles(kernel):
+    """(average launch us, file) of `kernel` from the committed kernel trace
+    of this benchmark's default command (rocprofv3 --kernel-trace --stats)."""
+    p = ROOT / "profiles" / f"{kernel}_c3_bench_kernel_trace.md"
+    if not p.exists():
+        return None, None
+    m = re.search(r"avg_launch_us:\s*([0-9.eE+]+)", p.read_text())
+    return (float(m.group(1)) if m else None), f"profiles/{p.name}"
+
+
 def cpu_baseline(steps, threads):
     """The CPU oracle (the reference algorithm restated, padded patches,
     per-step halo exchange) on the same C3 workload, `steps` steps."""
@@ -101,6 +113,24 @@ def cpu_baseline(steps, threads):
     o.advance_batch(steps)
     dt = time.perf_counter() - t0
     return steps * o.size() * 512 / dt, dt
+
+
+def host_cpu():
+    """This host's CPU model and logical CPU count, and the threads the CPU
+    baselines use: the job's CPU allotment (OMP_NUM_THREADS, else 16), at
+    most the logical CPUs present."""
+    model = "unknown"
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    allot = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return {"model": model, "nproc": nproc, "threads": max(1, min(allot, nproc)),
+            "threads_rule": "min(OMP_NUM_THREADS or 16, nproc): the job's CPU allotment"}
 
 
 def calib_lib():
@@ -224,7 +254,9 @@ def adjoint_section(amr, steps=300, checkpoint_every=20, reps=3):
         tape.adjoint(lam.data_ptr(), g.data_ptr())
         adj.append(time.perf_counter() - t0)
     f, a = min(fwd), min(adj)
-    log(f"adjoint: {info['steps']} steps, leaves {info['initial_leaves']}..{info['max_leaves']}, "
+    topo_leaves = [len(tape.keys(k)) for k in range(info["topologies"])]
+    log(f"adjoint: {info['steps']} steps, {info['topologies']} topologies, leaves initial {info['initial_leaves']}, "
+        f"min {min(topo_leaves)}, max {max(topo_leaves)}, final {info['final_leaves']}; "
         f"forward {f * 1e3:.1f} ms, adjoint {a * 1e3:.1f} ms, ratio {a / f:.2f}")
     tape.close()
     tree.close()
@@ -232,8 +264,12 @@ def adjoint_section(amr, steps=300, checkpoint_every=20, reps=3):
         "workload": "C5: 2D active AMR, 16^2 FP64 patches, levels 3..6, gradient criterion every 10 steps, 300 steps",
         "steps": info["steps"],
         "checkpoint_every": info["checkpoint_every"],
+        "topologies": info["topologies"],
+        "regrids": info["regrids"],
         "leaves_initial": info["initial_leaves"],
-        "leaves_max": info["max_leaves"],
+        "leaves_min": min(topo_leaves),
+        "leaves_max": max(topo_leaves),
+        "leaves_final": info["final_leaves"],
         "record_ms": rec * 1e3,
         "forward_ms": f * 1e3,
         "adjoint_ms": a * 1e3,
@@ -416,6 +452,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--euler-only", nargs="?", const="all", choices=["all", "3d", "2d", "2d_active"],
                     help="only the Euler benchmarks (profiling): all, or one of them")
+    ap.add_argument("--no-native-oracle", action="store_true",
+                    help="CPU baselines with the shipped oracle build instead of one compiled for this CPU")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the leaves would shard by contiguous "
@@ -424,6 +462,20 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 5)
         return
+
+    # the CPU baselines' oracle, compiled for this host's CPU while the GPU
+    # sections run (started before anything touches the GPU)
+    cpu_info = host_cpu()
+    native = None
+    want_cpu = not (args.no_cpu and args.no_c2 and args.no_euler)
+    if want_cpu and not args.euler_only and not args.no_native_oracle and shutil.which("g++"):
+        if str(ROOT) not in sys.path:
+            sys.path.insert(0, str(ROOT))
+        import oracle
+
+        native = oracle.build_native(tempfile.mkdtemp(prefix="oracle_native_"))
+
+    import torch
 
     amr = package()
     from gpu_amr_amd import workloads
@@ -474,25 +526,47 @@ def main():
     copy_gbs = 2 * 8 * cells / (copy_ms * 1e-3) / 1e9
     log(f"plain copy of {8 * cells / 1e6:.0f} MB: {copy_ms * 1e3:.1f} us, {copy_gbs:.0f} GB/s")
 
+    # timed: exactly args.steps steps.  Fused, the solver runs them as pairs
+    # (one launch per two steps); an odd count ends with one single-step
+    # launch.  The HIP events bracket the pairs only, so the launch time is
+    # that of the fused kernel; the wall-clock covers every step.
+    per_launch = 2 if fused and args.steps >= 2 else 1
+    head = args.steps - (args.steps % per_launch)
     g.synchronize()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev.record(e0, stream)
-    s.advance_batch_async(args.steps)
+    s.advance_batch_async(head)
     ev.record(e1, stream)
     _, executed = s.finish_advance_batch()
+    if head < args.steps:
+        s.advance_batch_async(args.steps - head)
+        executed += s.finish_advance_batch()[1]
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     assert executed == args.steps
-    # launches: pairs of steps when fused (a trailing odd step runs alone)
-    per_launch = 2 if fused else 1
-    launches = args.steps // per_launch
-    assert not fused or args.steps % 2 == 0, "time an even step count when fusing"
-    launch_ms = ev.elapsed_ms(e0, e1) / launches
-    step_ms = launch_ms / per_launch
+    launch_ms = ev.elapsed_ms(e0, e1) / (head // per_launch)
     value = args.steps * cells / wall
     achieved = BYTES_PER_CELL_UPDATE * cells * per_launch / (launch_ms * 1e-3) / 1e9
     log(f"gpu: {wall * 1e3 / args.steps:.3f} ms/step wall, {launch_ms * 1e3:.1f} us/launch "
-        f"({per_launch} step(s) per launch), {achieved:.0f} GB/s algorithmic")
+        f"({per_launch} step(s) per launch, {head // per_launch} launches timed), {achieved:.0f} GB/s algorithmic")
     g.close()
+    fused = per_launch == 2
+
+    if native is not None:
+        # every CPU baseline below uses the oracle built for this CPU when
+        # its compile succeeded
+        proc, path = native
+        rc = proc.wait(timeout=600)
+        import oracle
+
+        if rc == 0:
+            oracle.load(path)
+            cpu_info["oracle_build"] = "g++ -O3 -march=native -ffp-contract=off (compiled on this host)"
+        else:
+            log(f"native oracle build failed ({proc.stderr.read().decode()[-300:]}); shipped build used")
+    cpu_info.setdefault("oracle_build", "g++ -O3 -march=x86-64-v3 -ffp-contract=off (shipped build)")
+    threads = cpu_info["threads"]
 
     batched = None
     if not args.no_batched:
@@ -504,16 +578,16 @@ def main():
 
     c2 = None
     if not args.no_c2:
-        c2 = c2_section(amr)
+        c2 = c2_section(amr, threads=threads)
 
     euler = None
     if not args.no_euler:
-        euler = {"3d": euler_section(amr, "3d"), "2d": euler_section(amr, "2d", cpu_cycles=2),
-                 "2d_active": euler_section(amr, "2d_active", cpu_cycles=4)}
+        euler = {"3d": euler_section(amr, "3d", threads=threads),
+                 "2d": euler_section(amr, "2d", cpu_cycles=2, threads=threads),
+                 "2d_active": euler_section(amr, "2d_active", cpu_cycles=4, threads=threads)}
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         rate, secs = cpu_baseline(args.cpu_steps, threads)
         cpu = {
             "value": rate,
@@ -521,8 +595,10 @@ def main():
             "cores": threads,
             "kind": "port",
             "sample": f"{args.cpu_steps} steps of the same 32768-leaf 8^3 workload (step + halo exchange, padded patches), {secs:.1f} s",
+            "host": cpu_info,
         }
-        log(f"cpu: {rate:.3e} cell-updates/s on {threads} threads")
+        log(f"cpu: {rate:.3e} cell-updates/s on {threads} threads of {cpu_info['nproc']} "
+            f"({cpu_info['model']}; {cpu_info['oracle_build']})")
 
     if fused:
         kernel = "k_step2_3d8_uniform"
@@ -531,6 +607,9 @@ def main():
     else:
         kernel = "k_step_3d8"
         kernel_desc = f"single-step kernel variant '{names[best]}' ({how})"
+    traffic = traffic_from_profiles(kernel)
+    prof_us, prof_file = trace_from_profiles(kernel)
+    algo_bytes = BYTES_PER_CELL_UPDATE * cells * per_launch
     out = {
         "metric": "FP64 cell-updates/s (3D uniform octree, % of HBM roofline)",
         "value": value,
@@ -556,11 +635,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(kernel),
+            "traffic": traffic,
             "kernel": kernel_desc,
             "launch_us": launch_ms * 1e3,
             "steps_per_launch": per_launch,
-            "algorithmic_bytes_per_launch": BYTES_PER_CELL_UPDATE * cells * per_launch,
+            "algorithmic_bytes_per_launch": algo_bytes,
+            # HBM bytes the counters see per launch over this run's launch time
+            "dram_GBps": traffic / (launch_ms * 1e-3) / 1e9 if traffic else None,
+            "dram_frac": traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+            # the same kernel in the committed trace of this command
+            "profile": {"file": prof_file, "launch_us": prof_us,
+                        "frac": algo_bytes / (prof_us * 1e-6) / 1e9 / HBM_PEAK_GBS if prof_us else None},
             "single_step_variants_us": per_variant,
             "copy_GBps": copy_gbs,
             "frac_of_copy": achieved / copy_gbs,

@@ -75,11 +75,17 @@ typedef struct amr_halo_direction_metadata
 
 /* Refinement criterion evaluated on the device (the fill_refine_flags form,
  * include/ndtree/ndtree.hpp:889-894).  kind 0: maximum cell value over the
- * interior and face halos, starting from 0 (examples/fvm_solver_advection.e.cpp:75-101,
- * src/cuda/fvm_refinement_criterion.cu:27-67).  kind 1: maximum undivided
- * gradient max(|u[c+s]-u[c]|, |u[c]-u[c-s]|) over interior cells and dims.
- * Refine if value > refine_threshold and level < max_level, else Coarsen if
- * value < coarsen_threshold and level > min_level, else Stable. */
+ * interior and face halos, starting from 0, as the reference's CPU criterion
+ * functors compute it (examples/fvm_solver_advection.e.cpp:75-90,
+ * tests/advection_equation_amr.t.cpp:155-170).  The reference's CUDA kernel
+ * (src/cuda/fvm_refinement_criterion.cu:27-67) starts from -DBL_MAX instead
+ * and also scans the corner cells, which no halo exchange writes; its flags
+ * differ from these only on patches whose values are all negative, with a
+ * threshold between their maximum and 0.  kind 1: maximum undivided
+ * gradient max(|u[c+s]-u[c]|, |u[c]-u[c-s]|) over interior cells and dims
+ * (no reference counterpart).  Refine if value > refine_threshold and
+ * level < max_level, else Coarsen if value < coarsen_threshold and
+ * level > min_level, else Stable (fvm_refinement_criterion.cu:53-66). */
 typedef struct amr_criterion
 {
     int32_t kind;

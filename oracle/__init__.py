@@ -33,14 +33,16 @@ assert HALO_META_DTYPE.itemsize == 36
 _lib = None
 
 
-def load():
-    """Load (and if needed build) liboracle.so."""
+def load(path=None):
+    """Load liboracle.so (or the build at `path`, e.g. build_native()'s; the
+    first call decides which build every OracleTree / OracleEuler uses)."""
     global _lib
     if _lib is not None:
         return _lib
-    if not _LIB_PATH.exists():
-        raise RuntimeError(f"{_LIB_PATH} is missing: run `make -C oracle` (or __graft_entry__.build())")
-    lib = ctypes.CDLL(str(_LIB_PATH))
+    path = pathlib.Path(path) if path else _LIB_PATH
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: run `make -C oracle` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(str(path))
     vp, sz, dp, i8p, u64p = (
         ctypes.c_void_p,
         ctypes.c_size_t,
@@ -391,6 +393,18 @@ def build_ref(reference="/root/reference", verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"reference physics build failed:\n{r.stdout}\n{r.stderr}")
     return str(_REF_PATH)
+
+
+def build_native(out_dir):
+    """Start compiling the oracle for this host's CPU (-march=native, same
+    -ffp-contract=off arithmetic) into out_dir/liboracle_native.so; returns
+    (Popen, path).  The CPU baseline uses it where the compile succeeds."""
+    import subprocess
+
+    out = pathlib.Path(out_dir) / "liboracle_native.so"
+    cmd = ["g++", "-std=c++20", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+           "-o", str(out), str(_HERE / "cpu_ref_capi.cpp"), "-lpthread"]
+    return subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE), out
 
 
 def build(verbose=False):

@@ -3,8 +3,9 @@
 // morton_id.hpp:131-139 and 349-358: libmorton's 64-bit encode of the
 // coordinates at the 2^Depth resolution, shifted left by 6, OR the level),
 // from the reference's vendored libmorton headers where they lie
-// (lib/libmorton; built by `make -C oracle ref` into oracle/_ref/, never
-// shipped).  morton_id.hpp itself needs C++23 library functions this image
+// (lib/libmorton; built by `make -C oracle ref` into oracle/_ref/, which is
+// not committed and is opened only by the CPU tests of
+// tests/test_oracle_ref.py).  morton_id.hpp itself needs C++23 library functions this image
 // lacks, so the key is assembled here from libmorton exactly as it writes it.
 #include <libmorton/morton.h>
 

@@ -1,8 +1,10 @@
 // ref_physics.cpp -- test infrastructure: the reference's own physics
 // functions, compiled from its headers where they lie (include/solver/
 // EulerPhysics.hpp, include/solver/AdvectionPhysics.hpp; built by
-// `make -C oracle ref` into oracle/_ref/, never shipped), behind a C face so
-// that the tests can compare the oracle's restatement with them bit for bit.
+// `make -C oracle ref` into oracle/_ref/, which is not committed; the tree
+// copied to a GPU host carries it, but only the CPU tests of
+// tests/test_oracle_ref.py open it), behind a C face so that the tests can
+// compare the oracle's restatement with them bit for bit.
 // Both headers compile as they are with g++ -std=c++20; no stand-in is
 // supplied for anything.
 #include "solver/AdvectionPhysics.hpp"

@@ -342,3 +342,61 @@ def dense_upwind_step(u, vel, dt, dx):
         dtdx = dt / dx[d]
         total -= dtdx * (fR - fL)
     return u + total
+
+
+# ---------------------------------------------------------------------------
+# Refinement criteria, restated from their definitions on the padded patches
+# that expected_halo produces (face halos filled geometrically, corners NaN).
+# ---------------------------------------------------------------------------
+def criterion_values(padded, rank, n, h, kind, start=0.0):
+    """Per-leaf criterion value.
+
+    kind 0 (maximum value): max over the interior and face-halo cells, and
+    `start` -- the reference's CPU criteria start from 0
+    (examples/fvm_solver_advection.e.cpp:79, tests/advection_equation_amr.t.cpp:159);
+    corner cells are never written by a halo exchange and are not read.
+    kind 1 (undivided gradient): max over interior cells c and layout dims d
+    of max(|u[c+e_d] - u[c]|, |u[c] - u[c-e_d]|), and `start`.
+    """
+    pad = np.asarray(padded, dtype=np.float64)
+    nl = pad.shape[0]
+    out = np.full(nl, start, dtype=np.float64)
+    inner = slice(h, h + n)
+    if kind == 0:
+        for i in range(nl):
+            p = pad[i]
+            vals = [p[(inner,) * rank].ravel()]
+            for d in range(rank):
+                for band in (slice(0, h), slice(h + n, n + 2 * h)):
+                    sl = tuple(band if e == d else inner for e in range(rank))
+                    vals.append(p[sl].ravel())
+            out[i] = max(start, float(np.max(np.concatenate(vals))))
+        return out
+    for i in range(nl):
+        p = pad[i]
+        u = p[(inner,) * rank]
+        g = start
+        for d in range(rank):
+            up = p[tuple(slice(h + 1, h + n + 1) if e == d else inner for e in range(rank))]
+            um = p[tuple(slice(h - 1, h + n - 1) if e == d else inner for e in range(rank))]
+            g = max(g, float(np.max(np.abs(up - u))), float(np.max(np.abs(u - um))))
+        out[i] = g
+    return out
+
+
+def decide(values, keys, refine_thr, coarsen_thr, min_level, max_level):
+    """Refine if value > refine_thr and level < max_level, else Coarsen if
+    value < coarsen_thr and level > min_level, else Stable
+    (examples/fvm_solver_advection.e.cpp:84-89, src/cuda/fvm_refinement_criterion.cu:53-66)."""
+    levels = np.asarray(keys, dtype=np.uint64) & np.uint64(63)
+    flags = np.zeros(len(values), dtype=np.int8)
+    for i, (v, lv) in enumerate(zip(values, levels)):
+        if v > refine_thr and int(lv) < max_level:
+            flags[i] = 1
+        elif v < coarsen_thr and int(lv) > min_level:
+            flags[i] = 2
+    return flags
+
+
+def expected_flags(keys, padded, rank, n, h, kind, refine_thr, coarsen_thr, min_level, max_level):
+    return decide(criterion_values(padded, rank, n, h, kind), keys, refine_thr, coarsen_thr, min_level, max_level)

@@ -5,7 +5,10 @@ sources are present): primitiveToConservative, rusanovFlux and getMaxSpeed
 for 2D and 3D Euler states, and the advection rusanovFlux, bit for bit on
 seeded random states and edge cases.  This pins the rounding order of the
 Euler flux (the GPU kernels are pinned against the oracle by the GPU tests).
-Skipped where the reference library is not built."""
+Skipped where the reference library is not built.  The libraries are opened
+by fixtures when a test of this module runs, not when the module is
+collected, so a run that selects none of these tests (the GPU tests) never
+maps them."""
 import ctypes
 
 import numpy as np
@@ -13,8 +16,32 @@ import pytest
 
 import oracle
 
-REF = oracle.load_ref()
-pytestmark = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference sources absent)")
+REF = None
+REFM = None
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _reference_libraries():
+    """Opens oracle/_ref's libraries for the tests of this module."""
+    global REF, REFM
+    REF = oracle.load_ref()
+    REFM = oracle.load_ref_morton()
+    yield
+    REF = REFM = None
+
+
+@pytest.fixture
+def ref():
+    if REF is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    return REF
+
+
+@pytest.fixture
+def refm():
+    if REFM is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    return REFM
 
 
 def _p(a):
@@ -59,7 +86,7 @@ def same_bits(a, b):
 
 @pytest.mark.parametrize("rank", [2, 3])
 @pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0])
-def test_prim_to_cons_matches_reference(rank, gamma):
+def test_prim_to_cons_matches_reference(ref, rank, gamma):
     rng = np.random.default_rng(100 + rank)
     prims = random_prims(rng, rank, 400)
     ours = oracle.euler_prim_to_cons(rank, prims, gamma)
@@ -69,7 +96,7 @@ def test_prim_to_cons_matches_reference(rank, gamma):
 
 @pytest.mark.parametrize("rank", [2, 3])
 @pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0])
-def test_rusanov_flux_and_max_speed_match_reference(rank, gamma):
+def test_rusanov_flux_and_max_speed_match_reference(ref, rank, gamma):
     rng = np.random.default_rng(200 + rank)
     cons = oracle.euler_prim_to_cons(rank, random_prims(rng, rank, 600), gamma)
     for k in range(0, len(cons), 2):
@@ -89,7 +116,7 @@ def test_rusanov_flux_and_max_speed_match_reference(rank, gamma):
 
 
 @pytest.mark.parametrize("rank", [2, 3])
-def test_advection_flux_matches_reference(rank):
+def test_advection_flux_matches_reference(ref, rank):
     """The reference's AdvectionPhysics has the fixed velocity (1, 0.5, 0);
     the oracle's flux takes the velocity component as an argument."""
     rng = np.random.default_rng(300 + rank)
@@ -101,12 +128,8 @@ def test_advection_flux_matches_reference(rank):
             assert same_bits(ours, REF.ref_advection_flux(rank, vals[k], vals[k + 1], d)), (k, d)
 
 
-REFM = oracle.load_ref_morton()
-
-
-@pytest.mark.skipif(REFM is None, reason="oracle/_ref not built (reference sources absent)")
 @pytest.mark.parametrize("rank,depth,levels", [(2, 7, 3), (2, 7, 5), (3, 5, 2), (3, 5, 3)])
-def test_uniform_tree_keys_match_reference_encoding(rank, depth, levels):
+def test_uniform_tree_keys_match_reference_encoding(refm, rank, depth, levels):
     """Every key of a uniform tree equals morton_id<depth, rank>::encode of
     its cell: libmorton's encode of the coordinates at the 2^depth
     resolution, << 6, | level (the reference's dependency, not a retyped
