@@ -72,6 +72,7 @@ _SIGS = {
     "amr_tree_reconstruct_flags": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)],
     "amr_tree_reconstruct_criterion": [ctypes.c_void_p, ctypes.POINTER(Criterion), ctypes.POINTER(ctypes.c_int)],
     "amr_tree_compute_flags": [ctypes.c_void_p, ctypes.POINTER(Criterion), ctypes.c_void_p, ctypes.c_uint64],
+    "amr_tree_compute_flags_noalloc": [ctypes.c_void_p, ctypes.POINTER(Criterion), ctypes.c_void_p, ctypes.c_uint64],
     "amr_tree_set_interior": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
     "amr_tree_get_interior": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
     "amr_tree_get_patches": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
@@ -250,6 +251,13 @@ class NdTree:
     def compute_flags(self, crit: Criterion):
         out = np.zeros(self.size(), dtype=np.int8)
         _check(self.lib.amr_tree_compute_flags(self.h, ctypes.byref(crit), _vp(out), out.size))
+        return out
+
+    def compute_flags_noalloc(self, crit: Criterion):
+        """compute_flags staged in the library's fixed device array (no
+        device allocation; trees of up to 2**20 leaves)."""
+        out = np.zeros(self.size(), dtype=np.int8)
+        _check(self.lib.amr_tree_compute_flags_noalloc(self.h, ctypes.byref(crit), _vp(out), out.size))
         return out
 
     # -- fields -----------------------------------------------------------

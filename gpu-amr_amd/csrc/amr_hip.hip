@@ -19,6 +19,7 @@
 #include <fstream>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -504,6 +505,7 @@ struct amr_ctx
     amr_tree_config cfg{};
     ops const*      op = nullptr;
     hipStream_t     stream{};
+    int             device = 0; // the GPU current at creation (owns the stream and buffers)
     int             rank = 0, depth = 0, ni = 0, flat = 0, ndir = 0;
     uint64_t        cap = 0;
 
@@ -905,17 +907,54 @@ struct amr_batch
     }
 };
 
-// Refine flags of the criterion on the current state, into ctx->flags.
-static void run_criterion(amr_ctx* ctx, amr_criterion const* crit)
+// Refine flags of the criterion on the current state, into `out` (one per
+// leaf; ctx->flags unless given).
+static void run_criterion(amr_ctx* ctx, amr_criterion const* crit, int8_t* out = nullptr)
 {
     ctx->ensure_usable();
     if (!crit) throw amr_error(AMR_ERR_ARGUMENT, "null criterion");
     if (crit->kind != AMR_CRITERION_MAX_VALUE && crit->kind != AMR_CRITERION_GRADIENT)
         throw amr_error(AMR_ERR_ARGUMENT, "unknown criterion kind");
     ctx->op->criterion(crit->kind, ctx->u[ctx->cur], ctx->keys[ctx->kc], ctx->links, ctx->metas,
-                       crit->refine_threshold, crit->coarsen_threshold, crit->min_level, crit->max_level, ctx->flags,
-                       static_cast<uint32_t>(ctx->n), ctx->stream);
+                       crit->refine_threshold, crit->coarsen_threshold, crit->min_level, crit->max_level,
+                       out ? out : ctx->flags, static_cast<uint32_t>(ctx->n), ctx->stream);
 }
+
+// Staging of amr_tree_compute_flags_noalloc: one flag per leaf, for trees of
+// up to kNoAllocMaxLeaves leaves (1 MiB).  Every GPU the library runs on
+// holds its own copy until the process ends.  Host code uses the address
+// hipGetSymbolAddress gives on each device (noalloc_staging), never the
+// array's name, which on the host is only a stand-in for the symbol.
+constexpr uint64_t kNoAllocMaxLeaves = uint64_t{ 1 } << 20;
+static __device__ __align__(256) int8_t scratch[kNoAllocMaxLeaves];
+
+namespace
+{
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratch_mutex;
+int8_t*       g_scratch_address[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device
+// with that device current (the caller's current device is restored).
+int8_t* noalloc_staging(int device)
+{
+    if (device < 0 || device >= kMaxDevices) throw amr_error(AMR_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[device])
+    {
+        int prev = 0;
+        AMR_HIP_CHECK(hipGetDevice(&prev));
+        if (prev != device) AMR_HIP_CHECK(hipSetDevice(device));
+        void*            p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != device) (void)hipSetDevice(prev);
+        if (e != hipSuccess)
+            throw amr_error(AMR_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_address[device] = static_cast<int8_t*>(p);
+    }
+    return g_scratch_address[device];
+}
+} // namespace
 
 // The recorded forward pass of a solve with regrids, for the discrete
 // adjoint: every topology the solve went through, the dt of every step, the
@@ -1285,6 +1324,7 @@ static std::unique_ptr<amr_ctx> create_ctx(const amr_tree_config* cfg, int nvar)
                               : cfg->patch_size * cfg->patch_size * cfg->patch_size;
     const int P = cfg->patch_size + 2 * cfg->halo_width;
     c->flat     = cfg->rank == 2 ? P * P : P * P * P;
+    AMR_HIP_CHECK(hipGetDevice(&c->device));
     AMR_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     const size_t cap = cfg->capacity;
     for (int b = 0; b < 2; ++b)
@@ -1419,6 +1459,27 @@ int amr_tree_compute_flags(amr_ctx* ctx, const amr_criterion* crit, int8_t* host
             if (n < ctx->n) throw amr_error(AMR_ERR_ARGUMENT, "flag buffer smaller than the leaf count");
             run_criterion(ctx, crit);
             AMR_HIP_CHECK(hipMemcpyAsync(host_flags, ctx->flags, ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+        }
+    );
+}
+
+int amr_tree_compute_flags_noalloc(amr_ctx* ctx, const amr_criterion* crit, int8_t* host_flags, uint64_t n)
+{
+    return guarded(
+        [&]
+        {
+            if (!ctx || (!host_flags && n)) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
+            if (n < ctx->n) throw amr_error(AMR_ERR_ARGUMENT, "flag buffer smaller than the leaf count");
+            if (ctx->n > kNoAllocMaxLeaves)
+                throw amr_error(AMR_ERR_CAPACITY, "amr_tree_compute_flags_noalloc: " + std::to_string(ctx->n) +
+                                                      " leaves exceed its staging of " +
+                                                      std::to_string(kNoAllocMaxLeaves) + " leaves");
+            int8_t* stage = noalloc_staging(ctx->device);
+            // every leaf's flag is written before the copy: nothing in the
+            // staging needs clearing
+            run_criterion(ctx, crit, stage);
+            AMR_HIP_CHECK(hipMemcpyAsync(host_flags, stage, ctx->n, hipMemcpyDeviceToHost, ctx->stream));
             ctx->sync();
         }
     );

@@ -130,6 +130,15 @@ int amr_tree_reconstruct_criterion(amr_ctx* ctx, const amr_criterion* crit, int*
 /* The flags the criterion produces for the current leaves (copied out;
  * the reference's sync_refine_status_from_device, ndtree.hpp:639-656). */
 int amr_tree_compute_flags(amr_ctx* ctx, const amr_criterion* crit, int8_t* host_flags, uint64_t n);
+/* amr_tree_compute_flags without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-leaf flags are staged in a fixed 1 MiB device array that each GPU
+ * holds until the process ends, so trees of up to 1,048,576 leaves are
+ * accepted; larger trees return AMR_ERR_CAPACITY.  All calls on one GPU
+ * share that array: calls that overlap on one GPU (two threads, or two
+ * contexts' streams) race.  Results are those of amr_tree_compute_flags. */
+int amr_tree_compute_flags_noalloc(amr_ctx* ctx, const amr_criterion* crit, int8_t* host_flags, uint64_t n);
 
 /* ---- fields ------------------------------------------------------------ */
 
