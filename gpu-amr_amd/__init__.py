@@ -46,7 +46,20 @@ class TreeConfig(ctypes.Structure):
         ("halo_width", ctypes.c_int32),
         ("capacity", ctypes.c_uint64),
         ("domain_length", ctypes.c_double * 3),
+        ("patch_extents", ctypes.c_int32 * 3),
+        ("reserved", ctypes.c_int32),
     ]
+
+
+def patch_shape(rank, patch_size):
+    """Interior extents per layout dim (the reference's static_shape<N, M[, O]>,
+    last = physical x) from an int (square / cubic) or a sequence."""
+    if isinstance(patch_size, (int, np.integer)):
+        return (int(patch_size),) * rank
+    shape = tuple(int(x) for x in patch_size)
+    if len(shape) != rank:
+        raise ValueError(f"{rank}D patch needs {rank} extents, got {shape}")
+    return shape
 
 
 class Criterion(ctypes.Structure):
@@ -142,6 +155,16 @@ def load():
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or gpu-amr_amd/build.py")
+    # PyTorch ships its own HIP runtime under the same soname as the system's
+    # (libamdhip64.so.7).  Whichever is loaded first serves the whole
+    # process, and torch cannot initialise the device on a runtime loaded
+    # before it, so torch (when installed) is loaded first and the library
+    # shares its runtime; tensors and this library's buffers then live in one
+    # runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(str(LIB_PATH))
     lib.amr_last_error.restype = ctypes.c_char_p
     lib.amr_last_error.argtypes = []
@@ -178,9 +201,15 @@ class NdTree:
     """Periodic quad-/octree of leaf patches on one GPU (ndtree.hpp:94-2157)."""
 
     def __init__(self, rank, max_depth, patch_size, halo_width, capacity, domain_lengths=(1.0, 1.0, 1.0), _euler=None):
+        """patch_size: interior cells per dim (int), or the extents of layout
+        dims 0..rank-1 (static_shape<N, M[, O]>, the last is physical x)."""
         self.lib = load()
+        shape = patch_shape(rank, patch_size)
         cfg = TreeConfig()
-        cfg.rank, cfg.max_depth, cfg.patch_size, cfg.halo_width = rank, max_depth, patch_size, halo_width
+        cfg.rank, cfg.max_depth, cfg.halo_width = rank, max_depth, halo_width
+        cfg.patch_size = 0
+        for d in range(rank):
+            cfg.patch_extents[d] = shape[d]
         cfg.capacity = capacity
         L = list(domain_lengths) + [1.0] * (3 - len(domain_lengths))
         for i in range(3):
@@ -195,11 +224,13 @@ class NdTree:
         nv = ctypes.c_int32()
         _check(self.lib.amr_tree_fields(self.h, ctypes.byref(nv)))
         self.fields = int(nv.value)
-        self.rank, self.depth, self.n, self.halo = rank, max_depth, patch_size, halo_width
+        self.rank, self.depth, self.halo = rank, max_depth, halo_width
+        self.shape = shape
+        self.n = shape[0] if len(set(shape)) == 1 else shape  # square / cubic: the extent
         self.lengths = tuple(L[:rank])
-        self.ni = patch_size**rank
-        self.pad = patch_size + 2 * halo_width
-        self.flat = self.pad**rank
+        self.ni = int(np.prod(shape))
+        self.pad_shape = tuple(x + 2 * halo_width for x in shape)
+        self.flat = int(np.prod(self.pad_shape))
 
     def close(self):
         if getattr(self, "h", None):
@@ -263,7 +294,7 @@ class NdTree:
     # -- fields -----------------------------------------------------------
     def _field_shape(self, per_leaf):
         lead = (self.fields,) if self.fields > 1 else ()
-        return lead + (self.size(),) + (per_leaf,) * self.rank
+        return lead + (self.size(),) + tuple(per_leaf)
 
     def set_interior(self, values):
         """Interiors of all leaves; with several fields, field-major
@@ -274,13 +305,13 @@ class NdTree:
     def get_interior(self):
         out = np.zeros(self.fields * self.size() * self.ni, dtype=np.float64)
         _check(self.lib.amr_tree_get_interior(self.h, _vp(out), out.size))
-        return out.reshape(self._field_shape(self.n))
+        return out.reshape(self._field_shape(self.shape))
 
     def get_patches(self):
         """Padded patches after halo_exchange_update (corners 0)."""
         out = np.zeros(self.fields * self.size() * self.flat, dtype=np.float64)
         _check(self.lib.amr_tree_get_patches(self.h, _vp(out), out.size))
-        return out.reshape(self._field_shape(self.pad))
+        return out.reshape(self._field_shape(self.pad_shape))
 
     def write_vtk(self, path):
         """vtk_print::print (vtk_print.hpp:53-254): legacy-VTK binary
@@ -316,7 +347,7 @@ class NdTree:
         maxc = float(1 << self.depth)
         for i in range(self.rank):
             pm = np.array([float(1 << (self.depth - int(x))) for x in lv])
-            out[:, i] = (self.lengths[i] * pm / maxc) / float(self.n)
+            out[:, i] = (self.lengths[i] * pm / maxc) / float(self.shape[self.rank - 1 - i])
         return out
 
     def cell_centers(self):
@@ -324,7 +355,7 @@ class NdTree:
         coords, lv = decode_keys(self.keys(), self.rank)
         maxc = float(1 << self.depth)
         cs = self.cell_sizes(lv)
-        idx = np.indices((self.n,) * self.rank).reshape(self.rank, -1)  # layout order
+        idx = np.indices(self.shape).reshape(self.rank, -1)  # layout order
         out = np.zeros((coords.shape[0], self.ni, self.rank))
         for i in range(self.rank):
             origin = self.lengths[i] * coords[:, i].astype(np.float64) / maxc

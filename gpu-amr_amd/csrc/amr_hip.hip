@@ -78,31 +78,48 @@ enum : int
     NEEDS_RUNS = 1,
 };
 
-// Kernel table of one (rank, patch size, halo width) instance.
+// Kernel table of one patch shape: (rank, extents, halo width).  Every
+// function takes the table itself first, which carries the extents of the
+// any-shape instance (the fixed-shape instances have them built in).
 struct ops
 {
-    int rank, n, h;
-    void (*step)(int variant, const double*, double*, const uint64_t*, const face_link*, const uint32_t*,
+    int     rank, n, h; // n: the extent of every dim of a fixed cubic shape, 0 for the any-shape instance
+    int32_t ext[3];     // interior cells per layout dim
+    void (*step)(ops const&, int variant, const double*, double*, const uint64_t*, const face_link*, const uint32_t*,
                  step_params const&, const step_params*, const uint16_t*, uint32_t, step_aux const&, hipStream_t);
     int (*variant_count)();
     char const* (*variant_name)(int);
     int (*variant_needs)(int);
-    void (*padded)(const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
-    void (*criterion)(int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double, double,
-                      int, int, int8_t*, uint32_t, hipStream_t);
-    void (*emit)(const uint64_t*, const double*, uint32_t, int, const uint8_t*, const uint8_t*, const uint32_t*,
-                 const uint32_t*, uint64_t*, double*, regrid_summary*, int, size_t, hipStream_t);
-    void (*adj_step)(const double*, const double*, const uint64_t*, const face_link*, const uint32_t*,
+    void (*padded)(ops const&, const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
+    void (*criterion)(ops const&, int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double,
+                      double, int, int, int8_t*, uint32_t, hipStream_t);
+    void (*emit)(ops const&, const uint64_t*, const double*, uint32_t, int, const uint8_t*, const uint8_t*,
+                 const uint32_t*, const uint32_t*, uint64_t*, double*, regrid_summary*, int, size_t, hipStream_t);
+    void (*adj_step)(ops const&, const double*, const double*, const uint64_t*, const face_link*, const uint32_t*,
                      adjoint_params const&, double*, double*, uint32_t, hipStream_t);
-    void (*adj_regrid)(const double*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*, double*, uint32_t,
-                       hipStream_t);
+    void (*adj_regrid)(ops const&, const double*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
+                       double*, uint32_t, hipStream_t);
     // two fused steps of a single-level tree (null: not built for the shape)
     void (*step2)(const double*, double*, step2_params const&, uint32_t, hipStream_t);
     // Euler fields (field-major, stride fstride doubles between fields)
-    void (*euler_dt)(const double*, size_t, const uint64_t*, euler_params const&, euler_scalars*, uint32_t,
+    void (*euler_dt)(ops const&, const double*, size_t, const uint64_t*, euler_params const&, euler_scalars*, uint32_t,
                      hipStream_t);
-    void (*euler_step)(const double*, double*, size_t, const uint64_t*, const face_link*, const uint32_t*,
+    void (*euler_step)(ops const&, const double*, double*, size_t, const uint64_t*, const face_link*, const uint32_t*,
                        euler_params const&, euler_scalars*, uint32_t, hipStream_t);
+
+    int ni() const { return (rank == 2) ? ext[0] * ext[1] : ext[0] * ext[1] * ext[2]; }
+    int padded_cells() const
+    {
+        int p = 1;
+        for (int d = 0; d < rank; ++d) p *= ext[d] + 2 * h;
+        return p;
+    }
+    // whether two tables run the same kernels on the same shape
+    bool same_shape(ops const& o) const
+    {
+        return step == o.step && rank == o.rank && h == o.h && ext[0] == o.ext[0] && ext[1] == o.ext[1] &&
+               ext[2] == o.ext[2];
+    }
 };
 
 template <int R, int N, int H>
@@ -135,7 +152,7 @@ struct instance
             hipLaunchKernelGGL(k_step_2d16<NTS>, grid, dim3(64), lds, s, u, un, keys, links, metas, prm, table, inst, n);
     }
 
-    static void step(int variant, const double* u, double* un, const uint64_t* keys, const face_link* links,
+    static void step(ops const&, int variant, const double* u, double* un, const uint64_t* keys, const face_link* links,
                      const uint32_t* metas, step_params const& prm, const step_params* table,
                      const uint16_t* inst, uint32_t n, step_aux const& aux, hipStream_t s)
     {
@@ -173,14 +190,14 @@ struct instance
                                inst, n);
         check_launch("k_step");
     }
-    static void padded(const double* u, double* pad, const face_link* links, const uint32_t* metas, uint32_t n,
+    static void padded(ops const&, const double* u, double* pad, const face_link* links, const uint32_t* metas, uint32_t n,
                        hipStream_t s)
     {
         if (n == 0) return;
         hipLaunchKernelGGL((k_padded_view<R, N, H, NTP>), dim3(n), dim3(NTP), 0, s, u, pad, links, metas, n);
         check_launch("k_padded_view");
     }
-    static void criterion(int kind, const double* u, const uint64_t* keys, const face_link* links,
+    static void criterion(ops const&, int kind, const double* u, const uint64_t* keys, const face_link* links,
                           const uint32_t* metas, double r, double c, int mn, int mx, int8_t* flags, uint32_t n,
                           hipStream_t s)
     {
@@ -193,7 +210,7 @@ struct instance
                                mn, mx, flags, n);
         check_launch("k_criterion");
     }
-    static void emit(const uint64_t* keys, const double* u, uint32_t n, int depth, const uint8_t* mark,
+    static void emit(ops const&, const uint64_t* keys, const double* u, uint32_t n, int depth, const uint8_t* mark,
                      const uint8_t* head, const uint32_t* counts, const uint32_t* pos, uint64_t* keys_out,
                      double* u_out, regrid_summary* summary, int nfields, size_t fstride, hipStream_t s)
     {
@@ -202,7 +219,7 @@ struct instance
                            pos, keys_out, u_out, summary, nfields, fstride);
         check_launch("k_emit");
     }
-    static void adj_step(const double* lam_next, const double* u, const uint64_t* keys, const face_link* links,
+    static void adj_step(ops const&, const double* lam_next, const double* u, const uint64_t* keys, const face_link* links,
                          const uint32_t* metas, adjoint_params const& prm, double* lam, double* vpart, uint32_t n,
                          hipStream_t s)
     {
@@ -211,7 +228,7 @@ struct instance
                            vpart, n);
         check_launch("k_adj_step");
     }
-    static void adj_regrid(const double* lam_new, const uint8_t* mark, const uint8_t* head, const uint32_t* counts,
+    static void adj_regrid(ops const&, const double* lam_new, const uint8_t* mark, const uint8_t* head, const uint32_t* counts,
                            const uint32_t* pos, double* lam_old, uint32_t n_old, hipStream_t s)
     {
         if (n_old == 0) return;
@@ -235,14 +252,14 @@ struct instance
         }
         check_launch("k_step2");
     }
-    static void euler_dt(const double* u, size_t fs, const uint64_t* keys, euler_params const& prm, euler_scalars* sc,
+    static void euler_dt(ops const&, const double* u, size_t fs, const uint64_t* keys, euler_params const& prm, euler_scalars* sc,
                          uint32_t n, hipStream_t s)
     {
         if (n == 0) return;
         hipLaunchKernelGGL((k_euler_dt<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, fs, keys, prm, sc, n);
         check_launch("k_euler_dt");
     }
-    static void euler_step(const double* u, double* un, size_t fs, const uint64_t* keys, const face_link* links,
+    static void euler_step(ops const&, const double* u, double* un, size_t fs, const uint64_t* keys, const face_link* links,
                            const uint32_t* metas, euler_params const& prm, euler_scalars* sc, uint32_t n,
                            hipStream_t s)
     {
@@ -265,9 +282,22 @@ struct instance
     static constexpr ops table()
     {
         constexpr bool fused = R == 3 && N == 8;
-        return ops{ R,       N,          H,       &step,     &variant_count, &variant_name, &variant_needs,
-                    &padded, &criterion, &emit, &adj_step, &adj_regrid, fused ? &step2 : nullptr,
-                    &euler_dt, &euler_step };
+        return ops{ R,
+                    N,
+                    H,
+                    { N, N, R == 3 ? N : 1 },
+                    &step,
+                    &variant_count,
+                    &variant_name,
+                    &variant_needs,
+                    &padded,
+                    &criterion,
+                    &emit,
+                    &adj_step,
+                    &adj_regrid,
+                    fused ? &step2 : nullptr,
+                    &euler_dt,
+                    &euler_step };
     }
 };
 
@@ -287,6 +317,117 @@ ops const* find_ops(int rank, int n, int h)
     for (auto const& o : kOps)
         if (o.rank == rank && o.n == n && o.h == h) return &o;
     return nullptr;
+}
+
+// Kernels for every other patch shape (any even extents, square or not):
+// the box kernels, the extents passed at launch, one workgroup per leaf of
+// 64 threads for patches of up to 64 cells and 256 otherwise.  They read
+// neighbour cells from memory instead of LDS tiles; the arithmetic is the
+// fixed-shape kernels', so results are the same bits.
+template <int R>
+struct box_instance
+{
+    static box<R> shape(ops const& o) { return box<R>{ { o.ext[0], o.ext[1], o.ext[2] } }; }
+    static bool   small(ops const& o) { return o.ni() <= 64; }
+
+    static int         variant_count() { return 1; }
+    static char const* variant_name(int) { return "box"; }
+    static int         variant_needs(int) { return 0; }
+
+    static void step(ops const& o, int, const double* u, double* un, const uint64_t* keys, const face_link* links,
+                     const uint32_t* metas, step_params const& prm, const step_params* table, const uint16_t* inst,
+                     uint32_t n, step_aux const&, hipStream_t s)
+    {
+        if (n == 0) return;
+        const dim3 grid(xcd_grid(n));
+        if (small(o))
+            hipLaunchKernelGGL((k_step_box<R, 64>), grid, dim3(64), 0, s, shape(o), u, un, keys, links, metas, prm,
+                               table, inst, n);
+        else
+            hipLaunchKernelGGL((k_step_box<R, 256>), grid, dim3(256), 0, s, shape(o), u, un, keys, links, metas, prm,
+                               table, inst, n);
+        check_launch("k_step_box");
+    }
+    static void padded(ops const& o, const double* u, double* pad, const face_link* links, const uint32_t* metas,
+                       uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_padded_box<R, 256>), dim3(n), dim3(256), 0, s, shape(o), o.h, u, pad, links, metas, n);
+        check_launch("k_padded_box");
+    }
+    static void criterion(ops const& o, int kind, const double* u, const uint64_t* keys, const face_link* links,
+                          const uint32_t* metas, double r, double c, int mn, int mx, int8_t* flags, uint32_t n,
+                          hipStream_t s)
+    {
+        if (n == 0) return;
+        if (kind == AMR_CRITERION_GRADIENT)
+            hipLaunchKernelGGL((k_criterion_gradient_box<R, 256>), dim3(n), dim3(256), 0, s, shape(o), u, keys, links,
+                               metas, r, c, mn, mx, flags, n);
+        else
+            hipLaunchKernelGGL((k_criterion_max_box<R, 256>), dim3(n), dim3(256), 0, s, shape(o), o.h, u, keys, links,
+                               metas, r, c, mn, mx, flags, n);
+        check_launch("k_criterion_box");
+    }
+    static void emit(ops const& o, const uint64_t* keys, const double* u, uint32_t n, int depth, const uint8_t* mark,
+                     const uint8_t* head, const uint32_t* counts, const uint32_t* pos, uint64_t* keys_out,
+                     double* u_out, regrid_summary* summary, int nfields, size_t fstride, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_emit_box<R, 256>), dim3(xcd_grid(n)), dim3(256), 0, s, shape(o), keys, u, n, depth, mark,
+                           head, counts, pos, keys_out, u_out, summary, nfields, fstride);
+        check_launch("k_emit_box");
+    }
+    static void adj_step(ops const& o, const double* lam_next, const double* u, const uint64_t* keys,
+                         const face_link* links, const uint32_t* metas, adjoint_params const& prm, double* lam,
+                         double* vpart, uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_adj_step_box<R, 256>), dim3(n), dim3(256), 0, s, shape(o), lam_next, u, keys, links,
+                           metas, prm, lam, vpart, n);
+        check_launch("k_adj_step_box");
+    }
+    static void adj_regrid(ops const& o, const double* lam_new, const uint8_t* mark, const uint8_t* head,
+                           const uint32_t* counts, const uint32_t* pos, double* lam_old, uint32_t n_old, hipStream_t s)
+    {
+        if (n_old == 0) return;
+        hipLaunchKernelGGL((k_adj_regrid_box<R, 256>), dim3(n_old), dim3(256), 0, s, shape(o), lam_new, mark, head,
+                           counts, pos, lam_old, n_old);
+        check_launch("k_adj_regrid_box");
+    }
+    static void euler_dt(ops const& o, const double* u, size_t fs, const uint64_t* keys, euler_params const& prm,
+                         euler_scalars* sc, uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_euler_dt_box<R, 256>), dim3(xcd_grid(n)), dim3(256), 0, s, shape(o), u, fs, keys, prm, sc,
+                           n);
+        check_launch("k_euler_dt_box");
+    }
+    static void euler_step(ops const& o, const double* u, double* un, size_t fs, const uint64_t* keys,
+                           const face_link* links, const uint32_t* metas, euler_params const& prm, euler_scalars* sc,
+                           uint32_t n, hipStream_t s)
+    {
+        if (n == 0) return;
+        hipLaunchKernelGGL((k_euler_step_box<R, 256>), dim3(xcd_grid(n)), dim3(256), 0, s, shape(o), u, un, fs, keys,
+                           links, metas, prm, sc, n);
+        check_launch("k_euler_step_box");
+    }
+    static ops table(int h, const int32_t* ext)
+    {
+        return ops{ R,       0,          h,     { ext[0], ext[1], R == 3 ? ext[2] : 1 },
+                    &step,   &variant_count, &variant_name, &variant_needs,
+                    &padded, &criterion, &emit, &adj_step,
+                    &adj_regrid, nullptr, &euler_dt, &euler_step };
+    }
+};
+
+// The kernel table of a patch shape: a fixed-shape instance for the cubic
+// shapes built above, else the any-shape kernels.
+ops shape_ops(int rank, const int32_t* ext, int h)
+{
+    bool cubic = true;
+    for (int d = 1; d < rank; ++d) cubic = cubic && ext[d] == ext[0];
+    if (ops const* o = cubic ? find_ops(rank, ext[0], h) : nullptr) return *o;
+    return rank == 2 ? box_instance<2>::table(h, ext) : box_instance<3>::table(h, ext);
 }
 
 template <typename Fn>
@@ -503,7 +644,8 @@ __global__ void k_fill_instance(uint16_t* __restrict__ inst, uint64_t count, uin
 struct amr_ctx
 {
     amr_tree_config cfg{};
-    ops const*      op = nullptr;
+    ops             op_table{};     // this tree's kernel table (its patch shape)
+    ops const*      op = &op_table;
     hipStream_t     stream{};
     int             device = 0; // the GPU current at creation (owns the stream and buffers)
     int             rank = 0, depth = 0, ni = 0, flat = 0, ndir = 0;
@@ -568,7 +710,7 @@ struct amr_ctx
         const double pm   = static_cast<double>(1u << (depth - level));
         const double maxc = static_cast<double>(1u << depth);
         const double ps   = cfg.domain_length[axis] * pm / maxc;
-        return ps / static_cast<double>(cfg.patch_size);
+        return ps / static_cast<double>(op->ext[rank - 1 - axis]); // cells along physical axis `axis`
     }
 
     // compute_time_step_cpu (amr_solver.hpp:355-413): the per-cell minimum is
@@ -666,7 +808,7 @@ struct amr_ctx
         ensure_usable();
         step_params prm = make_params(dt);
         prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
-        op->step(variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
+        op->step(*op, variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
                  runs.view(), stream);
         cur = 1 - cur;
     }
@@ -682,7 +824,7 @@ struct amr_ctx
             [&](int v, const double* src, double* dst)
             {
                 prm.reverse ^= 1u; // trial steps alternate like a solve's
-                op->step(v, src, dst, keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
+                op->step(*op, v, src, dst, keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
                          runs.view(), stream);
             },
             stream, trials, best_us);
@@ -710,14 +852,14 @@ struct amr_ctx
         hipLaunchKernelGGL(k_euler_begin, dim3(1), dim3(1), 0, stream, esc, remaining);
         check_launch("k_euler_begin");
         const uint32_t nn = static_cast<uint32_t>(n);
-        if (steps > 0) op->euler_dt(u[cur], fstride(), keys[kc], prm, esc, nn, stream);
+        if (steps > 0) op->euler_dt(*op, u[cur], fstride(), keys[kc], prm, esc, nn, stream);
         for (uint64_t s = 0; s < steps; ++s)
         {
             hipLaunchKernelGGL(k_euler_finalize, dim3(1), dim3(1), 0, stream, esc, cfl);
             check_launch("k_euler_finalize");
             prm.reverse = static_cast<uint32_t>(sweeps++ & 1);
             prm.next_dt = s + 1 < steps ? 1u : 0u;
-            op->euler_step(u[cur], u[1 - cur], fstride(), keys[kc], links, metas, prm, esc, nn, stream);
+            op->euler_step(*op, u[cur], u[1 - cur], fstride(), keys[kc], links, metas, prm, esc, nn, stream);
             cur = 1 - cur;
         }
     }
@@ -736,7 +878,7 @@ struct amr_ctx
         const euler_params prm = make_euler_params();
         hipLaunchKernelGGL(k_euler_begin, dim3(1), dim3(1), 0, stream, esc, 0.0);
         check_launch("k_euler_begin");
-        op->euler_dt(u[cur], fstride(), keys[kc], prm, esc, static_cast<uint32_t>(n), stream);
+        op->euler_dt(*op, u[cur], fstride(), keys[kc], prm, esc, static_cast<uint32_t>(n), stream);
         const euler_scalars e = euler_read();
         double raw;
         std::memcpy(&raw, &e.dtmin_bits, sizeof(raw));
@@ -807,7 +949,7 @@ struct amr_ctx
         if (s.err & 2u) throw amr_error(AMR_ERR_UNSUPPORTED, "coarsening to the root level is not supported");
         if (s.n_refine == 0 && s.n_coarsen == 0) return false;
         if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
-        op->emit(keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, nvar,
+        op->emit(*op, keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, nvar,
                  fstride(), stream);
         cur = 1 - cur;
         kc  = 1 - kc;
@@ -822,7 +964,8 @@ struct amr_ctx
 // own velocity, cfl and dt (through the per-instance constant table).
 struct amr_batch
 {
-    ops const*               op = nullptr;
+    ops                      op_table{};
+    ops const*               op = &op_table;
     hipStream_t              stream{};
     int                      ni = 0, ndir = 0;
     uint64_t                 total = 0;
@@ -861,7 +1004,7 @@ struct amr_batch
     {
         step_params prm = host_table[0];
         prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
-        op->step(v, u[cur], u[1 - cur], keys, links, metas, prm, table, inst, static_cast<uint32_t>(total), runs.view(),
+        op->step(*op, v, u[cur], u[1 - cur], keys, links, metas, prm, table, inst, static_cast<uint32_t>(total), runs.view(),
                  stream);
         cur = 1 - cur;
     }
@@ -915,7 +1058,7 @@ static void run_criterion(amr_ctx* ctx, amr_criterion const* crit, int8_t* out =
     if (!crit) throw amr_error(AMR_ERR_ARGUMENT, "null criterion");
     if (crit->kind != AMR_CRITERION_MAX_VALUE && crit->kind != AMR_CRITERION_GRADIENT)
         throw amr_error(AMR_ERR_ARGUMENT, "unknown criterion kind");
-    ctx->op->criterion(crit->kind, ctx->u[ctx->cur], ctx->keys[ctx->kc], ctx->links, ctx->metas,
+    ctx->op->criterion(*ctx->op, crit->kind, ctx->u[ctx->cur], ctx->keys[ctx->kc], ctx->links, ctx->metas,
                        crit->refine_threshold, crit->coarsen_threshold, crit->min_level, crit->max_level,
                        out ? out : ctx->flags, static_cast<uint32_t>(ctx->n), ctx->stream);
 }
@@ -985,7 +1128,8 @@ struct amr_tape
         uint32_t*           pos    = nullptr;
     };
 
-    ops const*                 op = nullptr;
+    ops                        op_table{};
+    ops const*                 op = &op_table;
     amr_tree_config            cfg{};
     int                        rank = 0, depth = 0, ni = 0, ndir = 0, variant = 0;
     hipStream_t                stream{};
@@ -1039,7 +1183,7 @@ struct amr_tape
     {
         const double pm   = static_cast<double>(1u << (depth - level));
         const double maxc = static_cast<double>(1u << depth);
-        return (cfg.domain_length[axis] * pm / maxc) / static_cast<double>(cfg.patch_size);
+        return (cfg.domain_length[axis] * pm / maxc) / static_cast<double>(op->ext[cfg.rank - 1 - axis]);
     }
 
     step_params step_consts(double dt) const
@@ -1161,7 +1305,7 @@ struct amr_tape
     // The recorded regrid transfer (old -> new leaves) applied to src.
     void replay_transfer(regrid_record const& r, const double* src, double* dst)
     {
-        op->emit(topos[static_cast<size_t>(r.topo_before)].keys, src, static_cast<uint32_t>(r.n_old), depth, r.mark,
+        op->emit(*op, topos[static_cast<size_t>(r.topo_before)].keys, src, static_cast<uint32_t>(r.n_old), depth, r.mark,
                  r.head, r.counts, r.pos, scratch_keys, dst, scratch_summary, 1, 0, stream);
     }
 
@@ -1170,7 +1314,7 @@ struct amr_tape
         topology const& t   = topo(k);
         step_params     prm = step_consts(dts[k]);
         prm.reverse         = static_cast<uint32_t>(k & 1);
-        op->step(variant, src, dst, t.keys, t.links, t.metas, prm, nullptr, nullptr, static_cast<uint32_t>(t.n),
+        op->step(*op, variant, src, dst, t.keys, t.links, t.metas, prm, nullptr, nullptr, static_cast<uint32_t>(t.n),
                  step_aux{}, stream);
     }
 
@@ -1243,12 +1387,12 @@ struct amr_tape
             for (uint64_t k = e; k-- > c;)
             {
                 topology const& t = topo(k);
-                op->adj_step(l, states[k - c], t.keys, t.links, t.metas, adjoint_consts(dts[k]), lo,
+                op->adj_step(*op, l, states[k - c], t.keys, t.links, t.metas, adjoint_consts(dts[k]), lo,
                              vpart + 3 * vbase[k], static_cast<uint32_t>(t.n), stream);
                 std::swap(l, lo);
                 if (auto const* r = regrid_before(k))
                 {
-                    op->adj_regrid(l, r->mark, r->head, r->counts, r->pos, lo, static_cast<uint32_t>(r->n_old), stream);
+                    op->adj_regrid(*op, l, r->mark, r->head, r->counts, r->pos, lo, static_cast<uint32_t>(r->n_old), stream);
                     std::swap(l, lo);
                 }
             }
@@ -1302,6 +1446,31 @@ extern "C"
 
 const char* amr_last_error(void) { return g_err.c_str(); }
 
+// The interior extents of the configured patch (layout dims 0..rank-1),
+// checked: static_shape<N, M[, O]> (include/containers/static_layout.hpp)
+// with every extent even (the children split each dim in half) and at
+// least twice the halo width (a ghost band lies within one finer cell).
+static void patch_extents(const amr_tree_config* cfg, int32_t* ext)
+{
+    const bool given = cfg->patch_extents[0] || cfg->patch_extents[1] || cfg->patch_extents[2];
+    if (cfg->halo_width < 1 || cfg->halo_width > 8) throw amr_error(AMR_ERR_ARGUMENT, "halo_width must be 1..8");
+    int64_t cells = 1;
+    for (int d = 0; d < cfg->rank; ++d)
+    {
+        ext[d] = given ? cfg->patch_extents[d] : cfg->patch_size;
+        if (given && cfg->patch_size != 0 && cfg->patch_size != ext[d])
+            throw amr_error(AMR_ERR_ARGUMENT, "patch_size and patch_extents disagree (leave patch_size 0)");
+        if (ext[d] < 2 || ext[d] > 4096 || ext[d] % 2 != 0)
+            throw amr_error(AMR_ERR_ARGUMENT, "patch extents must be even and within 2..4096");
+        if (ext[d] < 2 * cfg->halo_width)
+            throw amr_error(AMR_ERR_ARGUMENT, "patch extents must be at least twice the halo width");
+        cells *= ext[d] + 2 * cfg->halo_width;
+    }
+    if (given && cfg->rank == 2 && cfg->patch_extents[2] != 0 && cfg->patch_extents[2] != 1)
+        throw amr_error(AMR_ERR_ARGUMENT, "a 2D patch has no third extent");
+    if (cells > (int64_t{ 1 } << 24)) throw amr_error(AMR_ERR_ARGUMENT, "padded patch larger than 2^24 cells");
+}
+
 // ndtree(capacity) with `nvar` fields per cell (ndtree.hpp:320-425).
 static std::unique_ptr<amr_ctx> create_ctx(const amr_tree_config* cfg, int nvar)
 {
@@ -1310,20 +1479,18 @@ static std::unique_ptr<amr_ctx> create_ctx(const amr_tree_config* cfg, int nvar)
         throw amr_error(AMR_ERR_ARGUMENT, "max_depth out of range");
     if (cfg->capacity < 1 || cfg->capacity > 0xFFFFFFFFull / 8)
         throw amr_error(AMR_ERR_ARGUMENT, "capacity out of range");
-    ops const* op = find_ops(cfg->rank, cfg->patch_size, cfg->halo_width);
-    if (!op) throw amr_error(AMR_ERR_UNSUPPORTED, "no kernels built for this (rank, patch_size, halo_width)");
-    auto c   = std::make_unique<amr_ctx>();
-    c->cfg   = *cfg;
-    c->op    = op;
-    c->rank  = cfg->rank;
-    c->depth = cfg->max_depth;
-    c->ndir  = 2 * cfg->rank;
-    c->cap   = cfg->capacity;
-    c->nvar  = nvar;
-    c->ni    = cfg->rank == 2 ? cfg->patch_size * cfg->patch_size
-                              : cfg->patch_size * cfg->patch_size * cfg->patch_size;
-    const int P = cfg->patch_size + 2 * cfg->halo_width;
-    c->flat     = cfg->rank == 2 ? P * P : P * P * P;
+    int32_t ext[3] = { 1, 1, 1 };
+    patch_extents(cfg, ext);
+    auto c      = std::make_unique<amr_ctx>();
+    c->cfg      = *cfg;
+    c->op_table = shape_ops(cfg->rank, ext, cfg->halo_width);
+    c->rank     = cfg->rank;
+    c->depth    = cfg->max_depth;
+    c->ndir     = 2 * cfg->rank;
+    c->cap      = cfg->capacity;
+    c->nvar     = nvar;
+    c->ni       = c->op->ni();
+    c->flat     = c->op->padded_cells();
     AMR_HIP_CHECK(hipGetDevice(&c->device));
     AMR_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     const size_t cap = cfg->capacity;
@@ -1348,7 +1515,7 @@ static std::unique_ptr<amr_ctx> create_ctx(const amr_tree_config* cfg, int nvar)
     AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
     c->n         = 1;
     c->max_level = 0;
-    if (run_lists::wanted(op)) c->runs.allocate(cap);
+    if (run_lists::wanted(c->op)) c->runs.allocate(cap);
     c->build_table();
     return c;
 }
@@ -1549,7 +1716,7 @@ static void materialize_padded(amr_ctx* ctx)
     ctx->ensure_usable();
     if (!ctx->padded) ctx->padded = dev_alloc<double>(ctx->cap * ctx->flat * ctx->nvar);
     for (int v = 0; v < ctx->nvar; ++v)
-        ctx->op->padded(ctx->u[ctx->cur] + v * ctx->fstride(), ctx->padded + v * ctx->cap * ctx->flat, ctx->links,
+        ctx->op->padded(*ctx->op, ctx->u[ctx->cur] + v * ctx->fstride(), ctx->padded + v * ctx->cap * ctx->flat, ctx->links,
                         ctx->metas, static_cast<uint32_t>(ctx->n), ctx->stream);
 }
 
@@ -1639,7 +1806,8 @@ int amr_tree_write_vtk(amr_ctx* ctx, const char* path)
         {
             if (!ctx || !path) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
             ctx->ensure_usable();
-            const int      R = ctx->rank, n = ctx->cfg.patch_size, h = ctx->cfg.halo_width;
+            const int      R = ctx->rank, h = ctx->cfg.halo_width;
+            const int32_t* ext = ctx->op->ext;
             const uint64_t leaves = ctx->n, ni = static_cast<uint64_t>(ctx->ni), cells = leaves * ni;
             std::vector<uint64_t> keys(leaves);
             std::vector<double>   u(cells * ctx->nvar);
@@ -1677,7 +1845,8 @@ int amr_tree_write_vtk(amr_ctx* ctx, const char* path)
                 for (uint64_t l = 0; l < ni; ++l)
                 {
                     // interior cell l, row-major with x last
-                    const int    ix = static_cast<int>(l % n), iy = static_cast<int>((l / n) % n);
+                    const int    ix = static_cast<int>(l % ext[R - 1]);
+                    const int    iy = static_cast<int>((l / ext[R - 1]) % ext[R - 2]);
                     const double x  = org[0] + static_cast<double>(ix) * cs[0];
                     const double y  = org[1] + static_cast<double>(iy) * cs[1];
                     if (R == 2)
@@ -1692,7 +1861,7 @@ int amr_tree_write_vtk(amr_ctx* ctx, const char* path)
                     }
                     else
                     {
-                        const int    iz = static_cast<int>(l / (static_cast<uint64_t>(n) * n));
+                        const int    iz = static_cast<int>(l / (static_cast<uint64_t>(ext[1]) * ext[2]));
                         const double z  = org[2] + static_cast<double>(iz) * cs[2];
                         // VTK_VOXEL corner order
                         for (int k = 0; k < 8; ++k)
@@ -1721,7 +1890,6 @@ int amr_tree_write_vtk(amr_ctx* ctx, const char* path)
             // the flags are reproduced as written.
             f << "SCALARS is_halo int 1\n";
             f << "LOOKUP_TABLE default\n";
-            const int P = n + 2 * h;
             std::vector<int> halo_flag(ni);
             for (uint64_t j = 0; j < ni; ++j)
             {
@@ -1729,10 +1897,12 @@ int amr_tree_write_vtk(amr_ctx* ctx, const char* path)
                 int      flag = 0;
                 for (int d = 0; d < R; ++d)
                 {
+                    // padded layout: extent n(d) + 2h, strides of the later dims
                     uint64_t stride = 1;
-                    for (int e = d + 1; e < R; ++e) stride *= static_cast<uint64_t>(P);
+                    for (int e = d + 1; e < R; ++e) stride *= static_cast<uint64_t>(ext[e] + 2 * h);
+                    const uint64_t P   = static_cast<uint64_t>(ext[d] + 2 * h);
                     const uint64_t rel = rem / stride;
-                    if (rel < static_cast<uint64_t>(h) || rel >= static_cast<uint64_t>(P - h)) flag = 1;
+                    if (rel < static_cast<uint64_t>(h) || rel >= P - static_cast<uint64_t>(h)) flag = 1;
                     rem %= stride;
                     if (flag) break;
                 }
@@ -2010,14 +2180,14 @@ int amr_batch_create(amr_ctx* const* trees, uint32_t count, amr_batch** out)
             if (count > 65535) throw amr_error(AMR_ERR_ARGUMENT, "at most 65535 instances per batch");
             *out = nullptr;
             auto b  = std::make_unique<amr_batch>();
-            b->op   = trees[0]->op;
+            b->op_table = *trees[0]->op;
             b->ni   = trees[0]->ni;
             b->ndir = trees[0]->ndir;
             for (uint32_t i = 0; i < count; ++i)
             {
                 amr_ctx* t = trees[i];
                 if (!t) throw amr_error(AMR_ERR_ARGUMENT, "null tree");
-                if (t->op != b->op || t->depth != trees[0]->depth)
+                if (!t->op->same_shape(*b->op) || t->depth != trees[0]->depth)
                     throw amr_error(AMR_ERR_ARGUMENT, "batched trees must share rank, depth, patch size and halo");
                 t->ensure_usable();
                 t->sync();
@@ -2157,7 +2327,7 @@ int amr_batch_autotune(amr_batch* b, uint32_t trial_steps, int32_t* chosen, floa
                 [&](int v, const double* src, double* dst)
                 {
                     prm.reverse ^= 1u;
-                    b->op->step(v, src, dst, b->keys, b->links, b->metas, prm, b->table, b->inst,
+                    b->op->step(*b->op, v, src, dst, b->keys, b->links, b->metas, prm, b->table, b->inst,
                                 static_cast<uint32_t>(b->total), b->runs.view(), b->stream);
                 },
                 b->stream, trial_steps, best_us);
@@ -2191,7 +2361,7 @@ int amr_tape_record(amr_ctx* ctx, const amr_criterion* criterion, uint64_t steps
             if (checkpoint_every == 0) throw amr_error(AMR_ERR_ARGUMENT, "checkpoint_every must be positive");
             if (ctx->n == 0) throw amr_error(AMR_ERR_ARGUMENT, "empty tree");
             auto t        = std::make_unique<amr_tape>();
-            t->op         = ctx->op;
+            t->op_table   = *ctx->op;
             t->cfg        = ctx->cfg;
             t->rank       = ctx->rank;
             t->depth      = ctx->depth;

@@ -105,6 +105,59 @@ struct patch_shape
 };
 
 // ---------------------------------------------------------------------------
+// Patch extents of any shape: layout dim d holds n(d) interior cells (the
+// reference's static_shape<N, M[, O]>: d = 0 slowest, the last is physical
+// x), row-major with the last index fastest.  Every extent is even (a child
+// covers half of each dim).  The general kernels take a box as an argument;
+// the fixed-shape kernels build one from constants, which the compiler folds.
+// ---------------------------------------------------------------------------
+template <int R>
+struct box
+{
+    static constexpr int F    = 1 << R;       // children
+    static constexpr int K    = 1 << (R - 1); // face children
+    static constexpr int NDIR = 2 * R;
+
+    int32_t e[3];
+
+    static constexpr box cube(int n) { return box{ { n, n, n } }; }
+
+    __host__ __device__ constexpr int n(int d) const { return e[d]; }
+    __host__ __device__ constexpr int half(int d) const { return e[d] / 2; }
+    __host__ __device__ constexpr int ni() const { return (R == 2) ? e[0] * e[1] : e[0] * e[1] * e[2]; }
+    // cells of a face normal to layout dim `dim`
+    __host__ __device__ constexpr int face(int dim) const { return ni() / e[dim]; }
+    // stride of layout dim d in the compact interior
+    __host__ __device__ constexpr int stride(int d) const
+    {
+        int s = 1;
+        for (int k = R - 1; k > d; --k) s *= e[k];
+        return s;
+    }
+    __host__ __device__ constexpr int lin(const int* c) const
+    {
+        int l = 0;
+        for (int d = 0; d < R; ++d) l = l * e[d] + c[d];
+        return l;
+    }
+    __host__ __device__ constexpr void coords(int l, int* c) const
+    {
+        for (int d = R - 1; d >= 0; --d)
+        {
+            c[d] = l % e[d];
+            l /= e[d];
+        }
+    }
+    // padded view with h ghost layers per side: (n(d) + 2h) per dim
+    __host__ __device__ constexpr int padded(int h) const
+    {
+        int p = 1;
+        for (int d = 0; d < R; ++d) p *= e[d] + 2 * h;
+        return p;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Workgroup -> item mapping.  Workgroups are dispatched round-robin over the
 // XCDs (workgroup b lands on XCD b % 8).  Giving each XCD one contiguous range
 // of Morton-ordered leaves keeps face neighbours inside the same L2.  The grid

@@ -43,25 +43,31 @@ struct adjoint_params
 };
 
 // In-patch part of (A^T lambda')_c.
-template <int R, int N>
-__host__ __device__ inline double adj_interior(const double* __restrict__ lam, int i, const double* dtdx,
-                                               const double* vp, const double* vm)
+template <int R>
+__host__ __device__ inline double adj_interior(box<R> const& B, const double* __restrict__ lam, int i,
+                                               const double* dtdx, const double* vp, const double* vm)
 {
-    using S = patch_shape<R, N>;
     int c[R];
-    S::coords(i, c);
+    B.coords(i, c);
     double diag = 1.0;
     for (int d = 0; d < R; ++d) diag -= dtdx[d] * (vp[d] - vm[d]);
     double acc = diag * lam[i];
     for (int d = 0; d < R; ++d)
     {
         const int j = R - 1 - d; // layout dim of physical axis d
-        int       s = 1;
-        for (int k = R - 1; k > j; --k) s *= N;
-        if (c[j] + 1 < N) acc += (dtdx[d] * vp[d]) * lam[i + s];  // c is u_L of c + e_d
-        if (c[j] - 1 >= 0) acc -= (dtdx[d] * vm[d]) * lam[i - s]; // c is u_R of c - e_d
+        const int s = B.stride(j);
+        if (c[j] + 1 < B.n(j)) acc += (dtdx[d] * vp[d]) * lam[i + s]; // c is u_L of c + e_d
+        if (c[j] - 1 >= 0) acc -= (dtdx[d] * vm[d]) * lam[i - s];     // c is u_R of c - e_d
     }
     return acc;
+}
+
+template <int R, int N>
+__host__ __device__ inline double adj_interior(const double* __restrict__ lam, int i, const double* dtdx,
+                                               const double* vp, const double* vm)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return adj_interior<R>(B, lam, i, dtdx, vp, vm);
 }
 
 // Transposed halo term for the leaf's boundary cells facing direction dir at
@@ -74,15 +80,15 @@ __host__ __device__ inline double adj_interior(const double* __restrict__ lam, i
 // The neighbour's boundary cell reads its ghost on side dir^1 with
 // coefficient dt/dx vp (ghost is its u_L, dir positive) or -dt/dx vm (ghost
 // is its u_R, dir negative), at the neighbour's own level.
-template <int R, int N>
-__host__ __device__ inline double adj_face_term(const double* __restrict__ lam, const face_link& L, uint32_t meta,
-                                                int dir, const int* c, int level, const adjoint_params& prm)
+template <int R>
+__host__ __device__ inline double adj_face_term(box<R> const& B, const double* __restrict__ lam, const face_link& L,
+                                                uint32_t meta, int dir, const int* c, int level,
+                                                const adjoint_params& prm)
 {
-    using S        = patch_shape<R, N>;
     const int  dim = dir >> 1;
     const bool pos = (dir & 1) != 0;
     const int  d   = R - 1 - dim;
-    const int  nb  = pos ? 0 : N - 1; // the neighbour's layer that touches us
+    const int  nb  = pos ? 0 : B.n(dim) - 1; // the neighbour's layer that touches us
     switch (rel_of(meta))
     {
         case REL_SAME:
@@ -90,7 +96,7 @@ __host__ __device__ inline double adj_face_term(const double* __restrict__ lam, 
             int cc[R];
             for (int k = 0; k < R; ++k) cc[k] = (k == dim) ? nb : c[k];
             const double coef = pos ? (prm.dtdx[level][d] * prm.vp[d]) : -(prm.dtdx[level][d] * prm.vm[d]);
-            return coef * lam[static_cast<size_t>(L.ids[0]) * S::NI + S::lin(cc)];
+            return coef * lam[static_cast<size_t>(L.ids[0]) * B.ni() + B.lin(cc)];
         }
         case REL_FINER:
         {
@@ -102,13 +108,13 @@ __host__ __device__ inline double adj_face_term(const double* __restrict__ lam, 
                     base[k] = nb;
                     continue;
                 }
-                kid |= (c[k] / S::HALF) << bit;
+                kid |= (c[k] / B.half(k)) << bit;
                 ++bit;
-                base[k] = (c[k] % S::HALF) * 2;
+                base[k] = (c[k] % B.half(k)) * 2;
             }
-            const double* q   = lam + static_cast<size_t>(L.ids[kid]) * S::NI;
+            const double* q   = lam + static_cast<size_t>(L.ids[kid]) * B.ni();
             double        sum = 0.0;
-            for (int o = 0; o < S::K; ++o)
+            for (int o = 0; o < box<R>::K; ++o)
             {
                 int cc[R], b = 0;
                 for (int k = 0; k < R; ++k)
@@ -121,7 +127,7 @@ __host__ __device__ inline double adj_face_term(const double* __restrict__ lam, 
                     cc[k] = base[k] + ((o >> (R - 2 - b)) & 1);
                     ++b;
                 }
-                sum += q[S::lin(cc)];
+                sum += q[B.lin(cc)];
             }
             const double coef =
                 pos ? (prm.dtdx[level + 1][d] * prm.vp[d]) : -(prm.dtdx[level + 1][d] * prm.vm[d]);
@@ -131,13 +137,31 @@ __host__ __device__ inline double adj_face_term(const double* __restrict__ lam, 
         {
             int cc[R];
             for (int k = 0; k < R; ++k)
-                cc[k] = (k == dim) ? nb : static_cast<int>(quad_of(meta, k)) * S::HALF + c[k] / 2;
+                cc[k] = (k == dim) ? nb : static_cast<int>(quad_of(meta, k)) * B.half(k) + c[k] / 2;
             const double coef =
                 pos ? (prm.dtdx[level - 1][d] * prm.vp[d]) : -(prm.dtdx[level - 1][d] * prm.vm[d]);
-            return coef * lam[static_cast<size_t>(L.ids[0]) * S::NI + S::lin(cc)] / static_cast<double>(S::F);
+            return coef * lam[static_cast<size_t>(L.ids[0]) * B.ni() + B.lin(cc)] / static_cast<double>(box<R>::F);
         }
         default: return 0.0;
     }
+}
+
+template <int R, int N>
+__host__ __device__ inline double adj_face_term(const double* __restrict__ lam, const face_link& L, uint32_t meta,
+                                                int dir, const int* c, int level, const adjoint_params& prm)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return adj_face_term<R>(B, lam, L, meta, dir, c, level, prm);
+}
+
+// G-form contribution of a cell to lambda' . du'/dv along physical axis d,
+// from its value and its two neighbours.
+__host__ __device__ inline double adj_velocity_term(double uc, double ul, double ur, double lam_c, double dtdx,
+                                                    double sgn)
+{
+    const double gR = 0.5 * (uc + ur) - 0.5 * sgn * (ur - uc);
+    const double gL = 0.5 * (ul + uc) - 0.5 * sgn * (uc - ul);
+    return lam_c * (-dtdx * (gR - gL));
 }
 
 // Contribution of cell i to lambda' . du'/dv, from the staged u tile (with
@@ -153,27 +177,53 @@ __host__ __device__ inline void adj_velocity_cell(const double* tile, int i, dou
     const double uc = tile[t];
     for (int d = 0; d < R; ++d)
     {
-        const int    s  = T::stride(R - 1 - d);
-        const double ul = tile[t - s], ur = tile[t + s];
-        const double gR = 0.5 * (uc + ur) - 0.5 * sgn[d] * (ur - uc);
-        const double gL = 0.5 * (ul + uc) - 0.5 * sgn[d] * (uc - ul);
-        acc[d] += lam_c * (-dtdx[d] * (gR - gL));
+        const int s = T::stride(R - 1 - d);
+        acc[d] += adj_velocity_term(uc, tile[t - s], tile[t + s], lam_c, dtdx[d], sgn[d]);
+    }
+}
+
+// adj_velocity_cell with the neighbours read from memory (any patch shape).
+template <int R>
+__host__ __device__ inline void adj_velocity_cell_direct(box<R> const& B, const double* __restrict__ u, uint32_t leaf,
+                                                         const face_link* __restrict__ links,
+                                                         const uint32_t* __restrict__ metas, int i, double lam_c,
+                                                         const double* dtdx, const double* sgn, double* acc)
+{
+    int c[R];
+    B.coords(i, c);
+    const double uc = u[static_cast<size_t>(leaf) * B.ni() + i];
+    for (int d = 0; d < R; ++d)
+    {
+        const int j = R - 1 - d;
+        c[j] -= 1;
+        const double ul = cell_or_ghost<R>(B, u, leaf, links, metas, c);
+        c[j] += 2;
+        const double ur = cell_or_ghost<R>(B, u, leaf, links, metas, c);
+        c[j] -= 1;
+        acc[d] += adj_velocity_term(uc, ul, ur, lam_c, dtdx[d], sgn[d]);
     }
 }
 
 // Face index of tangential coordinates c (dims other than dim), row-major.
-template <int R, int N>
-__host__ __device__ inline int face_index(const int* c, int dim)
+template <int R>
+__host__ __device__ inline int face_index(box<R> const& B, const int* c, int dim)
 {
     int f = 0;
     for (int k = 0; k < R; ++k)
-        if (k != dim) f = f * N + c[k];
+        if (k != dim) f = f * B.n(k) + c[k];
     return f;
 }
 
-// Tangential coordinates of face index f of direction dir (c[dim] = 0).
 template <int R, int N>
-__host__ __device__ inline void face_coords(int f, int dir, int* c)
+__host__ __device__ inline int face_index(const int* c, int dim)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return face_index<R>(B, c, dim);
+}
+
+// Tangential coordinates of face index f of direction dir (c[dim] = 0).
+template <int R>
+__host__ __device__ inline void face_coords(box<R> const& B, int f, int dir, int* c)
 {
     const int dim = dir >> 1;
     for (int k = R - 1; k >= 0; --k)
@@ -183,9 +233,16 @@ __host__ __device__ inline void face_coords(int f, int dir, int* c)
             c[k] = 0;
             continue;
         }
-        c[k] = f % N;
-        f /= N;
+        c[k] = f % B.n(k);
+        f /= B.n(k);
     }
+}
+
+template <int R, int N>
+__host__ __device__ inline void face_coords(int f, int dir, int* c)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    face_coords<R>(B, f, dir, c);
 }
 
 // lambda_c = (A^T lambda')_c + transposed halo terms h[dir][face] of the
@@ -209,40 +266,75 @@ __host__ __device__ inline double adj_cell(const double* __restrict__ lam, const
     return acc;
 }
 
+// adj_cell with lambda' and the transposed halo terms read from memory (any
+// patch shape; the terms are added in adj_cell's order, so the same bits).
+template <int R>
+__host__ __device__ inline double adj_cell_direct(box<R> const& B, const double* __restrict__ lam_next, uint32_t leaf,
+                                                  const face_link* __restrict__ L, const uint32_t* __restrict__ M,
+                                                  int i, int level, const adjoint_params& prm, const double* dtdx,
+                                                  const double* vp, const double* vm)
+{
+    double acc = adj_interior<R>(B, lam_next + static_cast<size_t>(leaf) * B.ni(), i, dtdx, vp, vm);
+    int    c[R];
+    B.coords(i, c);
+    for (int dir = 0; dir < box<R>::NDIR; ++dir)
+    {
+        const int dim   = dir >> 1;
+        const int depth = (dir & 1) ? (B.n(dim) - 1 - c[dim]) : c[dim];
+        if (depth == 0 || (depth == 1 && rel_of(M[dir]) == REL_COARSER))
+            acc += adj_face_term<R>(B, lam_next, L[dir], M[dir], dir, c, level, prm);
+    }
+    return acc;
+}
+
 // Transposed regrid transfer, value of cell t of old leaf i.
 // Refined leaf: sum of lambda over the 2^R fine cells injected from t.
-template <int R, int N>
-__host__ __device__ inline double adj_refine_value(const double* __restrict__ lam_new, uint32_t first_child, int t)
+template <int R>
+__host__ __device__ inline double adj_refine_value(box<R> const& B, const double* __restrict__ lam_new,
+                                                   uint32_t first_child, int t)
 {
-    using S = patch_shape<R, N>;
     int tc[R], base[R], kid = 0;
-    S::coords(t, tc);
+    B.coords(t, tc);
     for (int d = 0; d < R; ++d)
     {
-        kid |= (tc[d] / S::HALF) << (R - 1 - d);
-        base[d] = (tc[d] % S::HALF) * 2;
+        kid |= (tc[d] / B.half(d)) << (R - 1 - d);
+        base[d] = (tc[d] % B.half(d)) * 2;
     }
-    const double* q   = lam_new + static_cast<size_t>(first_child + kid) * S::NI;
+    const double* q   = lam_new + static_cast<size_t>(first_child + kid) * B.ni();
     double        sum = 0.0;
-    for (int off = 0; off < S::F; ++off)
+    for (int off = 0; off < box<R>::F; ++off)
     {
         int cc[R];
         for (int d = 0; d < R; ++d) cc[d] = base[d] + ((off >> (R - 1 - d)) & 1);
-        sum += q[S::lin(cc)];
+        sum += q[B.lin(cc)];
     }
     return sum;
 }
 
 // Coarsened family: cell f of child kid receives lambda of the parent cell
 // that averaged it, divided by 2^R.
+template <int R>
+__host__ __device__ inline double adj_restrict_value(box<R> const& B, const double* __restrict__ lam_parent, int kid,
+                                                     int f)
+{
+    int fc[R], t[R];
+    B.coords(f, fc);
+    for (int d = 0; d < R; ++d) t[d] = ((kid >> (R - 1 - d)) & 1) * B.half(d) + fc[d] / 2;
+    return lam_parent[B.lin(t)] / static_cast<double>(box<R>::F);
+}
+
+template <int R, int N>
+__host__ __device__ inline double adj_refine_value(const double* __restrict__ lam_new, uint32_t first_child, int t)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return adj_refine_value<R>(B, lam_new, first_child, t);
+}
+
 template <int R, int N>
 __host__ __device__ inline double adj_restrict_value(const double* __restrict__ lam_parent, int kid, int f)
 {
-    using S = patch_shape<R, N>;
-    int fc[R], t[R];
-    S::coords(f, fc);
-    for (int d = 0; d < R; ++d) t[d] = ((kid >> (R - 1 - d)) & 1) * S::HALF + fc[d] / 2;
-    return lam_parent[S::lin(t)] / static_cast<double>(S::F);
+    constexpr box<R> B = box<R>::cube(N);
+    return adj_restrict_value<R>(B, lam_parent, kid, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -321,8 +413,37 @@ __global__ __launch_bounds__(NT) void k_adj_step(const double* __restrict__ lam_
         }
 }
 
-// Transposed regrid transfer, one workgroup per leaf before the regrid
-// (mark / head / counts / pos recorded by the forward regrid).
+// Transposed regrid transfer of old leaf i (one workgroup; mark / head /
+// counts / pos recorded by the forward regrid).
+template <int R, int NT>
+__device__ inline void adj_regrid_leaf(box<R> const& B, const double* __restrict__ lam_new,
+                                       const uint8_t* __restrict__ mark, const uint8_t* __restrict__ head,
+                                       const uint32_t* __restrict__ counts, const uint32_t* __restrict__ pos,
+                                       double* __restrict__ lam_old, uint32_t i)
+{
+    if (counts[i] == 0) return;
+    const int      ni  = B.ni();
+    const uint32_t dst = pos[i];
+    double*        out = lam_old + static_cast<size_t>(i) * ni;
+    if (mark[i])
+    {
+        for (int t = threadIdx.x; t < ni; t += NT) out[t] = adj_refine_value<R>(B, lam_new, dst, t);
+    }
+    else if (head[i])
+    {
+        const double* par = lam_new + static_cast<size_t>(dst) * ni;
+        for (int kid = 0; kid < box<R>::F; ++kid)
+            for (int f = threadIdx.x; f < ni; f += NT)
+                lam_old[static_cast<size_t>(i + kid) * ni + f] = adj_restrict_value<R>(B, par, kid, f);
+    }
+    else
+    {
+        const double* src = lam_new + static_cast<size_t>(dst) * ni;
+        for (int t = threadIdx.x; t < ni; t += NT) out[t] = src[t];
+    }
+}
+
+// Transposed regrid transfer, one workgroup per leaf before the regrid.
 template <int R, int N, int NT>
 __global__ __launch_bounds__(NT) void k_adj_regrid(const double* __restrict__ lam_new, const uint8_t* __restrict__ mark,
                                                    const uint8_t* __restrict__ head,
@@ -330,27 +451,78 @@ __global__ __launch_bounds__(NT) void k_adj_regrid(const double* __restrict__ la
                                                    const uint32_t* __restrict__ pos, double* __restrict__ lam_old,
                                                    uint32_t n_old)
 {
-    using S          = patch_shape<R, N>;
+    constexpr box<R> B = box<R>::cube(N);
+    const uint32_t   i = blockIdx.x;
+    if (i >= n_old) return;
+    adj_regrid_leaf<R, NT>(B, lam_new, mark, head, counts, pos, lam_old, i);
+}
+
+// k_adj_regrid for patches of any shape.
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_adj_regrid_box(box<R> B, const double* __restrict__ lam_new,
+                                                       const uint8_t* __restrict__ mark,
+                                                       const uint8_t* __restrict__ head,
+                                                       const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ pos, double* __restrict__ lam_old,
+                                                       uint32_t n_old)
+{
     const uint32_t i = blockIdx.x;
-    if (i >= n_old || counts[i] == 0) return;
-    const uint32_t dst = pos[i];
-    double*        out = lam_old + static_cast<size_t>(i) * S::NI;
-    if (mark[i])
+    if (i >= n_old) return;
+    adj_regrid_leaf<R, NT>(B, lam_new, mark, head, counts, pos, lam_old, i);
+}
+
+// k_adj_step for patches of any shape: lambda', u and the transposed halo
+// terms read from memory (the operands and sums of k_adj_step).
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_adj_step_box(box<R> B, const double* __restrict__ lam_next,
+                                                     const double* __restrict__ u, const uint64_t* __restrict__ keys,
+                                                     const face_link* __restrict__ links,
+                                                     const uint32_t* __restrict__ metas, adjoint_params prm,
+                                                     double* __restrict__ lam, double* __restrict__ vpart,
+                                                     uint32_t nleaves)
+{
+    constexpr int NW = NT / 64;
+    static_assert(NT % 64 == 0, "whole waves");
+    __shared__ double red[R][NW];
+    const uint32_t    leaf = blockIdx.x;
+    if (leaf >= nleaves) return;
+    const face_link* L  = links + static_cast<size_t>(leaf) * box<R>::NDIR;
+    const uint32_t*  M  = metas + static_cast<size_t>(leaf) * box<R>::NDIR;
+    const int        lv = key_level(keys[leaf]);
+    double           dtdx[R], vp[R], vm[R], sgn[R], acc[R];
+    for (int d = 0; d < R; ++d)
     {
-        for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = adj_refine_value<R, N>(lam_new, dst, t);
+        dtdx[d] = prm.dtdx[lv][d];
+        vp[d]   = prm.vp[d];
+        vm[d]   = prm.vm[d];
+        sgn[d]  = prm.sgn[d];
+        acc[d]  = 0.0;
     }
-    else if (head[i])
+    const int     ni  = B.ni();
+    const double* lp  = lam_next + static_cast<size_t>(leaf) * ni;
+    double*       out = lam + static_cast<size_t>(leaf) * ni;
+    for (int i = threadIdx.x; i < ni; i += NT)
     {
-        const double* par = lam_new + static_cast<size_t>(dst) * S::NI;
-        for (int kid = 0; kid < S::F; ++kid)
-            for (int f = threadIdx.x; f < S::NI; f += NT)
-                lam_old[static_cast<size_t>(i + kid) * S::NI + f] = adj_restrict_value<R, N>(par, kid, f);
+        out[i] = adj_cell_direct<R>(B, lam_next, leaf, L, M, i, lv, prm, dtdx, vp, vm);
+        adj_velocity_cell_direct<R>(B, u, leaf, links, metas, i, lp[i], dtdx, sgn, acc);
     }
-    else
-    {
-        const double* src = lam_new + static_cast<size_t>(dst) * S::NI;
-        for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = src[t];
-    }
+    // the velocity partials reduced as in k_adj_step
+    for (int d = 0; d < R; ++d)
+        for (int o = 32; o > 0; o >>= 1) acc[d] += __shfl_down(acc[d], o, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < R; ++d) red[d][threadIdx.x >> 6] = acc[d];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d)
+        {
+            double sum = 0.0;
+            if (d < R)
+            {
+                sum = red[d][0];
+                for (int w = 1; w < NW; ++w) sum += red[d][w];
+            }
+            vpart[static_cast<size_t>(leaf) * 3 + d] = sum;
+        }
 }
 
 // Deterministic sum of `count` triples: stage 1 gives each of kReduceBlocks

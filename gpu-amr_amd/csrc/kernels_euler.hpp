@@ -127,15 +127,14 @@ struct euler_flux
 // once and the source cells of every field are loaded together.  Each
 // field's arithmetic (the copy, the injection, the fine mean summed from 0.0
 // in row-major order) is ghost_value's, so the values are the same bits.
-template <int R, int N, int NV>
-__device__ inline void ghost_values(const double* __restrict__ u, size_t fstride, const face_link& L, uint32_t meta,
-                                    int dir, const int* c, double* out)
+template <int R, int NV>
+__device__ inline void ghost_values(box<R> const& B, const double* __restrict__ u, size_t fstride,
+                                    const face_link& L, uint32_t meta, int dir, const int* c, double* out)
 {
-    using S            = patch_shape<R, N>;
     const uint32_t rel = rel_of(meta);
     if (rel == REL_SAME || rel == REL_COARSER)
     {
-        const size_t at = ghost_source<R, N>(L.ids[0], meta, dir, c);
+        const size_t at = ghost_source<R>(B, L.ids[0], meta, dir, c);
         for (int v = 0; v < NV; ++v) out[v] = u[v * fstride + at];
         return;
     }
@@ -145,55 +144,62 @@ __device__ inline void ghost_values(const double* __restrict__ u, size_t fstride
         return;
     }
     const int dim = dir >> 1;
-    const int nrm = (dir & 1) ? c[dim] - N : c[dim] + N;
+    const int nrm = (dir & 1) ? c[dim] - B.n(dim) : c[dim] + B.n(dim);
     int       k = 0, bit = 0, base[R];
     for (int d = 0; d < R; ++d)
     {
         const int w = (d == dim) ? nrm : c[d];
-        base[d]     = (w * 2) % N;
+        base[d]     = (w * 2) % B.n(d);
         if (d != dim)
         {
-            k |= (c[d] / S::HALF) << bit;
+            k |= (c[d] / B.half(d)) << bit;
             ++bit;
         }
     }
-    const size_t q = static_cast<size_t>(L.ids[k]) * S::NI;
+    const size_t q = static_cast<size_t>(L.ids[k]) * B.ni();
     double       sum[NV];
     for (int v = 0; v < NV; ++v) sum[v] = 0.0;
-    for (int o = 0; o < S::F; ++o)
+    for (int o = 0; o < box<R>::F; ++o)
     {
         int cc[R];
         for (int d = 0; d < R; ++d) cc[d] = base[d] + ((o >> (R - 1 - d)) & 1);
-        const size_t at = q + S::lin(cc);
+        const size_t at = q + B.lin(cc);
         for (int v = 0; v < NV; ++v) sum[v] += u[v * fstride + at];
     }
-    for (int v = 0; v < NV; ++v) out[v] = sum[v] / static_cast<double>(S::F);
+    for (int v = 0; v < NV; ++v) out[v] = sum[v] / static_cast<double>(box<R>::F);
+}
+
+template <int R, int N, int NV>
+__device__ inline void ghost_values(const double* __restrict__ u, size_t fstride, const face_link& L, uint32_t meta,
+                                    int dir, const int* c, double* out)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    ghost_values<R, NV>(B, u, fstride, L, meta, dir, c, out);
 }
 
 // The NV values of the cell with layout coordinates c of `leaf`: interior
 // cells from the leaf, ghost cells (one coordinate just outside) through
 // the ghost rules.
-template <int R, int N>
-__device__ inline void euler_cell(const double* __restrict__ u, size_t fstride, uint32_t leaf,
+template <int R>
+__device__ inline void euler_cell(box<R> const& B, const double* __restrict__ u, size_t fstride, uint32_t leaf,
                                   const face_link* __restrict__ links, const uint32_t* __restrict__ metas, const int* c,
                                   double* U)
 {
-    using S          = patch_shape<R, N>;
     constexpr int NV = R + 2;
     int dir = -1;
     for (int d = 0; d < R; ++d)
     {
         if (c[d] < 0) dir = 2 * d;
-        if (c[d] >= N) dir = 2 * d + 1;
+        if (c[d] >= B.n(d)) dir = 2 * d + 1;
     }
     if (dir < 0)
     {
-        const size_t at = static_cast<size_t>(leaf) * S::NI + S::lin(c);
+        const size_t at = static_cast<size_t>(leaf) * B.ni() + B.lin(c);
         for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + at];
         return;
     }
-    const size_t e = static_cast<size_t>(leaf) * S::NDIR + dir;
-    ghost_values<R, N, NV>(u, fstride, links[e], metas[e], dir, c, U);
+    const size_t e = static_cast<size_t>(leaf) * box<R>::NDIR + dir;
+    ghost_values<R, NV>(B, u, fstride, links[e], metas[e], dir, c, U);
 }
 
 // Every field's (n+2)^R LDS tile of `leaf` (field v at tile + v * TS): the interior cells, then the
@@ -275,24 +281,44 @@ __device__ inline void fold_dt_min(double dt, euler_scalars* __restrict__ sc)
 // and dims of dx_d / speed (speed > 1e-12), folded into the launch-wide
 // minimum.  The minimum is exact and order-free, so any reduction order
 // gives the oracle's value.
+template <int R, int NT>
+__device__ inline void euler_dt_leaf(box<R> const& B, const double* __restrict__ u, size_t fstride,
+                                     const uint64_t* __restrict__ keys, euler_params const& prm,
+                                     euler_scalars* __restrict__ sc, uint32_t leaf)
+{
+    constexpr int NV = R + 2;
+    const int     ni = B.ni();
+    const int     lv = key_level(keys[leaf]);
+    double        dt = 1.7976931348623157e308;
+    for (int i = threadIdx.x; i < ni; i += NT)
+    {
+        double U[NV];
+        for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + static_cast<size_t>(leaf) * ni + i];
+        dt = cell_dt<R>(U, prm.cs[lv], prm.gamma, dt);
+    }
+    fold_dt_min<NT>(dt, sc);
+}
+
 template <int R, int N, int NT>
 __global__ __launch_bounds__(NT) void k_euler_dt(const double* __restrict__ u, size_t fstride,
                                                  const uint64_t* __restrict__ keys, euler_params prm,
                                                  euler_scalars* __restrict__ sc, uint32_t nleaves)
 {
-    using S          = patch_shape<R, N>;
-    constexpr int NV = R + 2;
+    constexpr box<R> B    = box<R>::cube(N);
+    const uint32_t   leaf = xcd_item(blockIdx.x, gridDim.x);
+    if (leaf >= nleaves) return;
+    euler_dt_leaf<R, NT>(B, u, fstride, keys, prm, sc, leaf);
+}
+
+// k_euler_dt for patches of any shape.
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_euler_dt_box(box<R> B, const double* __restrict__ u, size_t fstride,
+                                                     const uint64_t* __restrict__ keys, euler_params prm,
+                                                     euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
     const uint32_t leaf = xcd_item(blockIdx.x, gridDim.x);
     if (leaf >= nleaves) return;
-    const int lv = key_level(keys[leaf]);
-    double    dt = 1.7976931348623157e308;
-    for (int i = threadIdx.x; i < S::NI; i += NT)
-    {
-        double U[NV];
-        for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + static_cast<size_t>(leaf) * S::NI + i];
-        dt = cell_dt<R>(U, prm.cs[lv], prm.gamma, dt);
-    }
-    fold_dt_min<NT>(dt, sc);
+    euler_dt_leaf<R, NT>(B, u, fstride, keys, prm, sc, leaf);
 }
 
 // The CPU loop's per-step dt logic (amr_solver.hpp:225-241) on the device:
@@ -337,23 +363,21 @@ __global__ void k_euler_begin(euler_scalars* __restrict__ sc, double remaining)
 // time_step_cpu (amr_solver.hpp:265-353) for the Euler fields: per cell and
 // physical dim d, fL = F(U[c - e_d], U[c]), fR = F(U[c], U[c + e_d]),
 // total[k] -= dt / dx_d * (fR[k] - fL[k]); out = U + total.  A step the dt
-// logic stopped copies the state (the CPU loop runs no step).
-template <int R, int N, int NT>
-__global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u, double* __restrict__ un,
-                                                   size_t fstride, const uint64_t* __restrict__ keys,
-                                                   const face_link* __restrict__ links,
-                                                   const uint32_t* __restrict__ metas, euler_params prm,
-                                                   euler_scalars* __restrict__ sc, uint32_t nleaves)
+// logic stopped copies the state (the CPU loop runs no step).  Neighbours
+// are read from memory.
+template <int R, int NT>
+__device__ inline void euler_step_leaf(box<R> const& B, const double* __restrict__ u, double* __restrict__ un,
+                                       size_t fstride, const uint64_t* __restrict__ keys,
+                                       const face_link* __restrict__ links, const uint32_t* __restrict__ metas,
+                                       euler_params const& prm, euler_scalars* __restrict__ sc, uint32_t leaf)
 {
-    using S          = patch_shape<R, N>;
-    constexpr int NV = R + 2;
-    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
-    if (leaf >= nleaves) return;
-    const double dt   = sc->step_dt;
-    const size_t base = static_cast<size_t>(leaf) * S::NI;
+    constexpr int NV   = R + 2;
+    const int     ni   = B.ni();
+    const double  dt   = sc->step_dt;
+    const size_t  base = static_cast<size_t>(leaf) * ni;
     if (!(dt > 0.0))
     {
-        for (int i = threadIdx.x; i < S::NI; i += NT)
+        for (int i = threadIdx.x; i < ni; i += NT)
             for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = u[v * fstride + base + i];
         return;
     }
@@ -361,10 +385,10 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
     double    dtdx[R];
     for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
     double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
-    for (int i = threadIdx.x; i < S::NI; i += NT)
+    for (int i = threadIdx.x; i < ni; i += NT)
     {
         int c[R];
-        S::coords(i, c);
+        B.coords(i, c);
         double Uc[NV], total[NV];
         for (int v = 0; v < NV; ++v)
         {
@@ -377,9 +401,9 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
             const int j = R - 1 - d; // layout dim of physical axis d
             double    UL[NV], UR[NV], fL[NV], fR[NV];
             c[j] -= 1;
-            euler_cell<R, N>(u, fstride, leaf, links, metas, c, UL);
+            euler_cell<R>(B, u, fstride, leaf, links, metas, c, UL);
             c[j] += 2;
-            euler_cell<R, N>(u, fstride, leaf, links, metas, c, UR);
+            euler_cell<R>(B, u, fstride, leaf, links, metas, c, UR);
             c[j] -= 1;
             euler_flux<R>::rusanov(UL, euler_flux<R>::state_of(UL, prm.gamma), Uc, stc, fL, d);
             euler_flux<R>::rusanov(Uc, stc, UR, euler_flux<R>::state_of(UR, prm.gamma), fR, d);
@@ -388,12 +412,38 @@ __global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u,
         double o[NV];
         for (int v = 0; v < NV; ++v)
         {
-            o[v]                              = Uc[v] + total[v];
+            o[v]                       = Uc[v] + total[v];
             un[v * fstride + base + i] = o[v];
         }
         if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
     }
     if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
+}
+
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_euler_step(const double* __restrict__ u, double* __restrict__ un,
+                                                   size_t fstride, const uint64_t* __restrict__ keys,
+                                                   const face_link* __restrict__ links,
+                                                   const uint32_t* __restrict__ metas, euler_params prm,
+                                                   euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    constexpr box<R> B    = box<R>::cube(N);
+    const uint32_t   leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    if (leaf >= nleaves) return;
+    euler_step_leaf<R, NT>(B, u, un, fstride, keys, links, metas, prm, sc, leaf);
+}
+
+// k_euler_step for patches of any shape.
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_euler_step_box(box<R> B, const double* __restrict__ u, double* __restrict__ un,
+                                                       size_t fstride, const uint64_t* __restrict__ keys,
+                                                       const face_link* __restrict__ links,
+                                                       const uint32_t* __restrict__ metas, euler_params prm,
+                                                       euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    if (leaf >= nleaves) return;
+    euler_step_leaf<R, NT>(B, u, un, fstride, keys, links, metas, prm, sc, leaf);
 }
 
 // k_euler_step with the leaf's fields staged in LDS: every field's (n+2)^R

@@ -65,35 +65,39 @@ __device__ inline const step_params& params_for(const step_params& prm, const st
 }
 
 // Ghost value of the cell with layout coordinates c (c[dim] lies outside
-// [0, N): >= N on the positive side, < 0 on the negative side) of the leaf
-// whose direction-`dir` neighbour record is (L, meta).  Restates
+// [0, n(dim)): >= n(dim) on the positive side, < 0 on the negative side) of
+// the leaf whose direction-`dir` neighbour record is (L, meta).  Restates
 // halo_exchange_impl_t::{same_t, finer_t, coarser_t}
 // (include/ndtree/patch_utils.hpp:315-441) in compact coordinates:
 //   same    -> neighbour cell at the wrapped normal coordinate;
 //   finer   -> mean of the 2^R fine cells, summed from 0.0 in row-major order;
 //   coarser -> injection of the coarse cell under the contact quadrant.
-template <int R, int N>
-__host__ __device__ inline double ghost_value(const double* __restrict__ u, const face_link& L, uint32_t meta, int dir,
-                                              const int* c)
+// The fine cells' base coordinate along every dim d is (2 v) mod n(d), as
+// the reference's device twin takes it (src/cuda/halo_exchange.cu:174-176).
+// Its CPU rule reduces every dim mod n(dim) of the face normal
+// (patch_utils.hpp:358), the same for square patches only; for other shapes
+// that rule addresses cells outside the fine patch's interior.
+template <int R>
+__host__ __device__ inline double ghost_value(box<R> const& B, const double* __restrict__ u, const face_link& L,
+                                              uint32_t meta, int dir, const int* c)
 {
-    using S        = patch_shape<R, N>;
     const int  dim = dir >> 1;
     const bool pos = (dir & 1) != 0;
-    const int  nrm = pos ? c[dim] - N : c[dim] + N; // reference's from_idxs[dim]
+    const int  nrm = pos ? c[dim] - B.n(dim) : c[dim] + B.n(dim); // reference's from_idxs[dim]
     switch (rel_of(meta))
     {
         case REL_SAME:
         {
             int cc[R];
             for (int d = 0; d < R; ++d) cc[d] = (d == dim) ? nrm : c[d];
-            return u[static_cast<size_t>(L.ids[0]) * S::NI + S::lin(cc)];
+            return u[static_cast<size_t>(L.ids[0]) * B.ni() + B.lin(cc)];
         }
         case REL_COARSER:
         {
             int cc[R];
             for (int d = 0; d < R; ++d)
-                cc[d] = static_cast<int>(quad_of(meta, d)) * S::HALF + ((d == dim) ? nrm : c[d]) / 2;
-            return u[static_cast<size_t>(L.ids[0]) * S::NI + S::lin(cc)];
+                cc[d] = static_cast<int>(quad_of(meta, d)) * B.half(d) + ((d == dim) ? nrm : c[d]) / 2;
+            return u[static_cast<size_t>(L.ids[0]) * B.ni() + B.lin(cc)];
         }
         case REL_FINER:
         {
@@ -101,25 +105,33 @@ __host__ __device__ inline double ghost_value(const double* __restrict__ u, cons
             for (int d = 0; d < R; ++d)
             {
                 const int v = (d == dim) ? nrm : c[d];
-                base[d]     = (v * 2) % N;
+                base[d]     = (v * 2) % B.n(d);
                 if (d != dim)
                 {
-                    k |= (c[d] / S::HALF) << bit;
+                    k |= (c[d] / B.half(d)) << bit;
                     ++bit;
                 }
             }
-            const double* q   = u + static_cast<size_t>(L.ids[k]) * S::NI;
+            const double* q   = u + static_cast<size_t>(L.ids[k]) * B.ni();
             double        sum = 0.0;
-            for (int o = 0; o < S::F; ++o)
+            for (int o = 0; o < box<R>::F; ++o)
             {
                 int cc[R];
                 for (int d = 0; d < R; ++d) cc[d] = base[d] + ((o >> (R - 1 - d)) & 1);
-                sum += q[S::lin(cc)];
+                sum += q[B.lin(cc)];
             }
-            return sum / static_cast<double>(S::F);
+            return sum / static_cast<double>(box<R>::F);
         }
         default: return 0.0;
     }
+}
+
+template <int R, int N>
+__host__ __device__ inline double ghost_value(const double* __restrict__ u, const face_link& L, uint32_t meta, int dir,
+                                              const int* c)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return ghost_value<R>(B, u, L, meta, dir, c);
 }
 
 // LDS tile of the patch plus one ghost layer: (N+2)^R doubles.
@@ -183,20 +195,43 @@ __host__ __device__ inline void stage_ghost(double* tile, const double* __restri
 // Index into u of the one interior cell that gives the ghost value of cell c
 // when the direction-`dir` neighbour (slot nb) is same-level (copy) or
 // coarser (injection): ghost_value's single-cell rules.
-template <int R, int N>
-__host__ __device__ inline size_t ghost_source(int32_t nb, uint32_t meta, int dir, const int* c)
+template <int R>
+__host__ __device__ inline size_t ghost_source(box<R> const& B, int32_t nb, uint32_t meta, int dir, const int* c)
 {
-    using S           = patch_shape<R, N>;
     const int  dim    = dir >> 1;
-    const int  nrm    = (dir & 1) ? c[dim] - N : c[dim] + N;
+    const int  nrm    = (dir & 1) ? c[dim] - B.n(dim) : c[dim] + B.n(dim);
     const bool coarse = rel_of(meta) == REL_COARSER;
     int        cc[R];
     for (int d = 0; d < R; ++d)
     {
         const int v = (d == dim) ? nrm : c[d];
-        cc[d]       = coarse ? static_cast<int>(quad_of(meta, d)) * S::HALF + v / 2 : v;
+        cc[d]       = coarse ? static_cast<int>(quad_of(meta, d)) * B.half(d) + v / 2 : v;
     }
-    return static_cast<size_t>(nb) * S::NI + S::lin(cc);
+    return static_cast<size_t>(nb) * B.ni() + B.lin(cc);
+}
+
+template <int R, int N>
+__host__ __device__ inline size_t ghost_source(int32_t nb, uint32_t meta, int dir, const int* c)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return ghost_source<R>(B, nb, meta, dir, c);
+}
+
+// Value of cell c of `leaf` (layout coordinates, at most one of them outside
+// the patch): the interior cell, or the ghost value by the face rules.
+template <int R>
+__host__ __device__ inline double cell_or_ghost(box<R> const& B, const double* __restrict__ u, uint32_t leaf,
+                                                const face_link* __restrict__ links,
+                                                const uint32_t* __restrict__ metas, const int* c)
+{
+    for (int d = 0; d < R; ++d)
+        if (c[d] < 0 || c[d] >= B.n(d))
+        {
+            const int    dir = 2 * d + (c[d] >= B.n(d) ? 1 : 0);
+            const size_t e   = static_cast<size_t>(leaf) * box<R>::NDIR + dir;
+            return ghost_value<R>(B, u, links[e], metas[e], dir, c);
+        }
+    return u[static_cast<size_t>(leaf) * B.ni() + B.lin(c)];
 }
 
 // A leaf's face records: relation/quadrant word and first neighbour slot per
@@ -417,35 +452,89 @@ __host__ __device__ inline double gradient_cell(const double* tile, int i)
     return g;
 }
 
-// Cell i of the padded view ((N+2H)^R, row-major): interior copy, face halo
-// gathered; *kind = 0 interior, 1 face halo, 2 edge/corner (value 0).
-template <int R, int N, int H>
-__host__ __device__ inline double padded_value(const double* __restrict__ u, uint32_t leaf,
+// Cell i of the padded view ((n(d) + 2h) per dim, row-major): interior copy,
+// face halo gathered; *kind = 0 interior, 1 face halo, 2 edge/corner (value 0).
+template <int R>
+__host__ __device__ inline double padded_value(box<R> const& B, int h, const double* __restrict__ u, uint32_t leaf,
                                                const face_link* __restrict__ links,
                                                const uint32_t* __restrict__ metas, int i, int* kind)
 {
-    using S         = patch_shape<R, N>;
-    constexpr int P = N + 2 * H;
     int c[R], r = i, outside = 0, odim = 0;
     for (int d = R - 1; d >= 0; --d)
     {
-        c[d] = r % P - H;
-        r /= P;
-        if (c[d] < 0 || c[d] >= N)
+        const int p = B.n(d) + 2 * h;
+        c[d]        = r % p - h;
+        r /= p;
+        if (c[d] < 0 || c[d] >= B.n(d))
         {
             ++outside;
             odim = d;
         }
     }
     *kind = outside < 2 ? outside : 2;
-    if (outside == 0) return u[static_cast<size_t>(leaf) * S::NI + S::lin(c)];
+    if (outside == 0) return u[static_cast<size_t>(leaf) * B.ni() + B.lin(c)];
     if (outside == 1)
     {
-        const int    dir = 2 * odim + (c[odim] >= N ? 1 : 0);
-        const size_t e   = static_cast<size_t>(leaf) * S::NDIR + dir;
-        return ghost_value<R, N>(u, links[e], metas[e], dir, c);
+        const int    dir = 2 * odim + (c[odim] >= B.n(odim) ? 1 : 0);
+        const size_t e   = static_cast<size_t>(leaf) * box<R>::NDIR + dir;
+        return ghost_value<R>(B, u, links[e], metas[e], dir, c);
     }
     return 0.0;
+}
+
+template <int R, int N, int H>
+__host__ __device__ inline double padded_value(const double* __restrict__ u, uint32_t leaf,
+                                               const face_link* __restrict__ links,
+                                               const uint32_t* __restrict__ metas, int i, int* kind)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return padded_value<R>(B, H, u, leaf, links, metas, i, kind);
+}
+
+// The upwind update of interior cell i of `leaf` with its neighbours read
+// from memory (interior cells, or ghost values at the patch boundary): the
+// same operands as upwind_cell on a staged tile, for patches of any shape.
+template <int R>
+__host__ __device__ inline double upwind_cell_direct(box<R> const& B, const double* __restrict__ u, uint32_t leaf,
+                                                     const face_link* __restrict__ links,
+                                                     const uint32_t* __restrict__ metas, int i, const double* dtdx,
+                                                     const double* vel, const double* hs)
+{
+    int c[R];
+    B.coords(i, c);
+    double lo[R], hi[R];
+    for (int d = 0; d < R; ++d)
+    {
+        const int j = R - 1 - d; // layout dim of physical axis d
+        c[j] -= 1;
+        lo[d] = cell_or_ghost<R>(B, u, leaf, links, metas, c);
+        c[j] += 2;
+        hi[d] = cell_or_ghost<R>(B, u, leaf, links, metas, c);
+        c[j] -= 1;
+    }
+    return upwind_update<R>(u[static_cast<size_t>(leaf) * B.ni() + i], lo, hi, dtdx, vel, hs);
+}
+
+// gradient_cell with the neighbours read from memory (any patch shape).
+template <int R>
+__host__ __device__ inline double gradient_cell_direct(box<R> const& B, const double* __restrict__ u, uint32_t leaf,
+                                                       const face_link* __restrict__ links,
+                                                       const uint32_t* __restrict__ metas, int i)
+{
+    int c[R];
+    B.coords(i, c);
+    const double uc = u[static_cast<size_t>(leaf) * B.ni() + i];
+    double       g  = 0.0;
+    for (int j = 0; j < R; ++j)
+    {
+        c[j] += 1;
+        const double up = cell_or_ghost<R>(B, u, leaf, links, metas, c);
+        c[j] -= 2;
+        const double um = cell_or_ghost<R>(B, u, leaf, links, metas, c);
+        c[j] += 1;
+        g = fmax(g, fmax(fabs(up - uc), fabs(uc - um)));
+    }
+    return g;
 }
 
 __host__ __device__ inline int8_t decide(double value, int level, double rthr, double cthr, int min_level,
@@ -852,6 +941,64 @@ __global__ __launch_bounds__(NT) void k_criterion_gradient(const double* __restr
     if (threadIdx.x == 0) flags[leaf] = decide(g, key_level(keys[leaf]), rthr, cthr, min_level, max_level);
 }
 
+// Face-halo cells of a patch with h ghost layers: h * face(dim) items per
+// direction, direction after direction.
+template <int R>
+__host__ __device__ inline int halo_items(box<R> const& B, int h)
+{
+    int items = 0;
+    for (int dir = 0; dir < box<R>::NDIR; ++dir) items += h * B.face(dir >> 1);
+    return items;
+}
+
+// Halo item g (0 <= g < halo_items): its direction (returned) and its cell c
+// in compact coordinates (layer `layer` beyond the face, tangential cells in
+// row-major order).
+template <int R>
+__host__ __device__ inline int halo_item(box<R> const& B, int h, int g, int* c)
+{
+    int dir = 0, r = g;
+    while (r >= h * B.face(dir >> 1))
+    {
+        r -= h * B.face(dir >> 1);
+        ++dir;
+    }
+    const int dim = dir >> 1, fc = B.face(dim), layer = r / fc;
+    r %= fc;
+    for (int d = R - 1; d >= 0; --d)
+    {
+        if (d == dim) continue;
+        c[d] = r % B.n(d);
+        r /= B.n(d);
+    }
+    c[dim] = (dir & 1) ? B.n(dim) + layer : -1 - layer;
+    return dir;
+}
+
+// Maximum over one leaf's interior and its h-deep face halos, starting from
+// 0, reduced over the workgroup (every thread must call it): the interior
+// with coalesced loads, then the face halos by the ghost rules (padded_value's
+// cells of kind 0 and 1; the maximum is order-free).
+template <int R, int NT>
+__device__ inline double leaf_max_value(box<R> const& B, int h, const double* __restrict__ u, uint32_t leaf,
+                                        const face_link* __restrict__ links, const uint32_t* __restrict__ metas,
+                                        double* red)
+{
+    double        m  = 0.0;
+    const int     ni = B.ni();
+    const double* up = u + static_cast<size_t>(leaf) * ni;
+    for (int i = threadIdx.x; i < ni; i += NT) m = fmax(m, up[i]);
+    const int items = halo_items<R>(B, h);
+    for (int g = threadIdx.x; g < items; g += NT)
+    {
+        int            c[R];
+        const int      dir = halo_item<R>(B, h, g, c);
+        const size_t   e   = static_cast<size_t>(leaf) * box<R>::NDIR + dir;
+        m                  = fmax(m, ghost_value<R>(B, u, links[e], metas[e], dir, c));
+    }
+    return block_max<NT>(m, red);
+}
+
 // Maximum-value criterion over the interior and the H-deep face halos,
 // starting from 0.
 template <int R, int N, int H, int NT>
@@ -862,32 +1009,90 @@ __global__ __launch_bounds__(NT) void k_criterion_max(const double* __restrict__
                                                       int min_level, int max_level, int8_t* __restrict__ flags,
                                                       uint32_t nleaves)
 {
-    using S             = patch_shape<R, N>;
-    constexpr int     NH = S::NDIR * H * S::FACE; // face-halo cells
+    constexpr box<R>  B = box<R>::cube(N);
     __shared__ double red[NT];
     const uint32_t    leaf = blockIdx.x;
     if (leaf >= nleaves) return;
-    // the interior with coalesced loads, then the face halos by the ghost
-    // rules (padded_value's cells of kind 0 and 1; the maximum is
-    // order-free)
-    double        m  = 0.0;
-    const double* up = u + static_cast<size_t>(leaf) * S::NI;
-    for (int i = threadIdx.x; i < S::NI; i += NT) m = fmax(m, up[i]);
-    for (int g = threadIdx.x; g < NH; g += NT)
+    const double m = leaf_max_value<R, NT>(B, H, u, leaf, links, metas, red);
+    if (threadIdx.x == 0) flags[leaf] = decide(m, key_level(keys[leaf]), rthr, cthr, min_level, max_level);
+}
+
+// ---------------------------------------------------------------------------
+// Kernels for patches of any shape (the box is a launch argument): one
+// workgroup per leaf, neighbour cells read from memory -- the operands and
+// expressions of the fixed-shape kernels, so the results are the same bits.
+// ---------------------------------------------------------------------------
+
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_step_box(box<R> B, const double* __restrict__ u, double* __restrict__ un,
+                                                 const uint64_t* __restrict__ keys,
+                                                 const face_link* __restrict__ links,
+                                                 const uint32_t* __restrict__ metas, step_params prm,
+                                                 const step_params* __restrict__ table,
+                                                 const uint16_t* __restrict__ inst, uint32_t nleaves)
+{
+    const uint32_t leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    if (leaf >= nleaves) return;
+    const step_params& P  = params_for(prm, table, inst, leaf);
+    const int          lv = key_level(keys[leaf]);
+    double             dtdx[R], vel[R], hs[R];
+    for (int d = 0; d < R; ++d)
     {
-        const int dir = g / (H * S::FACE), layer = (g / S::FACE) % H, dim = dir >> 1;
-        int       c[R], r = g % S::FACE;
-        for (int d = R - 1; d >= 0; --d)
-        {
-            if (d == dim) continue;
-            c[d] = r % N;
-            r /= N;
-        }
-        c[dim]         = (dir & 1) ? N + layer : -1 - layer;
-        const size_t e = static_cast<size_t>(leaf) * S::NDIR + dir;
-        m              = fmax(m, ghost_value<R, N>(u, links[e], metas[e], dir, c));
+        dtdx[d] = P.dtdx[lv][d];
+        vel[d]  = P.vel[d];
+        hs[d]   = P.hs[d];
     }
-    m = block_max<NT>(m, red);
+    const int ni  = B.ni();
+    double*   out = un + static_cast<size_t>(leaf) * ni;
+    for (int i = threadIdx.x; i < ni; i += NT)
+        out[i] = upwind_cell_direct<R>(B, u, leaf, links, metas, i, dtdx, vel, hs);
+}
+
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_padded_box(box<R> B, int h, const double* __restrict__ u,
+                                                   double* __restrict__ pad, const face_link* __restrict__ links,
+                                                   const uint32_t* __restrict__ metas, uint32_t nleaves)
+{
+    const uint32_t leaf = blockIdx.x;
+    if (leaf >= nleaves) return;
+    const int flat = B.padded(h);
+    for (int i = threadIdx.x; i < flat; i += NT)
+    {
+        int kind;
+        pad[static_cast<size_t>(leaf) * flat + i] = padded_value<R>(B, h, u, leaf, links, metas, i, &kind);
+    }
+}
+
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_criterion_gradient_box(box<R> B, const double* __restrict__ u,
+                                                               const uint64_t* __restrict__ keys,
+                                                               const face_link* __restrict__ links,
+                                                               const uint32_t* __restrict__ metas, double rthr,
+                                                               double cthr, int min_level, int max_level,
+                                                               int8_t* __restrict__ flags, uint32_t nleaves)
+{
+    __shared__ double red[NT];
+    const uint32_t    leaf = blockIdx.x;
+    if (leaf >= nleaves) return;
+    double    g  = 0.0;
+    const int ni = B.ni();
+    for (int i = threadIdx.x; i < ni; i += NT) g = fmax(g, gradient_cell_direct<R>(B, u, leaf, links, metas, i));
+    g = block_max<NT>(g, red);
+    if (threadIdx.x == 0) flags[leaf] = decide(g, key_level(keys[leaf]), rthr, cthr, min_level, max_level);
+}
+
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_criterion_max_box(box<R> B, int h, const double* __restrict__ u,
+                                                          const uint64_t* __restrict__ keys,
+                                                          const face_link* __restrict__ links,
+                                                          const uint32_t* __restrict__ metas, double rthr, double cthr,
+                                                          int min_level, int max_level, int8_t* __restrict__ flags,
+                                                          uint32_t nleaves)
+{
+    __shared__ double red[NT];
+    const uint32_t    leaf = blockIdx.x;
+    if (leaf >= nleaves) return;
+    const double m = leaf_max_value<R, NT>(B, h, u, leaf, links, metas, red);
     if (threadIdx.x == 0) flags[leaf] = decide(m, key_level(keys[leaf]), rthr, cthr, min_level, max_level);
 }
 

@@ -250,39 +250,51 @@ __host__ __device__ inline uint64_t emit_key(uint64_t key, int depth, bool refin
 
 // Cell t of child o of a refined leaf: injection (ndtree.hpp:1463-1497,
 // 1528-1555).
+template <int R>
+__host__ __device__ inline double refine_value(box<R> const& B, const double* __restrict__ src, int o, int t)
+{
+    int tc[R], sc[R];
+    B.coords(t, tc);
+    for (int d = 0; d < R; ++d) sc[d] = ((o >> (R - 1 - d)) & 1) * B.half(d) + tc[d] / 2;
+    return src[B.lin(sc)];
+}
+
 template <int R, int N>
 __host__ __device__ inline double refine_value(const double* __restrict__ src, int o, int t)
 {
-    using S = patch_shape<R, N>;
-    int tc[R], sc[R];
-    S::coords(t, tc);
-    for (int d = 0; d < R; ++d) sc[d] = ((o >> (R - 1 - d)) & 1) * S::HALF + tc[d] / 2;
-    return src[S::lin(sc)];
+    constexpr box<R> B = box<R>::cube(N);
+    return refine_value<R>(B, src, o, t);
 }
 
 // Cell t of the parent of the family whose first child is slot i: mean of
 // 2^R child cells summed from 0.0 in row-major order (ndtree.hpp:1499-1526,
 // intergrid_operator.hpp:91-106).
-template <int R, int N>
-__host__ __device__ inline double restrict_value(const double* __restrict__ u, uint32_t i, int t)
+template <int R>
+__host__ __device__ inline double restrict_value(box<R> const& B, const double* __restrict__ u, uint32_t i, int t)
 {
-    using S = patch_shape<R, N>;
     int tc[R], base[R], kid = 0;
-    S::coords(t, tc);
+    B.coords(t, tc);
     for (int d = 0; d < R; ++d)
     {
-        kid |= (tc[d] / S::HALF) << (R - 1 - d);
-        base[d] = (tc[d] % S::HALF) * 2;
+        kid |= (tc[d] / B.half(d)) << (R - 1 - d);
+        base[d] = (tc[d] % B.half(d)) * 2;
     }
-    const double* q   = u + static_cast<size_t>(i + kid) * S::NI;
+    const double* q   = u + static_cast<size_t>(i + kid) * B.ni();
     double        sum = 0.0;
-    for (int off = 0; off < S::F; ++off)
+    for (int off = 0; off < box<R>::F; ++off)
     {
         int cc[R];
         for (int d = 0; d < R; ++d) cc[d] = base[d] + ((off >> (R - 1 - d)) & 1);
-        sum += q[S::lin(cc)];
+        sum += q[B.lin(cc)];
     }
-    return sum / static_cast<double>(S::F);
+    return sum / static_cast<double>(box<R>::F);
+}
+
+template <int R, int N>
+__host__ __device__ inline double restrict_value(const double* __restrict__ u, uint32_t i, int t)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    return restrict_value<R>(B, u, i, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -528,18 +540,17 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_
     if (i < n) out[i] += block_sums[i / kScanBlock];
 }
 
-// New keys and interiors, one workgroup per old leaf.
-template <int R, int N, int NT>
-__global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ keys, const double* __restrict__ u,
-                                             uint32_t n, int depth, const uint8_t* __restrict__ mark,
-                                             const uint8_t* __restrict__ head, const uint32_t* __restrict__ counts,
-                                             const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys_out,
-                                             double* __restrict__ u_out, regrid_summary* __restrict__ summary,
-                                             int nfields, size_t fstride)
+// New keys and interiors of old leaf i (one workgroup; every thread calls
+// it): its copy, its 2^R injected children or its family's restriction.
+template <int R, int NT>
+__device__ inline void emit_leaf(box<R> const& B, const uint64_t* __restrict__ keys, const double* __restrict__ u,
+                                 uint32_t i, int depth, const uint8_t* __restrict__ mark,
+                                 const uint8_t* __restrict__ head, const uint32_t* __restrict__ counts,
+                                 const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys_out,
+                                 double* __restrict__ u_out, regrid_summary* __restrict__ summary, int nfields,
+                                 size_t fstride)
 {
-    using S          = patch_shape<R, N>;
-    const uint32_t i = xcd_item(blockIdx.x, gridDim.x);
-    if (i >= n) return;
+    const int      ni  = B.ni();
     const uint32_t cnt = counts[i];
     if (cnt == 0) return;
     const uint64_t key    = keys[i];
@@ -552,20 +563,20 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ keys, 
         for (int f = 0; f < nfields; ++f)
         {
             const double* uf  = u + f * fstride;
-            double*       out = u_out + f * fstride + static_cast<size_t>(dst + j) * S::NI;
+            double*       out = u_out + f * fstride + static_cast<size_t>(dst + j) * ni;
             if (refine)
             {
-                const double* src = uf + static_cast<size_t>(i) * S::NI;
-                for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = refine_value<R, N>(src, static_cast<int>(j), t);
+                const double* src = uf + static_cast<size_t>(i) * ni;
+                for (int t = threadIdx.x; t < ni; t += NT) out[t] = refine_value<R>(B, src, static_cast<int>(j), t);
             }
             else if (fam)
             {
-                for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = restrict_value<R, N>(uf, i, t);
+                for (int t = threadIdx.x; t < ni; t += NT) out[t] = restrict_value<R>(B, uf, i, t);
             }
             else
             {
-                const double* src = uf + static_cast<size_t>(i) * S::NI;
-                for (int t = threadIdx.x; t < S::NI; t += NT) out[t] = src[t];
+                const double* src = uf + static_cast<size_t>(i) * ni;
+                for (int t = threadIdx.x; t < ni; t += NT) out[t] = src[t];
             }
         }
         if (threadIdx.x == 0)
@@ -578,6 +589,36 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ keys, 
             if (lv > *reinterpret_cast<volatile uint32_t*>(&summary->max_level)) atomicMax(&summary->max_level, lv);
         }
     }
+}
+
+// New keys and interiors, one workgroup per old leaf.
+template <int R, int N, int NT>
+__global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ keys, const double* __restrict__ u,
+                                             uint32_t n, int depth, const uint8_t* __restrict__ mark,
+                                             const uint8_t* __restrict__ head, const uint32_t* __restrict__ counts,
+                                             const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys_out,
+                                             double* __restrict__ u_out, regrid_summary* __restrict__ summary,
+                                             int nfields, size_t fstride)
+{
+    constexpr box<R> B = box<R>::cube(N);
+    const uint32_t   i = xcd_item(blockIdx.x, gridDim.x);
+    if (i >= n) return;
+    emit_leaf<R, NT>(B, keys, u, i, depth, mark, head, counts, pos, keys_out, u_out, summary, nfields, fstride);
+}
+
+// k_emit for patches of any shape.
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void k_emit_box(box<R> B, const uint64_t* __restrict__ keys,
+                                                 const double* __restrict__ u, uint32_t n, int depth,
+                                                 const uint8_t* __restrict__ mark, const uint8_t* __restrict__ head,
+                                                 const uint32_t* __restrict__ counts,
+                                                 const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys_out,
+                                                 double* __restrict__ u_out, regrid_summary* __restrict__ summary,
+                                                 int nfields, size_t fstride)
+{
+    const uint32_t i = xcd_item(blockIdx.x, gridDim.x);
+    if (i >= n) return;
+    emit_leaf<R, NT>(B, keys, u, i, depth, mark, head, counts, pos, keys_out, u_out, summary, nfields, fstride);
 }
 
 } // namespace amrhip

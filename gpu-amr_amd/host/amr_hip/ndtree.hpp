@@ -123,21 +123,62 @@ struct device_criterion
     }
 };
 
-// ndtree<Rank, Depth, N, Halo>: a periodic tree of N^Rank-cell leaf patches
-// with Halo ghost layers in the padded view.
-template <unsigned Rank, unsigned Depth, unsigned N, unsigned Halo>
-class ndtree
+// static_shape<N, M[, O]> (include/containers/static_shape.hpp,
+// include/containers/static_layout.hpp:28-37): interior cells per layout
+// dim, row-major, the last dim is physical x.
+template <unsigned... Ns>
+struct static_shape
+{
+    static constexpr unsigned                            rank = sizeof...(Ns);
+    static constexpr std::array<unsigned, sizeof...(Ns)> sizes{ Ns... };
+    static constexpr std::size_t                         elements = (std::size_t{ 1 } * ... * Ns);
+    static_assert(rank == 2 || rank == 3, "2D or 3D patches");
+};
+
+namespace detail
+{
+template <unsigned Rank, unsigned N>
+struct cube;
+template <unsigned N>
+struct cube<2, N>
+{
+    using type = static_shape<N, N>;
+};
+template <unsigned N>
+struct cube<3, N>
+{
+    using type = static_shape<N, N, N>;
+};
+} // namespace detail
+
+// basic_ndtree<Depth, Shape, Halo>: a periodic tree of leaf patches of
+// Shape (a static_shape) with Halo ghost layers in the padded view.
+template <unsigned Depth, typename Shape, unsigned Halo>
+class basic_ndtree
 {
 public:
+    static constexpr unsigned Rank = Shape::rank;
+    using shape_t         = Shape;
     using patch_index_t   = morton_id<Depth, Rank>;
     using linear_index_t  = std::size_t;
     using refine_status_t = amr::hip::refine_status_t;
-    static constexpr unsigned s_rank        = Rank;
-    static constexpr unsigned s_padded      = N + 2 * Halo;
-    static constexpr std::size_t s_flat     = Rank == 2 ? s_padded * s_padded : s_padded * s_padded * s_padded;
-    static constexpr std::size_t s_interior = Rank == 2 ? N * N : N * N * N;
+    static constexpr unsigned    s_rank     = Rank;
+    static constexpr std::size_t s_interior = Shape::elements;
+    // padded extent per layout dim
+    static constexpr std::array<unsigned, Rank> s_padded = []
+    {
+        std::array<unsigned, Rank> p{};
+        for (unsigned d = 0; d < Rank; ++d) p[d] = Shape::sizes[d] + 2 * Halo;
+        return p;
+    }();
+    static constexpr std::size_t s_flat = []
+    {
+        std::size_t f = 1;
+        for (unsigned d : Shape::sizes) f *= d + 2 * Halo;
+        return f;
+    }();
 
-    explicit ndtree(std::size_t capacity, std::array<double, 3> lengths = { 1.0, 1.0, 1.0 })
+    explicit basic_ndtree(std::size_t capacity, std::array<double, 3> lengths = { 1.0, 1.0, 1.0 })
     {
         const auto cfg = config(capacity, lengths);
         check(amr_tree_create(&cfg, &m_ctx), "ndtree");
@@ -150,7 +191,7 @@ public:
     {
         double gamma, cfl;
     };
-    ndtree(std::size_t capacity, std::array<double, 3> lengths, euler_fields e)
+    basic_ndtree(std::size_t capacity, std::array<double, 3> lengths, euler_fields e)
     {
         const auto cfg = config(capacity, lengths);
         check(amr_euler_create(&cfg, e.gamma, e.cfl, &m_ctx), "ndtree (Euler)");
@@ -158,12 +199,12 @@ public:
         m_fields  = Rank + 2;
         refresh_keys();
     }
-    ~ndtree()
+    ~basic_ndtree()
     {
         if (m_ctx) amr_tree_destroy(m_ctx);
     }
-    ndtree(ndtree const&)            = delete;
-    ndtree& operator=(ndtree const&) = delete;
+    basic_ndtree(basic_ndtree const&)            = delete;
+    basic_ndtree& operator=(basic_ndtree const&) = delete;
 
     amr_ctx* handle() const noexcept { return m_ctx; }
     unsigned fields() const noexcept { return m_fields; }
@@ -202,7 +243,7 @@ public:
         m_view_valid = false;
     }
 
-    // Padded patch ((N+2*Halo)^Rank doubles, halos exchanged, corners 0) of
+    // Padded patch (prod (n_d + 2 Halo) doubles, halos exchanged, corners 0) of
     // field `field` (get_patch<Field>, field index in FieldTags order) by
     // linear index or by patch index (ndtree.hpp:477-512).  The padded view
     // is downloaded once and reused until the state changes.
@@ -250,7 +291,8 @@ public:
         std::array<double, Rank> r{};
         const double             pm   = static_cast<double>(1u << (Depth - k.level()));
         const double             maxc = static_cast<double>(1u << Depth);
-        for (unsigned i = 0; i < Rank; ++i) r[i] = (m_lengths[i] * pm / maxc) / static_cast<double>(N);
+        for (unsigned i = 0; i < Rank; ++i)
+            r[i] = (m_lengths[i] * pm / maxc) / static_cast<double>(Shape::sizes[Rank - 1 - i]);
         return r;
     }
     // Centre of interior cell (row-major interior index ci) of leaf k.
@@ -264,8 +306,8 @@ public:
         std::array<std::size_t, Rank> idx{};
         for (int d = static_cast<int>(Rank) - 1; d >= 0; --d)
         {
-            idx[static_cast<std::size_t>(d)] = rem % N;
-            rem /= N;
+            idx[static_cast<std::size_t>(d)] = rem % Shape::sizes[static_cast<std::size_t>(d)];
+            rem /= Shape::sizes[static_cast<std::size_t>(d)];
         }
         for (unsigned i = 0; i < Rank; ++i)
         {
@@ -282,7 +324,7 @@ private:
         amr_tree_config cfg{};
         cfg.rank       = static_cast<int32_t>(Rank);
         cfg.max_depth  = static_cast<int32_t>(Depth);
-        cfg.patch_size = static_cast<int32_t>(N);
+        for (unsigned d = 0; d < Rank; ++d) cfg.patch_extents[d] = static_cast<int32_t>(Shape::sizes[d]);
         cfg.halo_width = static_cast<int32_t>(Halo);
         cfg.capacity   = capacity;
         for (int i = 0; i < 3; ++i) cfg.domain_length[i] = lengths[i];
@@ -315,6 +357,10 @@ private:
     mutable std::vector<double> m_view;
     mutable bool                m_view_valid = false;
 };
+
+// ndtree<Rank, Depth, N, Halo>: square / cubic N^Rank-cell patches.
+template <unsigned Rank, unsigned Depth, unsigned N, unsigned Halo>
+using ndtree = basic_ndtree<Depth, typename detail::cube<Rank, N>::type, Halo>;
 
 // amr_solver<Tree> for linear advection with a run-time velocity.
 template <typename Tree>

@@ -47,17 +47,34 @@ enum amr_status
 typedef struct amr_ctx amr_ctx;
 
 /* Template parameters of ndtree<Cell, morton_id<Depth, Rank>,
- * patch_layout<static_layout<n...>, Halo>, linear_interpolator> and
- * physics_system<..., Domain_Sizes> (include/ndtree/ndtree.hpp:94-99,
- * include/solver/physics_system.hpp:106-110), as run-time values. */
+ * patch_layout<static_layout<static_shape<N, M[, O]>>, Halo>,
+ * linear_interpolator> and physics_system<..., Domain_Sizes>
+ * (include/ndtree/ndtree.hpp:94-99, include/containers/static_layout.hpp:28-37,
+ * include/solver/physics_system.hpp:13-17), as run-time values.
+ *
+ * The patch shape is patch_extents[0..rank-1] = N, M(, O): the interior
+ * cells of layout dims 0..rank-1, the last being physical x; or, with all
+ * three extents 0, patch_size cells in every dim.  Extents must be even,
+ * within 2..4096 and at least 2 * halo_width (AMR_ERR_ARGUMENT otherwise).
+ * Any such shape runs; cubic 16^2, 8^3, 10^2 and 50^2 (halo 2), 64^2, 4^2,
+ * 8^2, 4^3 have kernels specialised for their size.
+ *
+ * Face ghosts from finer neighbours take the fine cells at (2 v) mod n(d)
+ * along every dim d, as the reference's device path does
+ * (src/cuda/halo_exchange.cu:174-176); its CPU path reduces every dim mod
+ * the face normal's extent (include/ndtree/patch_utils.hpp:358), which
+ * agrees for square patches only and otherwise addresses cells outside the
+ * fine patch. */
 typedef struct amr_tree_config
 {
     int32_t  rank;             /* 2 or 3                                        */
     int32_t  max_depth;        /* Morton depth (7 in 2D, 5 in 3D in the configs) */
-    int32_t  patch_size;       /* interior cells per dimension                  */
-    int32_t  halo_width;       /* ghost layers of the padded view (1 or 2)      */
+    int32_t  patch_size;       /* interior cells per dimension (square / cubic)  */
+    int32_t  halo_width;       /* ghost layers of the padded view (1..8)        */
     uint64_t capacity;         /* maximum number of leaf patches                */
     double   domain_length[3]; /* physical lengths of x, y, z                   */
+    int32_t  patch_extents[3]; /* N, M(, O) per layout dim; all 0: patch_size   */
+    int32_t  reserved;         /* 0                                             */
 } amr_tree_config;
 
 /* One (leaf, direction) neighbour record in the reference's flattened form

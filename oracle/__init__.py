@@ -52,7 +52,7 @@ def load(path=None):
     )
     lib.oracle_last_error.restype = ctypes.c_char_p
     lib.oracle_create.restype = vp
-    lib.oracle_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, dp]
+    lib.oracle_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, sz, dp]
     lib.oracle_destroy.argtypes = [vp]
     lib.oracle_size.restype = sz
     lib.oracle_size.argtypes = [vp]
@@ -68,11 +68,12 @@ def load(path=None):
     lib.oracle_compute_dt.argtypes = [vp]
     lib.oracle_step.argtypes = [vp, ctypes.c_double]
     lib.oracle_step_kernel_only.argtypes = [vp, ctypes.c_double]
+    lib.oracle_swap_buffers.argtypes = [vp]
     lib.oracle_advance_batch.argtypes = [vp, sz, ctypes.c_double, dp, ctypes.POINTER(sz)]
     lib.oracle_criterion.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, i8p]
     # Euler system
     lib.oracle_euler_create.restype = vp
-    lib.oracle_euler_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, dp]
+    lib.oracle_euler_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, sz, dp]
     lib.oracle_euler_destroy.argtypes = [vp]
     lib.oracle_euler_size.restype = sz
     lib.oracle_euler_size.argtypes = [vp]
@@ -101,7 +102,24 @@ def _ptr(a, ctype):
     return a.ctypes.data_as(ctypes.POINTER(ctype))
 
 
-class OracleTree:
+class _Shaped:
+    """Patch shape of an oracle instance: n is the extent of every dim (int)
+    or the extents of layout dims 0..rank-1 (static_shape<N, M[, O]>, the
+    last is physical x)."""
+
+    def _shape(self, n):
+        shape = (int(n),) * self.rank if isinstance(n, (int, np.integer)) else tuple(int(x) for x in n)
+        assert len(shape) == self.rank
+        self.shape = shape
+        self.n = shape[0] if len(set(shape)) == 1 else shape
+        self.ni = int(np.prod(shape))
+        self.pad_shape = tuple(x + 2 * self.halo for x in shape)
+        self.pad = self.pad_shape[0] if len(set(self.pad_shape)) == 1 else self.pad_shape
+        self.flat = int(np.prod(self.pad_shape))
+        self._ext = (ctypes.c_int * 3)(*(list(shape) + [1] * (3 - len(shape))))
+
+
+class OracleTree(_Shaped):
     """The reference's ndtree + amr_solver (advection) restated on the CPU."""
 
     CRIT_MAX_VALUE = 0
@@ -109,12 +127,10 @@ class OracleTree:
 
     def __init__(self, rank, n, halo, depth, capacity, lengths=(1.0, 1.0, 1.0)):
         self.lib = load()
-        self.rank, self.n, self.halo, self.depth = rank, n, halo, depth
-        self.ni = n**rank
-        self.pad = n + 2 * halo
-        self.flat = self.pad**rank
+        self.rank, self.halo, self.depth = rank, halo, depth
+        self._shape(n)
         L = np.ascontiguousarray(np.array(list(lengths) + [1.0] * (3 - len(lengths)), dtype=np.float64)[:3])
-        self.h = self.lib.oracle_create(rank, n, halo, depth, capacity, _ptr(L, ctypes.c_double))
+        self.h = self.lib.oracle_create(rank, self._ext, halo, depth, capacity, _ptr(L, ctypes.c_double))
         if not self.h:
             raise RuntimeError(self.lib.oracle_last_error().decode())
 
@@ -167,7 +183,7 @@ class OracleTree:
     def get_interior(self):
         out = np.zeros(self.size() * self.ni, dtype=np.float64)
         self._check(self.lib.oracle_get_interior(self.h, _ptr(out, ctypes.c_double)))
-        return out.reshape(self.size(), *([self.n] * self.rank))
+        return out.reshape(self.size(), *self.shape)
 
     def set_padded(self, v):
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
@@ -177,7 +193,7 @@ class OracleTree:
     def get_padded(self):
         out = np.zeros(self.size() * self.flat, dtype=np.float64)
         self._check(self.lib.oracle_get_padded(self.h, _ptr(out, ctypes.c_double)))
-        return out.reshape(self.size(), *([self.pad] * self.rank))
+        return out.reshape(self.size(), *self.pad_shape)
 
     def cell_centers(self):
         out = np.zeros(self.size() * self.ni * self.rank, dtype=np.float64)
@@ -201,6 +217,10 @@ class OracleTree:
     def step_kernel_only(self, dt):
         self._check(self.lib.oracle_step_kernel_only(self.h, float(dt)))
 
+    def swap_buffers(self):
+        """ndtree::swap_buffers: the current and next state exchange roles."""
+        self._check(self.lib.oracle_swap_buffers(self.h))
+
     def advance_batch(self, steps, remaining=np.finfo(np.float64).max):
         s = ctypes.c_double(0.0)
         ex = ctypes.c_size_t(0)
@@ -216,20 +236,18 @@ class OracleTree:
         return out
 
 
-class OracleEuler:
+class OracleEuler(_Shaped):
     """The reference's ndtree + amr_solver<..., EulerPhysics<R>, R> restated on
     the CPU (EulerPhysics.hpp, amr_solver.hpp:105-413).  Host arrays are
     field-major: all leaves of rho, then rho*u, rho*v, (rho*w), E."""
 
     def __init__(self, rank, n, halo, depth, capacity, lengths=(1.0, 1.0, 1.0)):
         self.lib = load()
-        self.rank, self.n, self.halo, self.depth = rank, n, halo, depth
+        self.rank, self.halo, self.depth = rank, halo, depth
         self.nv = rank + 2
-        self.ni = n**rank
-        self.pad = n + 2 * halo
-        self.flat = self.pad**rank
+        self._shape(n)
         L = np.ascontiguousarray(np.array(list(lengths) + [1.0] * (3 - len(lengths)), dtype=np.float64)[:3])
-        self.h = self.lib.oracle_euler_create(rank, n, halo, depth, capacity, _ptr(L, ctypes.c_double))
+        self.h = self.lib.oracle_euler_create(rank, self._ext, halo, depth, capacity, _ptr(L, ctypes.c_double))
         if not self.h:
             raise RuntimeError(self.lib.oracle_last_error().decode())
 
@@ -274,12 +292,12 @@ class OracleEuler:
     def get_interior(self):
         out = np.zeros(self.nv * self.size() * self.ni, dtype=np.float64)
         self._check(self.lib.oracle_euler_get_interior(self.h, _ptr(out, ctypes.c_double)))
-        return out.reshape(self.nv, self.size(), *([self.n] * self.rank))
+        return out.reshape(self.nv, self.size(), *self.shape)
 
     def get_padded(self):
         out = np.zeros(self.nv * self.size() * self.flat, dtype=np.float64)
         self._check(self.lib.oracle_euler_get_padded(self.h, _ptr(out, ctypes.c_double)))
-        return out.reshape(self.nv, self.size(), *([self.pad] * self.rank))
+        return out.reshape(self.nv, self.size(), *self.pad_shape)
 
     def cell_centers(self):
         out = np.zeros(self.size() * self.ni * self.rank, dtype=np.float64)

@@ -201,8 +201,8 @@ inline void parallel_for(std::size_t n, int threads, Fn&& fn)
 }
 
 // ---------------------------------------------------------------------------
-// The tree + solver restatement.  R = rank, N = interior cells per dimension
-// (square / cubic patches), H = halo width.  Layout is row-major with the last
+// The tree + solver restatement.  R = rank, H = halo width, Ns = interior
+// cells of layout dims 0..R-1 (static_shape<N, M[, O]>).  Layout is row-major with the last
 // index fastest (include/containers/static_layout.hpp:28-37); physical x is the
 // last layout index (include/solver/physics_system.hpp:126-135,215-224).
 // ---------------------------------------------------------------------------
@@ -216,17 +216,19 @@ inline double advection_flux(double ul, double ur, double v)
     return 0.5 * (fl + fr) - 0.5 * smax * (ur - ul);
 }
 
-template <int R, int N, int H>
+template <int R, int H, int... Ns>
 class tree
 {
 public:
-    static constexpr int P    = N + 2 * H;                       // padded extent
-    static constexpr int FLAT = (R == 2) ? P * P : P * P * P;    // padded cells
-    static constexpr int NI   = (R == 2) ? N * N : N * N * N;    // interior cells
+    static_assert(sizeof...(Ns) == R, "one extent per layout dim");
+    static constexpr std::array<int, R> NS{ Ns... };             // interior cells per layout dim
+    static constexpr std::array<int, R> PS{ (Ns + 2 * H)... };   // padded extents
+    static constexpr int FLAT = ((Ns + 2 * H) * ...);            // padded cells
+    static constexpr int NI   = (Ns * ...);                      // interior cells
     static constexpr int F    = 1 << R;                          // nd fanout
     static constexpr int K    = 1 << (R - 1);                    // face children
     static constexpr int NDIR = 2 * R;
-    static_assert(N % 2 == 0 && N >= 2 * H, "halo must not span more than one finer cell");
+    static_assert(((Ns % 2 == 0 && Ns >= 2 * H) && ...), "even extents; a halo must not span more than one finer cell");
 
     using idx_t      = std::array<int, R>;
     using neighbor_t = neighbor<R>;
@@ -236,7 +238,7 @@ public:
     {
         std::array<int, R> s{};
         s[R - 1] = 1;
-        for (int d = R - 2; d >= 0; --d) s[d] = s[d + 1] * P;
+        for (int d = R - 2; d >= 0; --d) s[d] = s[d + 1] * PS[d + 1];
         return s;
     }
     static constexpr std::array<int, R> S = strides();
@@ -270,8 +272,11 @@ public:
     static void interior_for(Fn&& fn)
     {
         idx_t lo, hi;
-        lo.fill(H);
-        hi.fill(H + N);
+        for (int d = 0; d < R; ++d)
+        {
+            lo[d] = H;
+            hi[d] = H + NS[d];
+        }
         box_for(lo, hi, fn);
     }
 
@@ -405,7 +410,7 @@ public:
     {
         auto ps = patch_sizes(k);
         std::array<double, R> r{};
-        for (int i = 0; i < R; ++i) r[i] = ps[i] / static_cast<double>(N);
+        for (int i = 0; i < R; ++i) r[i] = ps[i] / static_cast<double>(NS[R - 1 - i]); // physical axis i
         return r;
     }
     std::array<double, R> patch_coord(key_t k) const
@@ -633,7 +638,7 @@ public:
                 idx_t fine_patch{}, base{};
                 for (int d = 0; d < R; ++d)
                 {
-                    const int section = N / 2;
+                    const int section = NS[d] / 2;
                     fine_patch[d]     = (idxs[d] - H) / section;
                     base[d]           = ((idxs[d] - H) % section) * 2 + H;
                 }
@@ -936,8 +941,8 @@ public:
             idx_t lo, hi;
             for (int d = 0; d < R; ++d)
             {
-                lo[d] = (d != dim) ? H : (pos ? P - H : 0);
-                hi[d] = (d != dim) ? P - H : (pos ? P : H);
+                lo[d] = (d != dim) ? H : (pos ? PS[d] - H : 0);
+                hi[d] = (d != dim) ? PS[d] - H : (pos ? PS[d] : H);
             }
             neighbor_t const& n = m_nbrs[i][k];
             switch (n.kind)
@@ -952,7 +957,7 @@ public:
                         [&](idx_t const& idxs)
                         {
                             idx_t from = idxs;
-                            from[dim]  = pos ? from[dim] - N : from[dim] + N;
+                            from[dim]  = pos ? from[dim] - NS[dim] : from[dim] + NS[dim];
                             self[lin(idxs)] = other[lin(from)];
                         }
                     );
@@ -969,15 +974,19 @@ public:
                         {
                             int   fine_patch = 0, stride = 1;
                             idx_t base = idxs;
-                            base[dim]  = pos ? base[dim] - N : base[dim] + N;
+                            base[dim]  = pos ? base[dim] - NS[dim] : base[dim] + NS[dim];
                             for (int d = 0; d < R; ++d)
                             {
-                                // the reference reduces modulo the normal
-                                // dimension's size for every d (:358)
-                                base[d] = ((base[d] - H) * 2) % N + H;
+                                // each dim reduced modulo its own extent, as
+                                // the device twin does
+                                // (src/cuda/halo_exchange.cu:174-176); the
+                                // CPU rule takes the normal dim's extent for
+                                // every d (:358), the same for square
+                                // patches only
+                                base[d] = ((base[d] - H) * 2) % NS[d] + H;
                                 if (d != dim)
                                 {
-                                    const int section = N / 2;
+                                    const int section = NS[d] / 2;
                                     fine_patch += ((idxs[d] - H) / section) * stride;
                                     stride *= 2;
                                 }
@@ -1009,11 +1018,11 @@ public:
                         [&](idx_t const& idxs)
                         {
                             idx_t from = idxs;
-                            from[dim]  = pos ? from[dim] - N : from[dim] + N;
+                            from[dim]  = pos ? from[dim] - NS[dim] : from[dim] + NS[dim];
                             for (int d = 0; d < R; ++d)
                             {
                                 const int fine_mapped = from[d] - H;
-                                from[d] = H + n.quadrant[d] * (N / 2) + fine_mapped / 2;
+                                from[d] = H + n.quadrant[d] * (NS[d] / 2) + fine_mapped / 2;
                             }
                             self[lin(idxs)] = other[lin(from)];
                         }
@@ -1131,14 +1140,13 @@ public:
         {
             double const* u  = &m_cur[p * FLAT];
             double        mx = 0.0;
-            idx_t         lo{}, hi;
-            hi.fill(P);
+            idx_t         lo{}, hi = PS;
             box_for(
                 lo, hi,
                 [&](idx_t const& idxs)
                 {
                     int outside = 0;
-                    for (int d = 0; d < R; ++d) outside += (idxs[d] < H || idxs[d] >= H + N) ? 1 : 0;
+                    for (int d = 0; d < R; ++d) outside += (idxs[d] < H || idxs[d] >= H + NS[d]) ? 1 : 0;
                     if (outside <= 1) mx = std::max(mx, u[lin(idxs)]);
                 }
             );
@@ -1281,11 +1289,11 @@ struct euler_physics
     }
 };
 
-template <int R, int N, int H>
+template <int R, int H, int... Ns>
 class euler_system
 {
   public:
-    using tree_t               = tree<R, N, H>;
+    using tree_t               = tree<R, H, Ns...>;
     using phys                 = euler_physics<R>;
     static constexpr int NV    = phys::NV;
     static constexpr int FLAT  = tree_t::FLAT;

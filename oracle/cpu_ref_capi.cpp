@@ -32,14 +32,15 @@ struct iface
     virtual double      dt() const                                        = 0;
     virtual void        step(double dt)                                   = 0;
     virtual void        step_kernel_only(double dt)                       = 0;
+    virtual void        swap()                                            = 0;
     virtual double      advance(std::size_t steps, double rem, std::size_t* ex) = 0;
     virtual void        criterion(int kind, double r, double c, int mn, int mx, std::int8_t* f) const = 0;
 };
 
-template <int R, int N, int H>
+template <int R, int H, int... Ns>
 struct impl final : iface
 {
-    using tree_t = oracle::tree<R, N, H>;
+    using tree_t = oracle::tree<R, H, Ns...>;
     tree_t t;
 
     impl(int depth, std::size_t cap, std::array<double, 3> L)
@@ -95,6 +96,7 @@ struct impl final : iface
     }
     double dt() const override { return t.compute_time_step(); }
     void   step(double dt) override { t.time_step(dt); }
+    void   swap() override { t.swap_buffers(); }
     void   step_kernel_only(double dt) override
     {
         t.time_step_kernel(dt);
@@ -110,20 +112,43 @@ struct impl final : iface
     }
 };
 
-iface* make(int rank, int n, int h, int depth, std::size_t cap, std::array<double, 3> L)
+// Whether (rank, ext, h) is the instance <R, H, Ns...>.
+template <int R, int H, int... Ns>
+bool is(int rank, int const* ext, int h)
 {
-#define ORACLE_CASE(R_, N_, H_) \
-    if (rank == R_ && n == N_ && h == H_) return new impl<R_, N_, H_>(depth, cap, L);
-    ORACLE_CASE(2, 4, 1)
-    ORACLE_CASE(2, 4, 2)
-    ORACLE_CASE(2, 8, 1)
-    ORACLE_CASE(2, 16, 1)
-    ORACLE_CASE(2, 10, 2)
-    ORACLE_CASE(2, 50, 2)
-    ORACLE_CASE(3, 4, 1)
-    ORACLE_CASE(3, 8, 1)
+    constexpr int ns[] = { Ns... };
+    if (rank != R || h != H) return false;
+    for (int d = 0; d < R; ++d)
+        if (ext[d] != ns[d]) return false;
+    return true;
+}
+
+// Instances: the square / cubic shapes of the configurations and the
+// reference's tests and benchmarks, small shapes for exhaustive tests, and
+// non-square shapes (static_shape<N, M[, O]>) in both orientations.
+#define ORACLE_SHAPES(X) \
+    X(2, 1, 4, 4)        \
+    X(2, 2, 4, 4)        \
+    X(2, 1, 6, 6)        \
+    X(2, 1, 8, 8)        \
+    X(2, 1, 16, 16)      \
+    X(2, 1, 64, 64)      \
+    X(2, 2, 10, 10)      \
+    X(2, 2, 50, 50)      \
+    X(2, 1, 16, 8)       \
+    X(2, 1, 8, 16)       \
+    X(2, 2, 12, 8)       \
+    X(3, 1, 4, 4, 4)     \
+    X(3, 1, 8, 8, 8)     \
+    X(3, 1, 4, 6, 8)
+
+iface* make(int rank, int const* ext, int h, int depth, std::size_t cap, std::array<double, 3> L)
+{
+#define ORACLE_CASE(R_, H_, ...) \
+    if (is<R_, H_, __VA_ARGS__>(rank, ext, h)) return new impl<R_, H_, __VA_ARGS__>(depth, cap, L);
+    ORACLE_SHAPES(ORACLE_CASE)
 #undef ORACLE_CASE
-    throw std::invalid_argument("no oracle instance for this (rank, patch size, halo)");
+    throw std::invalid_argument("no oracle instance for this (rank, patch extents, halo)");
 }
 
 // The Euler system (several fields over one topology).
@@ -149,10 +174,10 @@ struct eiface
 };
 
 // Field-major host arrays: all leaves of field 0, then field 1, ...
-template <int R, int N, int H>
+template <int R, int H, int... Ns>
 struct eimpl final : eiface
 {
-    using sys_t  = oracle::euler_system<R, N, H>;
+    using sys_t  = oracle::euler_system<R, H, Ns...>;
     using tree_t = typename sys_t::tree_t;
     sys_t e;
 
@@ -208,20 +233,13 @@ struct eimpl final : eiface
     }
 };
 
-eiface* make_euler(int rank, int n, int h, int depth, std::size_t cap, std::array<double, 3> L)
+eiface* make_euler(int rank, int const* ext, int h, int depth, std::size_t cap, std::array<double, 3> L)
 {
-#define ORACLE_CASE(R_, N_, H_) \
-    if (rank == R_ && n == N_ && h == H_) return new eimpl<R_, N_, H_>(depth, cap, L);
-    ORACLE_CASE(2, 4, 1)
-    ORACLE_CASE(2, 8, 1)
-    ORACLE_CASE(2, 16, 1)
-    ORACLE_CASE(2, 64, 1)
-    ORACLE_CASE(2, 10, 2)
-    ORACLE_CASE(2, 50, 2)
-    ORACLE_CASE(3, 4, 1)
-    ORACLE_CASE(3, 8, 1)
+#define ORACLE_CASE(R_, H_, ...) \
+    if (is<R_, H_, __VA_ARGS__>(rank, ext, h)) return new eimpl<R_, H_, __VA_ARGS__>(depth, cap, L);
+    ORACLE_SHAPES(ORACLE_CASE)
 #undef ORACLE_CASE
-    throw std::invalid_argument("no Euler oracle instance for this (rank, patch size, halo)");
+    throw std::invalid_argument("no Euler oracle instance for this (rank, patch extents, halo)");
 }
 
 template <typename Fn>
@@ -246,10 +264,10 @@ extern "C"
 
 const char* oracle_last_error(void) { return g_err.c_str(); }
 
-void* oracle_create(int rank, int n, int h, int depth, std::size_t capacity, double const* lengths)
+void* oracle_create(int rank, int const* ext, int h, int depth, std::size_t capacity, double const* lengths)
 {
     iface* p = nullptr;
-    guarded([&] { p = make(rank, n, h, depth, capacity, { lengths[0], lengths[1], lengths[2] }); });
+    guarded([&] { p = make(rank, ext, h, depth, capacity, { lengths[0], lengths[1], lengths[2] }); });
     return p;
 }
 
@@ -316,6 +334,11 @@ int    oracle_step(void* o, double dt)
 {
     return guarded([&] { static_cast<iface*>(o)->step(dt); });
 }
+// ndtree::swap_buffers (ndtree.hpp:1558-1579): current <-> next.
+int oracle_swap_buffers(void* o)
+{
+    return guarded([&] { static_cast<iface*>(o)->swap(); });
+}
 int oracle_step_kernel_only(void* o, double dt)
 {
     return guarded([&] { static_cast<iface*>(o)->step_kernel_only(dt); });
@@ -331,10 +354,10 @@ int oracle_criterion(void* o, int kind, double r, double c, int mn, int mx, std:
 
 // ---- Euler system -------------------------------------------------------
 
-void* oracle_euler_create(int rank, int n, int h, int depth, std::size_t capacity, double const* lengths)
+void* oracle_euler_create(int rank, int const* ext, int h, int depth, std::size_t capacity, double const* lengths)
 {
     eiface* p = nullptr;
-    guarded([&] { p = make_euler(rank, n, h, depth, capacity, { lengths[0], lengths[1], lengths[2] }); });
+    guarded([&] { p = make_euler(rank, ext, h, depth, capacity, { lengths[0], lengths[1], lengths[2] }); });
     return p;
 }
 void        oracle_euler_destroy(void* o) { delete static_cast<eiface*>(o); }

@@ -20,6 +20,16 @@ import numpy as np
 from oracle import HALO_META_DTYPE
 
 
+def extents(rank, n):
+    """Interior cells per layout dim: n (every dim) or a sequence of rank
+    extents (static_shape<N, M[, O]>, the last is physical x)."""
+    if isinstance(n, (int, np.integer)):
+        return (int(n),) * rank
+    ext = tuple(int(x) for x in n)
+    assert len(ext) == rank
+    return ext
+
+
 def interleave(coords):
     m = 0
     r = len(coords)
@@ -164,27 +174,27 @@ def expected_halo(keys, interior, rank, n, h, depth):
     L = Leaves(keys, rank, depth)
     M, D = L.M, depth
     nl = len(L.keys)
-    P = n + 2 * h
-    out = np.full((nl,) + (P,) * rank, np.nan)
-    interior = np.asarray(interior).reshape((nl,) + (n,) * rank)
-    G = M * n  # level-`depth` cells per axis
+    ext = extents(rank, n)
+    na = [ext[rank - 1 - a] for a in range(rank)]  # cells per patch along physical axis a
+    out = np.full((nl,) + tuple(x + 2 * h for x in ext), np.nan)
+    interior = np.asarray(interior).reshape((nl,) + ext)
     for i in range(nl):
-        out[(i,) + (slice(h, h + n),) * rank] = interior[i]
+        out[(i,) + tuple(slice(h, h + x) for x in ext)] = interior[i]
         c, lv = L.dec[i]
         cs = 1 << (D - lv)
-        for idx in itertools.product(range(P), repeat=rank):
-            halo_dims = [d for d in range(rank) if idx[d] < h or idx[d] >= h + n]
+        for idx in itertools.product(*[range(x + 2 * h) for x in ext]):
+            halo_dims = [d for d in range(rank) if idx[d] < h or idx[d] >= h + ext[d]]
             if len(halo_dims) != 1:
                 continue
             cg = [x - h for x in idx]  # compact layout coordinates
-            O = [0] * rank  # ghost origin per physical axis, level-depth cell units
+            O = [0] * rank  # ghost origin per physical axis, level-`depth` cell units
             for a in range(rank):
-                O[a] = (c[a] * n + cg[rank - 1 - a] * cs) % G
-            j = L.find([o // n for o in O])
+                O[a] = (c[a] * na[a] + cg[rank - 1 - a] * cs) % (M * na[a])
+            j = L.find([O[a] // na[a] for a in range(rank)])
             cj, lq = L.dec[j]
             fs = 1 << (D - lq)
             assert abs(lq - lv) <= 1
-            loc = [(O[a] - cj[a] * n) // fs for a in range(rank)]
+            loc = [(O[a] - cj[a] * na[a]) // fs for a in range(rank)]
             lay = [loc[rank - 1 - jd] for jd in range(rank)]
             if lq <= lv:
                 out[(i,) + tuple(idx)] = interior[j][tuple(lay)]
@@ -267,9 +277,10 @@ def expected_transfer(old_keys, old_interior, new_keys, rank, n, depth):
     """Interiors after a regrid: copy / injection / row-major mean."""
     old = Leaves(old_keys, rank, depth)
     new = Leaves(new_keys, rank, depth)
-    old_int = np.asarray(old_interior).reshape((len(old.keys),) + (n,) * rank)
-    out = np.zeros((len(new.keys),) + (n,) * rank)
-    h2 = n // 2
+    ext = extents(rank, n)
+    old_int = np.asarray(old_interior).reshape((len(old.keys),) + ext)
+    out = np.zeros((len(new.keys),) + ext)
+    h2 = [x // 2 for x in ext]
     for i, key in enumerate(new.keys):
         c, lv = new.dec[i]
         j = old.index.get((lv, c))
@@ -281,17 +292,17 @@ def expected_transfer(old_keys, old_interior, new_keys, rank, n, depth):
         j = old.index.get((lv - 1, pc)) if lv > 0 else None
         if j is not None:  # injected child
             pos = [(c[a] - pc[a]) // s for a in range(rank)]
-            for t in itertools.product(range(n), repeat=rank):
-                src = tuple(pos[rank - 1 - jd] * h2 + t[jd] // 2 for jd in range(rank))
+            for t in itertools.product(*[range(x) for x in ext]):
+                src = tuple(pos[rank - 1 - jd] * h2[jd] + t[jd] // 2 for jd in range(rank))
                 out[(i,) + t] = old_int[j][src]
             continue
         kids = {}
         for o in range(1 << rank):
             cc = tuple(c[a] + ((o >> a) & 1) * (s // 2) for a in range(rank))
             kids[tuple((o >> (rank - 1 - jd)) & 1 for jd in range(rank))] = old.index[(lv + 1, cc)]
-        for t in itertools.product(range(n), repeat=rank):
-            kid = kids[tuple(t[jd] // h2 for jd in range(rank))]
-            base = [(t[jd] % h2) * 2 for jd in range(rank)]
+        for t in itertools.product(*[range(x) for x in ext]):
+            kid = kids[tuple(t[jd] // h2[jd] for jd in range(rank))]
+            base = [(t[jd] % h2[jd]) * 2 for jd in range(rank)]
             tot = 0.0
             for off in itertools.product(range(2), repeat=rank):
                 tot += old_int[kid][tuple(base[d] + off[d] for d in range(rank))]
@@ -307,21 +318,23 @@ def dense_from_uniform(keys, interior, rank, n, depth):
     assert all(d[1] == lv for d in L.dec)
     s = 1 << (depth - lv)
     nb = 1 << lv
-    out = np.zeros((nb * n,) * rank)
-    interior = np.asarray(interior).reshape((len(L.keys),) + (n,) * rank)
+    ext = extents(rank, n)
+    out = np.zeros(tuple(nb * x for x in ext))
+    interior = np.asarray(interior).reshape((len(L.keys),) + ext)
     for i, (c, _) in enumerate(L.dec):
         blk = [c[rank - 1 - jd] // s for jd in range(rank)]
-        out[tuple(slice(b * n, (b + 1) * n) for b in blk)] = interior[i]
+        out[tuple(slice(b * x, (b + 1) * x) for b, x in zip(blk, ext))] = interior[i]
     return out
 
 
 def split_uniform(keys, dense, rank, n, depth):
     L = Leaves(keys, rank, depth)
     s = 1 << (depth - L.dec[0][1])
-    out = np.zeros((len(L.keys),) + (n,) * rank)
+    ext = extents(rank, n)
+    out = np.zeros((len(L.keys),) + ext)
     for i, (c, _) in enumerate(L.dec):
         blk = [c[rank - 1 - jd] // s for jd in range(rank)]
-        out[i] = dense[tuple(slice(b * n, (b + 1) * n) for b in blk)]
+        out[i] = dense[tuple(slice(b * x, (b + 1) * x) for b, x in zip(blk, ext))]
     return out
 
 
@@ -360,25 +373,26 @@ def criterion_values(padded, rank, n, h, kind, start=0.0):
     """
     pad = np.asarray(padded, dtype=np.float64)
     nl = pad.shape[0]
+    ext = extents(rank, n)
     out = np.full(nl, start, dtype=np.float64)
-    inner = slice(h, h + n)
+    inner = [slice(h, h + x) for x in ext]
     if kind == 0:
         for i in range(nl):
             p = pad[i]
-            vals = [p[(inner,) * rank].ravel()]
+            vals = [p[tuple(inner)].ravel()]
             for d in range(rank):
-                for band in (slice(0, h), slice(h + n, n + 2 * h)):
-                    sl = tuple(band if e == d else inner for e in range(rank))
+                for band in (slice(0, h), slice(h + ext[d], ext[d] + 2 * h)):
+                    sl = tuple(band if e == d else inner[e] for e in range(rank))
                     vals.append(p[sl].ravel())
             out[i] = max(start, float(np.max(np.concatenate(vals))))
         return out
     for i in range(nl):
         p = pad[i]
-        u = p[(inner,) * rank]
+        u = p[tuple(inner)]
         g = start
         for d in range(rank):
-            up = p[tuple(slice(h + 1, h + n + 1) if e == d else inner for e in range(rank))]
-            um = p[tuple(slice(h - 1, h + n - 1) if e == d else inner for e in range(rank))]
+            up = p[tuple(slice(h + 1, h + ext[d] + 1) if e == d else inner[e] for e in range(rank))]
+            um = p[tuple(slice(h - 1, h + ext[d] - 1) if e == d else inner[e] for e in range(rank))]
             g = max(g, float(np.max(np.abs(up - u))), float(np.max(np.abs(u - um))))
         out[i] = g
     return out

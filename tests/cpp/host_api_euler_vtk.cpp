@@ -75,7 +75,7 @@ int main()
         const auto        rho = tree.get_patch(std::size_t{ 0 }, 0);
         const auto        e   = tree.get_patch(std::size_t{ 0 }, 4);
         constexpr auto    P   = tree_t::s_padded;
-        const std::size_t mid = (P / 2 * P + P / 2) * P + P / 2; // an interior cell
+        const std::size_t mid = (P[0] / 2 * P[1] + P[1] / 2) * P[2] + P[2] / 2; // an interior cell
         expect(rho.size() == tree_t::s_flat && rho[mid] > 0.0 && e[mid] > rho[mid], "per-field padded patches");
         amr::hip::vtk_print vtk("host_api_euler");
         vtk.print(tree, ".vtk");

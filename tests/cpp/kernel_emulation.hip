@@ -210,6 +210,167 @@ void adj_regrid(const double* lam_new, const uint8_t* mark, const uint8_t* head,
     }
 }
 
+// ---- the any-shape (box) kernels' bodies, serialised ----------------------
+
+template <int R>
+box<R> box_of(const int* ext)
+{
+    return box<R>{ { ext[0], ext[1], R == 3 ? ext[2] : 1 } };
+}
+
+template <int R>
+void step_box(const int* ext, const uint64_t* keys, uint32_t n, const face_link* links, const uint32_t* metas,
+              const double* u, double* un, const double* dtdx_tab, const double* vel, const double* hs)
+{
+    const box<R> B = box_of<R>(ext);
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+    {
+        const int lv = key_level(keys[leaf]);
+        double    dtdx[3];
+        for (int d = 0; d < R; ++d) dtdx[d] = dtdx_tab[lv * 3 + d];
+        for (int i = 0; i < B.ni(); ++i)
+            un[static_cast<size_t>(leaf) * B.ni() + i] = upwind_cell_direct<R>(B, u, leaf, links, metas, i, dtdx, vel, hs);
+    }
+}
+
+template <int R>
+void padded_box(const int* ext, int h, uint32_t n, const face_link* links, const uint32_t* metas, const double* u,
+                double* pad)
+{
+    const box<R> B    = box_of<R>(ext);
+    const int    flat = B.padded(h);
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+        for (int i = 0; i < flat; ++i)
+        {
+            int kind;
+            pad[static_cast<size_t>(leaf) * flat + i] = padded_value<R>(B, h, u, leaf, links, metas, i, &kind);
+        }
+}
+
+// k_criterion_gradient_box / k_criterion_max_box (leaf_max_value's interior
+// and halo items).
+template <int R>
+void flags_box(const int* ext, int h, int kind, const uint64_t* keys, uint32_t n, const face_link* links,
+               const uint32_t* metas, const double* u, double rthr, double cthr, int mn, int mx, int8_t* out)
+{
+    const box<R> B = box_of<R>(ext);
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+    {
+        double v = 0.0;
+        if (kind == 1)
+            for (int i = 0; i < B.ni(); ++i) v = fmax(v, gradient_cell_direct<R>(B, u, leaf, links, metas, i));
+        else
+        {
+            for (int i = 0; i < B.ni(); ++i) v = fmax(v, u[static_cast<size_t>(leaf) * B.ni() + i]);
+            for (int g = 0; g < halo_items<R>(B, h); ++g)
+            {
+                int          c[R];
+                const int    dir = halo_item<R>(B, h, g, c);
+                const size_t e   = static_cast<size_t>(leaf) * box<R>::NDIR + dir;
+                v                = fmax(v, ghost_value<R>(B, u, links[e], metas[e], dir, c));
+            }
+        }
+        out[leaf] = decide(v, key_level(keys[leaf]), rthr, cthr, mn, mx);
+    }
+}
+
+// The transfer of emit_leaf for one regrid plan (maps rows: mark, head,
+// counts, pos).
+template <int R>
+void transfer_box(const int* ext, int depth, const uint64_t* keys, uint32_t n, const uint32_t* maps, const double* u,
+                  uint64_t* keys_out, double* u_out)
+{
+    const box<R>    B      = box_of<R>(ext);
+    const int       ni     = B.ni();
+    const uint32_t* mark   = maps;
+    const uint32_t* head   = maps + n;
+    const uint32_t* counts = maps + 2 * n;
+    const uint32_t* pos    = maps + 3 * n;
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < counts[i]; ++j)
+        {
+            double* out = u_out + static_cast<size_t>(pos[i] + j) * ni;
+            for (int t = 0; t < ni; ++t)
+            {
+                if (mark[i])
+                    out[t] = refine_value<R>(B, u + static_cast<size_t>(i) * ni, static_cast<int>(j), t);
+                else if (head[i])
+                    out[t] = restrict_value<R>(B, u, i, t);
+                else
+                    out[t] = u[static_cast<size_t>(i) * ni + t];
+            }
+            keys_out[pos[i] + j] = emit_key<R>(keys[i], depth, mark[i] != 0, head[i] != 0, static_cast<int>(j));
+        }
+}
+
+template <int R>
+void adj_step_box(const int* ext, const uint64_t* keys, uint32_t n, const face_link* links, const uint32_t* metas,
+                  const double* lam_next, const double* u, const adjoint_params& prm, double* lam, double* vpart)
+{
+    const box<R> B = box_of<R>(ext);
+    for (uint32_t leaf = 0; leaf < n; ++leaf)
+    {
+        const face_link* L  = links + static_cast<size_t>(leaf) * box<R>::NDIR;
+        const uint32_t*  M  = metas + static_cast<size_t>(leaf) * box<R>::NDIR;
+        const int        lv = key_level(keys[leaf]);
+        double           dtdx[R], vp[R], vm[R], sgn[R], acc[R];
+        for (int d = 0; d < R; ++d)
+        {
+            dtdx[d] = prm.dtdx[lv][d];
+            vp[d]   = prm.vp[d];
+            vm[d]   = prm.vm[d];
+            sgn[d]  = prm.sgn[d];
+            acc[d]  = 0.0;
+        }
+        for (int i = 0; i < B.ni(); ++i)
+        {
+            lam[static_cast<size_t>(leaf) * B.ni() + i] = adj_cell_direct<R>(B, lam_next, leaf, L, M, i, lv, prm, dtdx, vp, vm);
+            adj_velocity_cell_direct<R>(B, u, leaf, links, metas, i, lam_next[static_cast<size_t>(leaf) * B.ni() + i],
+                                        dtdx, sgn, acc);
+        }
+        for (int d = 0; d < 3; ++d) vpart[static_cast<size_t>(leaf) * 3 + d] = d < R ? acc[d] : 0.0;
+    }
+}
+
+template <int R>
+void adj_regrid_box(const int* ext, const double* lam_new, const uint32_t* maps, uint32_t n_old, double* lam_old)
+{
+    const box<R>    B      = box_of<R>(ext);
+    const int       ni     = B.ni();
+    const uint32_t* mark   = maps;
+    const uint32_t* head   = maps + n_old;
+    const uint32_t* counts = maps + 2 * n_old;
+    const uint32_t* pos    = maps + 3 * n_old;
+    for (uint32_t i = 0; i < n_old; ++i)
+    {
+        if (counts[i] == 0) continue;
+        for (int t = 0; t < ni; ++t)
+        {
+            if (mark[i])
+                lam_old[static_cast<size_t>(i) * ni + t] = adj_refine_value<R>(B, lam_new, pos[i], t);
+            else if (head[i])
+                for (int kid = 0; kid < box<R>::F; ++kid)
+                    lam_old[static_cast<size_t>(i + kid) * ni + t] =
+                        adj_restrict_value<R>(B, lam_new + static_cast<size_t>(pos[i]) * ni, kid, t);
+            else
+                lam_old[static_cast<size_t>(i) * ni + t] = lam_new[static_cast<size_t>(pos[i]) * ni + t];
+        }
+    }
+}
+
+adjoint_params make_adjoint_params(const double* dtdx, const double* vel)
+{
+    adjoint_params prm{};
+    std::memcpy(prm.dtdx, dtdx, sizeof(prm.dtdx));
+    for (int d = 0; d < 3; ++d)
+    {
+        prm.vp[d]  = 0.5 * (vel[d] + std::abs(vel[d]));
+        prm.vm[d]  = 0.5 * (vel[d] - std::abs(vel[d]));
+        prm.sgn[d] = vel[d] > 0.0 ? 1.0 : (vel[d] < 0.0 ? -1.0 : 0.0);
+    }
+    return prm;
+}
+
 } // namespace
 
 #define EMU_SHAPES(X) X(2, 4, 1) X(2, 4, 2) X(2, 8, 1) X(2, 16, 1) X(2, 10, 2) X(2, 50, 2) X(3, 4, 1) X(3, 8, 1)
@@ -353,6 +514,75 @@ int emu_uniform_neighbors(uint32_t level, uint32_t m, uint32_t* out)
 #undef PLANE
 #undef ROW
 #undef N3
+    return 0;
+}
+
+// ---- any-shape (box) kernels: ext = extents of layout dims 0..rank-1 ------
+
+int emu_step_box(int rank, const int* ext, const uint64_t* keys, uint32_t n, const void* links,
+                 const uint32_t* metas, const double* u, double* un, const double* dtdx, const double* vel,
+                 const double* hs)
+{
+    auto const* L = static_cast<const face_link*>(links);
+    if (rank == 2)
+        step_box<2>(ext, keys, n, L, metas, u, un, dtdx, vel, hs);
+    else
+        step_box<3>(ext, keys, n, L, metas, u, un, dtdx, vel, hs);
+    return 0;
+}
+
+int emu_padded_box(int rank, const int* ext, int h, uint32_t n, const void* links, const uint32_t* metas,
+                   const double* u, double* pad)
+{
+    auto const* L = static_cast<const face_link*>(links);
+    if (rank == 2)
+        padded_box<2>(ext, h, n, L, metas, u, pad);
+    else
+        padded_box<3>(ext, h, n, L, metas, u, pad);
+    return 0;
+}
+
+int emu_flags_box(int rank, const int* ext, int h, int kind, const uint64_t* keys, uint32_t n, const void* links,
+                  const uint32_t* metas, const double* u, double rthr, double cthr, int mn, int mx, int8_t* out)
+{
+    auto const* L = static_cast<const face_link*>(links);
+    if (rank == 2)
+        flags_box<2>(ext, h, kind, keys, n, L, metas, u, rthr, cthr, mn, mx, out);
+    else
+        flags_box<3>(ext, h, kind, keys, n, L, metas, u, rthr, cthr, mn, mx, out);
+    return 0;
+}
+
+int emu_transfer_box(int rank, const int* ext, int depth, const uint64_t* keys, uint32_t n, const uint32_t* maps,
+                     const double* u, uint64_t* keys_out, double* u_out)
+{
+    if (rank == 2)
+        transfer_box<2>(ext, depth, keys, n, maps, u, keys_out, u_out);
+    else
+        transfer_box<3>(ext, depth, keys, n, maps, u, keys_out, u_out);
+    return 0;
+}
+
+int emu_adj_step_box(int rank, const int* ext, const uint64_t* keys, uint32_t n, const void* links,
+                     const uint32_t* metas, const double* lam_next, const double* u, const double* dtdx,
+                     const double* vel, double* lam, double* vpart)
+{
+    auto const*          L   = static_cast<const face_link*>(links);
+    const adjoint_params prm = make_adjoint_params(dtdx, vel);
+    if (rank == 2)
+        adj_step_box<2>(ext, keys, n, L, metas, lam_next, u, prm, lam, vpart);
+    else
+        adj_step_box<3>(ext, keys, n, L, metas, lam_next, u, prm, lam, vpart);
+    return 0;
+}
+
+int emu_adj_regrid_box(int rank, const int* ext, const double* lam_new, const uint32_t* maps, uint32_t n_old,
+                       double* lam_old)
+{
+    if (rank == 2)
+        adj_regrid_box<2>(ext, lam_new, maps, n_old, lam_old);
+    else
+        adj_regrid_box<3>(ext, lam_new, maps, n_old, lam_old);
     return 0;
 }
 

@@ -26,7 +26,8 @@ def load():
             build()
         _lib = ctypes.CDLL(str(LIB))
         for name in ("emu_build_table", "emu_step", "emu_padded", "emu_flags", "emu_regrid", "emu_adj_step",
-                     "emu_adj_regrid", "emu_runs", "emu_uniform_neighbors"):
+                     "emu_adj_regrid", "emu_runs", "emu_uniform_neighbors", "emu_step_box", "emu_padded_box",
+                     "emu_flags_box", "emu_transfer_box", "emu_adj_step_box", "emu_adj_regrid_box"):
             getattr(_lib, name).restype = ctypes.c_int
     return _lib
 
@@ -64,10 +65,12 @@ def to_reference_metadata(rank, links, metas):
 
 
 def step_params(rank, n, depth, max_level, vel, dt, lengths=(1.0, 1.0, 1.0)):
+    """n: cells per dim (int) or the extents of layout dims 0..rank-1."""
+    ext = (n,) * rank if isinstance(n, int) else tuple(n)
     dtdx = np.zeros((17, 3))
     for lv in range(depth + 1):
         for d in range(rank):
-            cs = (lengths[d] * float(1 << (depth - lv)) / float(1 << depth)) / float(n)
+            cs = (lengths[d] * float(1 << (depth - lv)) / float(1 << depth)) / float(ext[rank - 1 - d])
             dtdx[lv, d] = dt / cs
     v = np.zeros(3)
     v[:rank] = vel
@@ -160,3 +163,70 @@ def uniform_neighbors(level, slot):
     out = np.zeros(27, dtype=np.uint32)
     load().emu_uniform_neighbors(ctypes.c_uint32(level), ctypes.c_uint32(slot), _p(out))
     return out
+
+
+# ---- the any-shape (box) kernels; ext = extents of layout dims 0..rank-1 ----
+def _ext(ext):
+    e = (ctypes.c_int * 3)(*(list(ext) + [1] * (3 - len(ext))))
+    return e
+
+
+def step_box(rank, ext, keys, links, metas, u, dtdx, vel, hs):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    un = np.zeros_like(u)
+    load().emu_step_box(rank, _ext(ext), _p(keys), ctypes.c_uint32(keys.size), _p(links), _p(metas), _p(u), _p(un),
+                        _p(np.ascontiguousarray(dtdx)), _p(vel), _p(hs))
+    return un
+
+
+def padded_box(rank, ext, h, links, metas, u, nleaves):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    pshape = tuple(x + 2 * h for x in ext)
+    pad = np.zeros(nleaves * int(np.prod(pshape)))
+    load().emu_padded_box(rank, _ext(ext), h, ctypes.c_uint32(nleaves), _p(links), _p(metas), _p(u), _p(pad))
+    return pad.reshape((nleaves,) + pshape)
+
+
+def flags_box(rank, ext, h, kind, keys, links, metas, u, rthr, cthr, mn, mx):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(keys.size, dtype=np.int8)
+    load().emu_flags_box(rank, _ext(ext), h, kind, _p(keys), ctypes.c_uint32(keys.size), _p(links), _p(metas), _p(u),
+                         ctypes.c_double(rthr), ctypes.c_double(cthr), mn, mx, _p(out))
+    return out
+
+
+def transfer_box(rank, ext, depth, keys, maps, u):
+    """The regrid transfer (emit) of the box kernels for a regrid plan."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    maps = np.ascontiguousarray(maps, dtype=np.uint32)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    total = int(maps[2].sum())
+    ni = int(np.prod(ext))
+    keys_out = np.zeros(total, dtype=np.uint64)
+    u_out = np.zeros(total * ni)
+    load().emu_transfer_box(rank, _ext(ext), depth, _p(keys), ctypes.c_uint32(keys.size), _p(maps), _p(u),
+                            _p(keys_out), _p(u_out))
+    return keys_out, u_out
+
+
+def adj_step_box(rank, ext, keys, links, metas, lam_next, u, dtdx, vel):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    lam_next = np.ascontiguousarray(lam_next, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.zeros(3)
+    v[: len(vel)] = vel
+    lam = np.zeros_like(lam_next)
+    vpart = np.zeros((keys.size, 3))
+    load().emu_adj_step_box(rank, _ext(ext), _p(keys), ctypes.c_uint32(keys.size), _p(links), _p(metas),
+                            _p(lam_next), _p(u), _p(np.ascontiguousarray(dtdx)), _p(v), _p(lam), _p(vpart))
+    return lam, vpart
+
+
+def adj_regrid_box(rank, ext, lam_new, maps, n_old):
+    lam_new = np.ascontiguousarray(lam_new, dtype=np.float64)
+    maps = np.ascontiguousarray(maps, dtype=np.uint32)
+    lam_old = np.zeros(n_old * int(np.prod(ext)))
+    load().emu_adj_regrid_box(rank, _ext(ext), _p(lam_new), _p(maps), ctypes.c_uint32(n_old), _p(lam_old))
+    return lam_old

@@ -349,9 +349,12 @@ def test_capacity_and_root_errors(amr):
     with pytest.raises(amr.AmrError) as e:
         g.reconstruct_tree(np.full(4, 2, dtype=np.int8))
     assert e.value.code == amr.AMR_ERR_UNSUPPORTED
+    # any even patch extent runs (12^2 has no specialised kernels); an odd
+    # one cannot be split between children
+    amr.NdTree(2, 7, 12, 1, 10).close()
     with pytest.raises(amr.AmrError) as e:
-        amr.NdTree(2, 7, 12, 1, 10)
-    assert e.value.code == amr.AMR_ERR_UNSUPPORTED
+        amr.NdTree(2, 7, 13, 1, 10)
+    assert e.value.code == amr.AMR_ERR_ARGUMENT
 
 
 def test_reference_tests_through_cpp_api():
@@ -511,3 +514,59 @@ def test_fusion_follows_the_topology(amr):
     s2 = amr.AmrSolver(g2, (1.0, 0.5), 0.4)
     g2.refine_all()
     assert s2.step_fusion() == (True, False), "no fused kernel for 2D patches"
+
+
+def test_swap_buffers_semantics(amr):
+    """amr_tree_swap_buffers is ndtree::swap_buffers (ndtree.hpp:1558-1579):
+    the current and next state exchange roles.  After one step the next
+    buffer holds the state the step read, so a swap returns to it (bit for
+    bit, as the oracle's swap does), a second swap returns to the stepped
+    state, and stepping on continues from there.  A fused pair of steps
+    (single-level trees) writes u^(n+2) over no intermediate state: the next
+    buffer then holds u^n."""
+    say("case: swap_buffers after single and fused steps")
+    g, s, o = pair(amr, 2, 16, 1, 7, 300, (1.0, 0.5), 0.4)
+    for _ in range(2):
+        g.refine_all()
+        o.refine_all()
+    g.reconstruct_tree(np.array([1] + [0] * 15, dtype=np.int8))  # two levels: no fusion
+    o.reconstruct(np.array([1] + [0] * 15, dtype=np.int8))
+    rng = np.random.default_rng(13)
+    u0 = rng.standard_normal(g.size() * 256)
+    g.set_interior(u0)
+    o.set_interior(u0)
+    o.halo_exchange()
+    assert s.step_fusion() == (True, False)
+    dt = s.advance()
+    o.step(dt)
+    u1 = o.get_interior().copy()
+    assert_fields(g.get_interior(), u1, "after one step")
+    g.swap_buffers()
+    o.swap_buffers()
+    assert_fields(g.get_interior(), o.get_interior(), "swapped")
+    assert np.array_equal(g.get_interior().reshape(-1).view(np.uint64), u0.view(np.uint64)), "swap returns to u^n"
+    g.swap_buffers()
+    o.swap_buffers()
+    assert_fields(g.get_interior(), u1, "swapped back")
+    o.halo_exchange()
+    s.advance_batch_async(3)
+    s.finish_advance_batch()
+    o.advance_batch(3)
+    assert_fields(g.get_interior(), o.get_interior(), "3 more steps")
+    g.close()
+    # single-level 8^3 tree: a fused pair
+    g = amr.NdTree(3, 5, 8, 1, 128)
+    s = amr.AmrSolver(g, (1.0, 0.5, 0.25), 0.3)
+    for _ in range(2):
+        g.refine_all()
+    w0 = rng.standard_normal(g.size() * 512)
+    g.set_interior(w0)
+    assert s.step_fusion() == (True, True)
+    s.advance_batch_async(2)
+    s.finish_advance_batch()
+    u2 = g.get_interior().copy()
+    g.swap_buffers()
+    assert np.array_equal(g.get_interior().reshape(-1).view(np.uint64), w0.view(np.uint64))
+    g.swap_buffers()
+    assert np.array_equal(g.get_interior().view(np.uint64), u2.view(np.uint64))
+    g.close()

@@ -1,0 +1,265 @@
+"""Patch shapes outside the fixed-shape kernels, through the C ABI against
+the CPU oracle: non-square patches (static_shape<N, M[, O]>, both
+orientations, halo 1 and 2, 2D and 3D) and a square shape without
+specialised kernels.  These run the any-shape kernels (the box instance).
+Bar: keys, neighbour tables and flags bit-exact; halos against brute-force
+geometry; fields bit-exact (no contraction on either side), with the 1e-12
+relative tolerance asserted as well."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import brute
+from amrlib import load_package
+from oracle import OracleEuler, OracleTree, euler_prim_to_cons
+from test_gpu_parity import assert_fields, assert_topology
+from test_oracle_tree import random_flags
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(2, (16, 8), 1, 6), (2, (8, 16), 1, 6), (2, (12, 8), 2, 5), (3, (4, 6, 8), 1, 4), (2, (6, 6), 1, 5)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def amr():
+    return load_package()
+
+
+def pair(amr, rank, ext, h, depth, cap, vel, cfl):
+    g = amr.NdTree(rank, depth, ext, h, cap)
+    s = amr.AmrSolver(g, vel, cfl)
+    o = OracleTree(rank, ext, h, depth, 2 * cap + 64)
+    o.set_params(vel, cfl, threads=8)
+    return g, s, o
+
+
+@pytest.mark.parametrize("rank,ext,h,depth", SHAPES)
+def test_any_shape_regrids_halos_steps(amr, rank, ext, h, depth):
+    say(f"case: any-shape regrid sequence rank={rank} ext={ext} h={h} depth={depth}")
+    vel = (1.0, -0.5, 0.25)[:rank]
+    cap = (1 << (rank * depth)) + 64
+    g, s, o = pair(amr, rank, ext, h, depth, cap, vel, 0.4)
+    assert s.step_variants() == ["box"]
+    for _ in range(2 if rank == 2 else 1):
+        g.refine_all()
+        o.refine_all()
+    assert_topology(g, o, "start")
+    assert np.array_equal(g.cell_centers().view(np.uint64), o.cell_centers().view(np.uint64))
+    rng = np.random.default_rng(sum(ext) + h)
+    for rd in range(5):
+        keys = o.keys()
+        v = rng.standard_normal(o.size() * o.ni)
+        g.set_interior(v)
+        o.set_interior(v)
+        flags = random_flags(rng, keys, rank, depth)
+        assert g.reconstruct_tree(flags) == o.reconstruct(flags), f"round {rd}: changed flag"
+        assert_topology(g, o, f"round {rd}")
+        assert_fields(g.get_interior(), o.get_interior(), f"round {rd}: transfer")
+        o.halo_exchange()
+        gp, op = g.get_patches(), o.get_padded()
+        eh = brute.expected_halo(o.keys(), o.get_interior(), rank, ext, h, depth)
+        m = ~np.isnan(eh)
+        assert_fields(gp[m], op[m], f"round {rd}: halos")
+        assert_fields(gp[m], eh[m], f"round {rd}: halos vs geometry")
+        for kind, thr in ((0, (0.5, -1.0)), (1, (1.0, 0.2))):
+            crit = amr.make_criterion(kind, thr[0], thr[1], 1, depth)
+            np.testing.assert_array_equal(g.compute_flags(crit), o.criterion(kind, thr[0], thr[1], 1, depth),
+                                          err_msg=f"round {rd}: criterion {kind}")
+        assert s.compute_dt() == o.compute_dt()
+        s.advance_batch_async(3)
+        tot, ex = s.finish_advance_batch()
+        otot, oex = o.advance_batch(3)
+        assert ex == oex == 3 and tot == otot
+        assert_fields(g.get_interior(), o.get_interior(), f"round {rd}: 3 steps")
+    say("leaves at end:", g.size())
+    g.close()
+
+
+@pytest.mark.parametrize("ext", [(16, 8), (8, 16)])
+def test_non_square_uniform_100_steps(amr, ext):
+    """The 2D uniform configuration with 16x8 / 8x16 patches: a Gaussian bump
+    advected 100 steps, bit-exact against the oracle, mass conserved."""
+    say(f"case: 2D uniform level 4, {ext} patches, 100 steps")
+    g, s, o = pair(amr, 2, ext, 1, 7, 256, (1.0, 0.5), 0.4)
+    for _ in range(4):
+        g.refine_all()
+        o.refine_all()
+    cen = o.cell_centers()
+    u0 = np.exp(-((cen[..., 0] - 0.3) ** 2 + (cen[..., 1] - 0.3) ** 2) / (2 * 0.1**2))
+    g.set_interior(u0)
+    o.set_interior(u0)
+    o.halo_exchange()
+    dx = 1.0 / (16 * ext[1])
+    dy = 1.0 / (16 * ext[0])
+    assert s.compute_dt() == o.compute_dt() == 0.4 * min(dx / 1.0, dy / 0.5)
+    s.advance_batch_async(100)
+    tot, ex = s.finish_advance_batch()
+    otot, oex = o.advance_batch(100)
+    assert ex == oex == 100 and tot == otot
+    assert_fields(g.get_interior(), o.get_interior(), "after 100 steps")
+    assert abs(g.get_interior().sum() - u0.sum()) <= 1e-12 * u0.sum()
+    g.close()
+
+
+def test_non_square_batch_matches_sequential(amr):
+    """Three 2D trees of 16x8 patches with their own velocities, stepped as
+    one batch (graph replay), against each tree stepped alone."""
+    say("case: batch of 3 non-square trees")
+    rng = np.random.default_rng(17)
+    trees, solvers, ref = [], [], []
+    for i in range(3):
+        g = amr.NdTree(2, 7, (16, 8), 1, 2000)
+        s = amr.AmrSolver(g, tuple(rng.uniform(-1, 1, 2)), 0.4)
+        for _ in range(2):
+            g.refine_all()
+        for _ in range(2):
+            g.reconstruct_tree(random_flags(rng, g.keys(), 2, 7))
+        g.set_interior(rng.standard_normal(g.size() * 128))
+        trees.append(g)
+        solvers.append(s)
+    init = [t.get_interior().copy() for t in trees]
+    for s in solvers:
+        s.advance_batch_async(12)
+        s.finish_advance_batch()
+        ref.append(s.tree.get_interior().copy())
+    for t, u in zip(trees, init):
+        t.set_interior(u)
+    b = amr.Batch(trees)
+    b.advance(12, use_graph=True)
+    b.synchronize()
+    b.scatter()
+    for t, r in zip(trees, ref):
+        assert np.array_equal(t.get_interior().view(np.uint64), r.view(np.uint64))
+    b.close()
+    for t in trees:
+        t.close()
+
+
+def euler_ic(rank):
+    def ic(cen):
+        r2 = sum((cen[..., d] - 0.5) ** 2 for d in range(rank))
+        pert = 10.0 * np.exp(-r2 / 0.01)
+        prim = np.zeros(cen.shape[:-1] + (rank + 2,))
+        prim[..., 0] = 0.5 + 0.2 * pert
+        for d in range(rank):
+            prim[..., 1 + d] = (0.3, -0.2, 0.1)[d]
+        prim[..., rank + 1] = 1.0 + pert
+        return prim
+
+    return ic
+
+
+@pytest.mark.parametrize("rank,ext,depth,start,cycles", [(2, (16, 8), 7, 2, 4), (3, (4, 6, 8), 5, 1, 3)])
+def test_euler_non_square_adaptive(amr, rank, ext, depth, start, cycles):
+    say(f"case: Euler adaptive rank={rank} ext={ext}")
+    cap = (1 << (rank * min(depth, start + cycles))) + 64
+    s = amr.EulerSolver(rank, depth, ext, 1, cap)
+    o = OracleEuler(rank, ext, 1, depth, 2 * cap + 64)
+    o.set_params(1.4, 0.3, threads=16)
+    for _ in range(start):
+        s.tree.refine_all()
+        o.refine_all()
+    ic = euler_ic(rank)
+    s.initialize(ic)
+    prim = ic(o.cell_centers())
+    cons = euler_prim_to_cons(rank, prim.reshape(-1, rank + 2), 1.4).reshape(prim.shape)
+    o.set_interior(np.moveaxis(cons, -1, 0))
+    o.halo_exchange()
+    assert_fields(s.tree.get_interior(), o.get_interior(), "initial state")
+    top = min(depth, start + cycles)
+    crit = amr.make_criterion(amr.CRITERION_MAX_VALUE, 0.53, 0.49, 1, top)
+    for it in range(cycles):
+        s.advance_batch_async(4)
+        got = s.finish_advance_batch()
+        assert got == o.advance_batch(4), it
+        fo = o.criterion(0, 0.53, 0.49, 1, top)
+        np.testing.assert_array_equal(s.tree.compute_flags(crit), fo, err_msg=f"cycle {it}: flags")
+        assert s.tree.reconstruct_tree(crit) == o.reconstruct(fo)
+        o.halo_exchange()
+        np.testing.assert_array_equal(s.tree.keys(), o.keys())
+        assert_fields(s.tree.get_interior(), o.get_interior(), f"cycle {it}")
+    say("leaves at end:", s.tree.size())
+
+
+def test_adjoint_non_square(amr):
+    """The discrete adjoint on 16x8 patches: the replay reproduces the
+    recorded solve, the adjoint satisfies the dot-product identity, and the
+    velocity gradient matches central differences of the replay."""
+    import torch
+
+    from gpu_amr_amd.adjoint import advection_solve
+
+    say("case: tape + adjoint on 16x8 patches, 30 steps, regrid every 10")
+    g = amr.NdTree(2, 7, (16, 8), 1, 4096 + 64)
+    s = amr.AmrSolver(g, (1.0, 0.5), 0.4)
+    for _ in range(3):
+        g.refine_all()
+    s.initialize(lambda c: np.exp(-((c[..., 0] - 0.4) ** 2 + (c[..., 1] - 0.5) ** 2) / (2 * 0.05**2)))
+    u0 = torch.from_numpy(g.get_interior().reshape(-1).copy()).cuda()
+    crit = amr.make_criterion(amr.CRITERION_GRADIENT, 0.05, 0.01, 3, 5)
+    tape = amr.Tape(s, 30, crit, regrid_every=10, checkpoint_every=7)
+    info = tape.info()
+    assert info["regrids"] == 2
+    final = g.get_interior().reshape(-1).copy()
+    vel = torch.tensor((1.0, 0.5), dtype=torch.float64)
+    with torch.no_grad():
+        uT = advection_solve(u0, vel, tape).cpu().numpy()
+    assert np.array_equal(uT.view(np.uint64), final.view(np.uint64))
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(u0.numel())
+    lam = rng.standard_normal(info["final_leaves"] * 128)
+    with torch.no_grad():
+        fx = advection_solve(torch.from_numpy(x).cuda(), vel, tape).cpu().numpy()
+    gu = torch.empty_like(u0)
+    gv = tape.adjoint(torch.from_numpy(lam).cuda().data_ptr(), gu.data_ptr())
+    lhs = math.fsum((lam * fx).ravel())
+    rhs = math.fsum((gu.cpu().numpy() * x).ravel())
+    rel = abs(lhs - rhs) / math.sqrt(math.fsum(lam * lam) * math.fsum(x * x))
+    say(f"dot product: rel {rel:.2e}, grad_v {gv}")
+    assert rel <= 1e-13
+    # d<lam, F_v(x)>/dv by central differences (F is linear in x, smooth in v
+    # away from v_d = 0; dt is held fixed by the tape)
+    for d in range(2):
+        e = torch.zeros(2, dtype=torch.float64)
+        e[d] = 1e-5
+        with torch.no_grad():
+            fp = advection_solve(torch.from_numpy(x).cuda(), vel + e, tape).cpu().numpy()
+            fm = advection_solve(torch.from_numpy(x).cuda(), vel - e, tape).cpu().numpy()
+        fd = (math.fsum((lam * fp).ravel()) - math.fsum((lam * fm).ravel())) / 2e-5
+        # adjoint of the last forward pass (x at v)
+        with torch.no_grad():
+            advection_solve(torch.from_numpy(x).cuda(), vel, tape)
+        gv = tape.adjoint(torch.from_numpy(lam).cuda().data_ptr(), gu.data_ptr())
+        say(f"d/dv{d}: fd {fd:.12e}, adjoint {gv[d]:.12e}")
+        assert abs(fd - gv[d]) <= 1e-6 * max(abs(gv[d]), 1.0)
+    tape.close()
+    g.close()
+
+
+@pytest.mark.parametrize("rank,ext,h,depth", [(2, (16, 8), 1, 7), (2, (12, 8), 2, 7), (3, (4, 6, 8), 1, 5)])
+def test_vtk_non_square(amr, rank, ext, h, depth, tmp_path):
+    from test_gpu_vtk import check, flags_every_third
+
+    say(f"case: VTK rank={rank} ext={ext} h={h}")
+    L = (2.0, 1.0, 0.5)[:rank]
+    g = amr.NdTree(rank, depth, ext, h, 600, L)
+    o = OracleTree(rank, ext, h, depth, 1400, L)
+    for _ in range(2 if rank == 2 else 1):
+        g.refine_all()
+        o.refine_all()
+    fl = flags_every_third(g.keys(), depth)
+    g.reconstruct_tree(fl)
+    o.reconstruct(fl)
+    np.testing.assert_array_equal(g.keys(), o.keys())
+    g.set_interior(np.random.default_rng(3).standard_normal(g.size() * g.ni))
+    path = tmp_path / "tree.vtk"
+    g.write_vtk(path)
+    check(g, path, rank, ext, h, ["Scalar"], o)
+    g.close()

@@ -65,18 +65,22 @@ def parse_vtk(path):
 
 
 def expected_halo(rank, n, h):
-    P = n + 2 * h
+    """is_halo as vtk_print computes it (vtk_print.hpp:217-227): the interior
+    element index j read as a padded linear index."""
+    ext = (n,) * rank if isinstance(n, int) else tuple(n)
+    P = tuple(x + 2 * h for x in ext)
     flags = []
-    for j in range(n**rank):
-        idx = np.unravel_index(j, (P,) * rank)
-        flags.append(int(any(i < h or i >= P - h for i in idx)))
+    for j in range(int(np.prod(ext))):
+        idx = np.unravel_index(j, P)
+        flags.append(int(any(i < h or i >= p - h for i, p in zip(idx, P))))
     return np.array(flags)
 
 
 def check(tree, path, rank, n, h, names, oracle):
     v = parse_vtk(path)
     assert v["header"] == ["# vtk DataFile Version 3.0", "AMR Tree Structure", "BINARY", "DATASET UNSTRUCTURED_GRID"]
-    leaves, ni, box = tree.size(), n**rank, 1 << rank
+    ext = (n,) * rank if isinstance(n, int) else tuple(n)
+    leaves, ni, box = tree.size(), int(np.prod(ext)), 1 << rank
     cells = leaves * ni
     assert v["ncells"] == cells and v["points"].shape == (cells * box, 3)
     # connectivity, types, cell_index
@@ -92,7 +96,8 @@ def check(tree, path, rank, n, h, names, oracle):
     keys = oracle.keys()
     lv = (keys & np.uint64(63)).astype(int)
     L = np.array(tree.lengths)
-    cs = np.repeat((L[None, :] / (2.0 ** lv[:, None])) / n, ni, axis=0)
+    per_axis = np.array([ext[rank - 1 - a] for a in range(rank)], dtype=np.float64)  # cells along physical axis a
+    cs = np.repeat((L[None, :] / (2.0 ** lv[:, None])) / per_axis[None, :], ni, axis=0)
     pts = v["points"].reshape(cells, box, 3)
     lo = cen - 0.5 * cs
     for k in range(box):

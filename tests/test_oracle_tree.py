@@ -51,7 +51,7 @@ def run_sequence(rank, n, h, depth, start_level, rounds, seed, check_halo=True):
     coarsened = 0
     for rd in range(rounds):
         keys = t.keys()
-        vals = rng.standard_normal(t.size() * n**rank)
+        vals = rng.standard_normal(t.size() * t.ni)
         t.set_interior(vals)
         flags = random_flags(rng, keys, rank, depth)
         exp_keys, exp_r, exp_c = brute.expected_regrid(keys, flags, rank, depth)
@@ -92,6 +92,69 @@ def test_regrid_sequences_2d_halo2():
 def test_regrid_sequences_3d(seed):
     ch, co = run_sequence(3, 4, 1, 4, 1, 6, seed, check_halo=(seed == 4))
     assert ch >= 3
+
+
+# Non-square patches (static_shape<N, M[, O]>): extents per layout dim, the
+# last is physical x; both orientations, a halo-2 shape and a 3D box.
+@pytest.mark.parametrize(
+    "rank,ext,h,depth,start,seed",
+    [(2, (16, 8), 1, 5, 2, 21), (2, (8, 16), 1, 5, 2, 22), (2, (12, 8), 2, 5, 2, 23), (3, (4, 6, 8), 1, 4, 1, 24)],
+)
+def test_regrid_sequences_non_square(rank, ext, h, depth, start, seed):
+    ch, _ = run_sequence(rank, ext, h, depth, start, 6 if rank == 2 else 4, seed)
+    assert ch >= 2
+
+
+@pytest.mark.parametrize("ext,h", [((16, 8), 1), ((8, 16), 1), ((12, 8), 2)])
+def test_dense_uniform_step_non_square_bitwise(ext, h):
+    """Uniform tree of non-square patches = one dense periodic array of
+    (2^level n_0) x (2^level n_1) cells with dx per axis: the textbook sweep,
+    bit for bit."""
+    rank, depth, lv = 2, 7, 3
+    t = OracleTree(rank, ext, h, depth, 2 * 4**lv + 8)
+    for _ in range(lv):
+        t.refine_all()
+    rng = np.random.default_rng(sum(ext))
+    t.set_interior(rng.random(t.size() * t.ni))
+    t.halo_exchange()
+    t.set_params((1.0, -0.5), 0.4)
+    dx = (1.0 / (8 * ext[1]), 1.0 / (8 * ext[0]))  # physical x is layout dim 1
+    dt = t.compute_dt()
+    assert dt == 0.4 * min(dx[0] / 1.0, dx[1] / 0.5)
+    keys = t.keys()
+    dense = brute.dense_from_uniform(keys, t.get_interior(), rank, ext, depth)
+    assert dense.shape == (8 * ext[0], 8 * ext[1])
+    for _ in range(4):
+        dense = brute.dense_upwind_step(dense, (1.0, -0.5), dt, dx)
+        t.step(dt)
+    exp = brute.split_uniform(keys, dense, rank, ext, depth)
+    assert np.array_equal(t.get_interior().view(np.uint64), exp.view(np.uint64))
+
+
+def test_finer_ghost_rule_for_non_square_patches():
+    """The finer-neighbour ghost of a non-square patch: the fine cells lie at
+    (2 v) mod n(d) along every dim d (the reference's device rule,
+    src/cuda/halo_exchange.cu:174-176), which is where geometry puts them
+    (expected_halo, checked in the regrid sequences above).  The reference's
+    CPU rule reduces every dim mod the face normal's extent
+    (include/ndtree/patch_utils.hpp:358): for an 8 x 16 patch (x = layout
+    dim 1) its -x/+x ghosts along y would start at padded row up to
+    (2*7) % 16 + h = 15, past the 8 + 2h = 10 rows of the fine patch, so
+    that rule has no result to reproduce there; for square patches the two
+    rules coincide."""
+    h = 1
+    for ext in ((8, 16), (16, 8), (12, 8)):
+        for dim in range(2):  # face normal
+            tang = 1 - dim
+            device = [((v * 2) % ext[tang]) + h for v in range(ext[tang])]
+            cpu = [((v * 2) % ext[dim]) + h for v in range(ext[tang])]
+            # the device rule stays inside the fine patch's interior
+            assert all(h <= b and b + 1 < h + ext[tang] for b in device)
+            if ext[dim] > ext[tang]:
+                assert cpu != device
+                if ext[dim] >= 2 * ext[tang]:
+                    # past the fine patch's padded rows
+                    assert max(cpu) + 1 >= ext[tang] + 2 * h
 
 
 def test_dense_uniform_step_2d_bitwise():
