@@ -96,7 +96,7 @@ struct ops
     void (*emit)(ops const&, const uint64_t*, const double*, uint32_t, int, const uint8_t*, const uint8_t*,
                  const uint32_t*, const uint32_t*, uint64_t*, double*, regrid_summary*, int, size_t, hipStream_t);
     void (*adj_step)(ops const&, const double*, const double*, const uint64_t*, const face_link*, const uint32_t*,
-                     adjoint_params const&, double*, double*, uint32_t, hipStream_t);
+                     const adjoint_params*, double*, double*, uint32_t, hipStream_t);
     void (*adj_regrid)(ops const&, const double*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
                        double*, uint32_t, hipStream_t);
     // two fused steps of a single-level tree (null: not built for the shape)
@@ -220,7 +220,7 @@ struct instance
         check_launch("k_emit");
     }
     static void adj_step(ops const&, const double* lam_next, const double* u, const uint64_t* keys, const face_link* links,
-                         const uint32_t* metas, adjoint_params const& prm, double* lam, double* vpart, uint32_t n,
+                         const uint32_t* metas, const adjoint_params* prm, double* lam, double* vpart, uint32_t n,
                          hipStream_t s)
     {
         if (n == 0) return;
@@ -378,7 +378,7 @@ struct box_instance
         check_launch("k_emit_box");
     }
     static void adj_step(ops const& o, const double* lam_next, const double* u, const uint64_t* keys,
-                         const face_link* links, const uint32_t* metas, adjoint_params const& prm, double* lam,
+                         const face_link* links, const uint32_t* metas, const adjoint_params* prm, double* lam,
                          double* vpart, uint32_t n, hipStream_t s)
     {
         if (n == 0) return;
@@ -1145,21 +1145,32 @@ struct amr_tape
     // work buffers (max_n leaves each)
     double*                    lam[2]{};
     double*                    work  = nullptr;
-    double*                    vpart = nullptr; // per-leaf v-gradient triples of every step
-    std::vector<uint64_t>      vbase;           // first triple of step k in vpart
-    double*                    chunk = nullptr;
-    double*                    acc   = nullptr;
+    double*                    vpart       = nullptr; // per-leaf v-gradient triples of one segment's steps
+    double*                    step_sums   = nullptr; // the triple sums of every step, step after step
+    uint32_t*                  step_leaves = nullptr; // leaf count of every step
+    double*                    chunk     = nullptr;
+    double*                    acc       = nullptr;
+    // Per-step constants in device memory, written before every pass: the
+    // velocity-dependent ones change between calls while the captured
+    // launches stay the same.  inst0: instance 0 for every leaf (the step
+    // kernels' per-leaf table lookup).
+    step_params*               step_tab = nullptr;
+    adjoint_params*            adj_tab  = nullptr;
+    uint16_t*                  inst0    = nullptr;
+    std::vector<step_params>   h_step_tab;
+    std::vector<adjoint_params> h_adj_tab;
+    std::vector<uint32_t>      h_leaves;
     uint64_t*                  scratch_keys    = nullptr;
     regrid_summary*            scratch_summary = nullptr;
     std::vector<double*>       seg; // recomputed states of one segment
     // The launch sequence of a recorded tape is fixed (steps, regrids,
     // buffers), so the forward replay and the adjoint pass are each captured
-    // once into a hipGraph and replayed.  The velocity is part of the kernels'
-    // arguments: a call with another velocity captures the pass again.
+    // once into a hipGraph and replayed; the kernels read the velocity-
+    // dependent constants from step_tab / adj_tab, so a new velocity needs no
+    // new capture.
     struct pass_graph
     {
-        hipGraphExec_t exec = nullptr;
-        double         v[3]{};
+        hipGraphExec_t exec   = nullptr;
         double*        result = nullptr; // buffer that holds the pass's output
     };
     pass_graph g_forward, g_adjoint;
@@ -1169,7 +1180,8 @@ struct amr_tape
         if (stream) (void)hipStreamSynchronize(stream);
         for (pass_graph* g : { &g_forward, &g_adjoint })
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
-        std::vector<void*> ptrs = { lam[0], lam[1], work, vpart, chunk, acc, scratch_keys, scratch_summary };
+        std::vector<void*> ptrs = { lam[0], lam[1], work, vpart, step_sums, step_leaves, chunk, acc, scratch_keys,
+                                    scratch_summary, step_tab, adj_tab, inst0 };
         for (auto& t : topos) ptrs.insert(ptrs.end(), { t.keys, t.links, t.metas });
         for (auto& r : regrids) ptrs.insert(ptrs.end(), { r.mark, r.head, r.counts, r.pos });
         for (double* c : ckpts) ptrs.push_back(c);
@@ -1283,17 +1295,52 @@ struct amr_tape
         // work buffers
         for (int b = 0; b < 2; ++b) lam[b] = dev_alloc<double>(max_n * ni);
         work            = dev_alloc<double>(max_n * ni);
-        // Step k's partials sit at vbase[k] whatever the checkpoint interval,
-        // and are summed once after the backward sweep: grad_v is the same
-        // bits for every interval.
-        vbase.assign(steps + 1, 0);
-        for (uint64_t k = 0; k < steps; ++k) vbase[k + 1] = vbase[k] + topo(k).n;
-        vpart           = dev_alloc<double>(vbase[steps] * 3);
+        // The adjoint steps of a segment write their per-leaf partials side
+        // by side; after the segment each step's are summed in a fixed order
+        // into its slot of step_sums, and after the sweep the slots are summed
+        // in step order: grad_v is the same bits for every checkpoint
+        // interval, and the partials need one segment's worth of memory.
+        vpart           = dev_alloc<double>(ckpt_every * max_n * 3);
+        step_sums       = dev_alloc<double>(steps * 3);
+        step_leaves     = dev_alloc<uint32_t>(steps);
         chunk           = dev_alloc<double>(kReduceBlocks * 3);
         acc             = dev_alloc<double>(3);
         scratch_keys    = dev_alloc<uint64_t>(max_n);
         scratch_summary = dev_alloc<regrid_summary>(1);
+        step_tab        = dev_alloc<step_params>(steps);
+        adj_tab         = dev_alloc<adjoint_params>(steps);
+        inst0           = dev_alloc<uint16_t>(max_n);
+        AMR_HIP_CHECK(hipMemsetAsync(inst0, 0, max_n * sizeof(uint16_t), stream));
+        h_leaves.resize(steps);
+        for (uint64_t k = 0; k < steps; ++k) h_leaves[k] = static_cast<uint32_t>(topo(k).n);
+        AMR_HIP_CHECK(hipMemcpyAsync(step_leaves, h_leaves.data(), steps * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                     stream));
         for (uint64_t j = 1; j < ckpt_every; ++j) seg.push_back(dev_alloc<double>(max_n * ni));
+        upload_constants();
+    }
+
+    // Every step's forward and adjoint constants for the current velocity,
+    // copied to the device ahead of the next pass (stream order); skipped
+    // when the device tables already hold this velocity's.
+    double uploaded_v[3]{};
+    bool   uploaded = false;
+    void   upload_constants()
+    {
+        if (uploaded && std::memcmp(uploaded_v, vel, sizeof(vel)) == 0) return;
+        AMR_HIP_CHECK(hipStreamSynchronize(stream)); // the host tables of the last upload are free
+        h_step_tab.resize(steps);
+        h_adj_tab.resize(steps);
+        for (uint64_t k = 0; k < steps; ++k)
+        {
+            h_step_tab[k] = step_consts(dts[k]);
+            h_adj_tab[k]  = adjoint_consts(dts[k]);
+        }
+        AMR_HIP_CHECK(hipMemcpyAsync(step_tab, h_step_tab.data(), steps * sizeof(step_params), hipMemcpyHostToDevice,
+                                     stream));
+        AMR_HIP_CHECK(hipMemcpyAsync(adj_tab, h_adj_tab.data(), steps * sizeof(adjoint_params), hipMemcpyHostToDevice,
+                                     stream));
+        std::memcpy(uploaded_v, vel, sizeof(vel));
+        uploaded = true;
     }
 
     regrid_record const* regrid_before(uint64_t k) const
@@ -1311,10 +1358,10 @@ struct amr_tape
 
     void forward_step(uint64_t k, const double* src, double* dst)
     {
-        topology const& t   = topo(k);
-        step_params     prm = step_consts(dts[k]);
-        prm.reverse         = static_cast<uint32_t>(k & 1);
-        op->step(*op, variant, src, dst, t.keys, t.links, t.metas, prm, nullptr, nullptr, static_cast<uint32_t>(t.n),
+        topology const& t = topo(k);
+        step_params     prm{}; // the sweep direction; the constants come from step_tab[k]
+        prm.reverse = static_cast<uint32_t>(k & 1);
+        op->step(*op, variant, src, dst, t.keys, t.links, t.metas, prm, step_tab + k, inst0, static_cast<uint32_t>(t.n),
                  step_aux{}, stream);
     }
 
@@ -1322,6 +1369,7 @@ struct amr_tape
     {
         AMR_HIP_CHECK(hipDeviceSynchronize()); // inputs written on other streams
         for (int d = 0; d < 3; ++d) vel[d] = d < rank ? v[d] : 0.0;
+        upload_constants();
         AMR_HIP_CHECK(hipMemcpyAsync(lam[0], u0, field_bytes(topos[0].n), hipMemcpyDeviceToDevice, stream));
         run_pass(g_forward, [this]() {
             double* a = lam[0];
@@ -1387,8 +1435,8 @@ struct amr_tape
             for (uint64_t k = e; k-- > c;)
             {
                 topology const& t = topo(k);
-                op->adj_step(*op, l, states[k - c], t.keys, t.links, t.metas, adjoint_consts(dts[k]), lo,
-                             vpart + 3 * vbase[k], static_cast<uint32_t>(t.n), stream);
+                op->adj_step(*op, l, states[k - c], t.keys, t.links, t.metas, adj_tab + k, lo,
+                             vpart + (k - c) * max_n * 3, static_cast<uint32_t>(t.n), stream);
                 std::swap(l, lo);
                 if (auto const* r = regrid_before(k))
                 {
@@ -1396,26 +1444,28 @@ struct amr_tape
                     std::swap(l, lo);
                 }
             }
+            // the segment's per-step velocity partials, each step into its slot
+            hipLaunchKernelGGL(k_sum3_steps, dim3(static_cast<uint32_t>(e - c)), dim3(256), 0, stream, vpart,
+                               static_cast<uint64_t>(max_n), step_leaves, c, step_sums);
+            check_launch("k_sum3_steps");
         }
-        hipLaunchKernelGGL(k_sum3_chunks, dim3(kReduceBlocks), dim3(256), 0, stream, vpart, vbase[steps], chunk);
+        hipLaunchKernelGGL(k_sum3_chunks, dim3(kReduceBlocks), dim3(256), 0, stream, step_sums, steps, chunk);
         check_launch("k_sum3_chunks");
         hipLaunchKernelGGL(k_sum3_final, dim3(1), dim3(256), 0, stream, chunk, acc);
         check_launch("k_sum3_final");
         return l;
     }
 
-    // Replays g, or captures body() into it first (no graph yet, or one made
-    // for another velocity).  body returns the buffer that holds the output.
+    // Replays g, or captures body() into it first.  body returns the buffer
+    // that holds the output.
     template <class F>
     void run_pass(pass_graph& g, F&& body)
     {
-        if (g.exec && std::memcmp(g.v, vel, sizeof(g.v)) == 0)
+        if (g.exec)
         {
             AMR_HIP_CHECK(hipGraphLaunch(g.exec, stream));
             return;
         }
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        g.exec           = nullptr;
         hipGraph_t graph = nullptr;
         AMR_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try
@@ -1436,7 +1486,6 @@ struct amr_tape
             g.exec = nullptr;
             throw amr_error(AMR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
-        std::memcpy(g.v, vel, sizeof(g.v));
         AMR_HIP_CHECK(hipGraphLaunch(g.exec, stream));
     }
 };

@@ -342,16 +342,18 @@ __host__ __device__ inline double adj_restrict_value(const double* __restrict__ 
 // ---------------------------------------------------------------------------
 
 // One adjoint step, one workgroup per leaf: lam = transpose(step)(lam_next)
-// and vpart[leaf][d] = sum over the leaf's cells of lam_next . du'/dv_d.
-// u is the state the forward step started from.
+// and vpart[leaf][d] = sum over the leaf's cells of lam_next . du'/dv_d.  u
+// is the state the forward step started from; the step's constants are read
+// from prm_p (device memory, so a captured launch picks up new values).
 template <int R, int N, int NT>
 __global__ __launch_bounds__(NT) void k_adj_step(const double* __restrict__ lam_next, const double* __restrict__ u,
                                                  const uint64_t* __restrict__ keys,
                                                  const face_link* __restrict__ links,
-                                                 const uint32_t* __restrict__ metas, adjoint_params prm,
-                                                 double* __restrict__ lam, double* __restrict__ vpart,
-                                                 uint32_t nleaves)
+                                                 const uint32_t* __restrict__ metas,
+                                                 const adjoint_params* __restrict__ prm_p, double* __restrict__ lam,
+                                                 double* __restrict__ vpart, uint32_t nleaves)
 {
+    const adjoint_params& prm = *prm_p;
     using S = patch_shape<R, N>;
     __shared__ double tile[tile_shape<R, N>::TS];
     __shared__ double lt[S::NI];
@@ -477,10 +479,12 @@ template <int R, int NT>
 __global__ __launch_bounds__(NT) void k_adj_step_box(box<R> B, const double* __restrict__ lam_next,
                                                      const double* __restrict__ u, const uint64_t* __restrict__ keys,
                                                      const face_link* __restrict__ links,
-                                                     const uint32_t* __restrict__ metas, adjoint_params prm,
+                                                     const uint32_t* __restrict__ metas,
+                                                     const adjoint_params* __restrict__ prm_p,
                                                      double* __restrict__ lam, double* __restrict__ vpart,
                                                      uint32_t nleaves)
 {
+    const adjoint_params& prm = *prm_p;
     constexpr int NW = NT / 64;
     static_assert(NT % 64 == 0, "whole waves");
     __shared__ double red[R][NW];
@@ -523,6 +527,31 @@ __global__ __launch_bounds__(NT) void k_adj_step_box(box<R> B, const double* __r
             }
             vpart[static_cast<size_t>(leaf) * 3 + d] = sum;
         }
+}
+
+// The velocity partials of a run of adjoint steps, one workgroup per step:
+// step first + b wrote its leaves' triples at vpart + b * stride * 3 (leaves[]
+// per step); they are summed in one fixed order (leaves t, t + 256, ... per
+// thread, a shuffle tree inside each wave, the waves in order) into that
+// step's slot of step_sums.
+__global__ __launch_bounds__(256) void k_sum3_steps(const double* __restrict__ vpart, uint64_t stride,
+                                                    const uint32_t* __restrict__ leaves, uint64_t first,
+                                                    double* __restrict__ step_sums)
+{
+    __shared__ double red[3][4];
+    const uint64_t    step = first + blockIdx.x;
+    const uint32_t    n    = leaves[step];
+    const double*     v    = vpart + blockIdx.x * stride * 3;
+    double            s[3] = { 0.0, 0.0, 0.0 };
+    for (uint32_t l = threadIdx.x; l < n; l += 256)
+        for (int d = 0; d < 3; ++d) s[d] += v[static_cast<size_t>(l) * 3 + d];
+    for (int d = 0; d < 3; ++d)
+        for (int o = 32; o > 0; o >>= 1) s[d] += __shfl_down(s[d], o, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < 3; ++d) red[d][threadIdx.x >> 6] = s[d];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) step_sums[step * 3 + d] = ((red[d][0] + red[d][1]) + red[d][2]) + red[d][3];
 }
 
 // Deterministic sum of `count` triples: stage 1 gives each of kReduceBlocks

@@ -14,10 +14,11 @@
  *   - key = (interleave(x, y[, z]) << 6) | level, x in bit 0 of each group
  *     (include/morton/morton_id.hpp:132-140, :350-359); leaves are kept
  *     sorted by key (include/ndtree/ndtree.hpp:1335-1414).
- *   - a patch is n^rank interior cells, row-major with the last index fastest;
- *     physical x is the last index (include/containers/static_layout.hpp:28-37,
+ *   - a patch is n_0 x .. x n_(rank-1) interior cells (amr_tree_config), row-
+ *     major with the last index fastest; physical x is the last index
+ *     (include/containers/static_layout.hpp:28-37,
  *     include/solver/physics_system.hpp:126-135).  The padded view adds
- *     halo_width ghost layers per side ((n + 2h)^rank doubles per patch).
+ *     halo_width ghost layers per side (prod (n_d + 2h) doubles per patch).
  *   - direction k addresses layout dim k/2, negative when k is even
  *     (include/ndtree/neighbor.hpp:156-211).
  */
@@ -331,9 +332,9 @@ int amr_tape_get_keys(const amr_tape* tape, uint64_t index, uint64_t* keys, uint
  * v).  Device pointers; the call waits for all prior device work before
  * reading and for its own work before returning.  Refreshes the checkpoints:
  * the next amr_tape_adjoint differentiates this pass.  The pass runs as a
- * hipGraph captured on the first call and on each call with a velocity other
- * than the previous call's (amr_tape_adjoint likewise); results are the same
- * bits either way. */
+ * hipGraph captured on the first call (amr_tape_adjoint likewise); the
+ * velocity-dependent constants are copied to the device before each replay,
+ * so a new velocity needs no new capture. */
 int amr_tape_forward(amr_tape* tape, const double* u0_device, const double velocity[3], double* uT_device);
 /* grad_u0 = (du(T)/du0)^T lambda_T and grad_velocity = (du(T)/dv)^T lambda_T
  * of the most recent forward pass (the recording or amr_tape_forward), with

@@ -132,10 +132,11 @@ def test_adjoint_dot_product_and_checkpoint_invariance(amr, torch):
 
 
 def test_pass_graphs_replay_bit_exact(amr, torch):
-    """The forward replay and the adjoint pass run as captured hipGraphs,
-    captured again when the velocity changes.  Every replay, with other
-    passes and the program's own eager work between the replays, gives the
-    bits of the first (captured) run."""
+    """The forward replay and the adjoint pass run as hipGraphs captured on
+    the first call; a new velocity only rewrites the per-step constants the
+    kernels read.  Every replay, with other passes (at the other velocity)
+    and the program's own eager work between the replays, gives the bits of
+    the first run at that velocity."""
     say("case: graph replays, 40 steps (regrid every 10, checkpoint 7), velocities (1, 0.5) / (0.7, -0.2)")
     tree, solver, tape, u0, _ = record_c5(amr, 40, 7)
     info = tape.info()
@@ -157,7 +158,7 @@ def test_pass_graphs_replay_bit_exact(amr, torch):
             got = both(v)
             for a, b in zip(got, first[v]):
                 assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (r, v)
-    # the two velocities give different results (the graphs are not reused across them)
+    # the two velocities give different results (the constants reach the replays)
     assert not np.array_equal(first[(1.0, 0.5)][0], first[(0.7, -0.2)][0])
     tape.close()
     tree.close()
