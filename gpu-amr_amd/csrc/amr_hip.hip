@@ -139,7 +139,7 @@ struct instance
     static constexpr bool        kWave           = (R == 3 && N == 8) || (R == 2 && N == 16);
     static int                   variant_count() { return kWave ? 6 : 1; }
     static char const*           variant_name(int v) { return kWave ? kVariantsWave[v] : kVariantsAny[v]; }
-    static int                   variant_needs(int v) { return (kWave && v >= 4) ? NEEDS_RUNS : 0; }
+    static int                   variant_needs(int v) { return (kWave && (v == 4 || v == 5)) ? NEEDS_RUNS : 0; }
 
     template <bool NTS>
     static void launch_wave(dim3 grid, unsigned lds, const double* u, double* un, const uint64_t* keys,
@@ -160,7 +160,7 @@ struct instance
         const dim3 per_leaf(xcd_grid(n));
         if constexpr (kWave)
         {
-            const int rk = variant - 4; // run list index of the run variants
+            const int rk = (variant == 4 || variant == 5) ? variant - 4 : -1; // run list of the run variants
             if (rk >= 0 && !aux.list[rk]) variant = 0; // no run list: wave-per-leaf kernel
             switch (variant)
             {
