@@ -451,6 +451,38 @@ def euler_section(amr, name="3d", cpu_cycles=6, threads=16):
     }
 
 
+def euler_uniform_section(amr, steps=20):
+    """The 3D Euler step on a uniform depth-5 octree (32768 leaves of 8^3,
+    domain 1000^3, the 3D benchmark's acoustic pulse): HIP-event time per
+    step of advance_batch_async(steps) on the context stream (the step
+    kernel, the per-step dt logic, and the dt reduction once per batch).
+    80 algorithmic bytes per cell-update (five FP64 fields read and
+    written)."""
+    from gpu_amr_amd import workloads
+
+    cfg = dict(workloads.EULER_CONFIGS["3d"])
+    s = amr.EulerSolver(3, 5, 8, 1, 32768 + 64, (1000.0,) * 3, 1.4, 0.3)
+    for _ in range(5):
+        s.tree.refine_all()
+    s.initialize(workloads.euler_ic(cfg))
+    cells = s.tree.size() * 512
+    s.advance_batch_async(3)
+    s.finish_advance_batch()
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    st = s.tree.stream()
+    ev.record(e0, st)
+    s.advance_batch_async(steps)
+    ev.record(e1, st)
+    _, ex = s.finish_advance_batch()
+    us = ev.elapsed_ms(e0, e1) / ex * 1e3
+    gbs = 80 * cells / (us * 1e-6) / 1e9
+    log(f"euler uniform 3d: {s.tree.size()} leaves, {ex} steps, {us:.1f} us per step, {gbs:.0f} GB/s algorithmic")
+    return {"workload": "3D Euler, uniform depth-5 octree, 32768 leaves of 8^3 FP64, acoustic pulse, domain 1000^3",
+            "steps": ex, "us_per_step": us, "cell_updates_per_s": cells / (us * 1e-6),
+            "algorithmic_GBps": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -468,6 +500,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--euler-only", nargs="?", const="all", choices=["all", "3d", "2d", "2d_active"],
                     help="only the Euler benchmarks (profiling): all, or one of them")
+    ap.add_argument("--euler-uniform", type=int, default=0, metavar="STEPS",
+                    help="only the 3D Euler step on the uniform 32768-leaf tree (profiling)")
     ap.add_argument("--no-native-oracle", action="store_true",
                     help="CPU baselines with the shipped oracle build instead of one compiled for this CPU")
     args = ap.parse_args()
@@ -495,6 +529,10 @@ def main():
 
     amr = package()
     from gpu_amr_amd import workloads
+
+    if args.euler_uniform:
+        print(json.dumps({"euler_uniform_3d": euler_uniform_section(amr, args.euler_uniform)}), flush=True)
+        return
 
     if args.euler_only:
         names = ["3d", "2d", "2d_active"] if args.euler_only == "all" else [args.euler_only]

@@ -232,41 +232,55 @@ __device__ inline void stage_fields(double* tile, const double* __restrict__ u,
     }
 }
 
-// compute_time_step_cpu's inner loop (amr_solver.hpp:380-400) for one cell
-// of cell sizes cs: dt = min(dt, cs[d] / speed_d) over the dims whose speed
-// exceeds 1e-12.
+// The wave speed |u_d| + a of one cell along each dim (getMaxSpeed,
+// EulerPhysics.hpp:132-155), folded into the running maxima smax.  A NaN
+// speed never enters (the comparison fails), as the reference's
+// `speed > 1e-12` test skips it.
 template <int R>
-__device__ inline double cell_dt(const double* U, const double* cs, double gamma, double dt)
+__device__ inline void cell_speeds(const double* U, double gamma, double* smax)
 {
     const auto st = euler_flux<R>::state_of(U, gamma);
     for (int d = 0; d < R; ++d)
     {
         const double speed = euler_flux<R>::max_speed(U, st, d);
-        if (speed > 1e-12)
-        {
-            const double q = cs[d] / speed;
-            dt             = (q < dt) ? q : dt;
-        }
+        smax[d]            = (speed > smax[d]) ? speed : smax[d];
     }
-    return dt;
 }
 
-// Workgroup minimum of the lanes' dt, folded into the launch-wide minimum.
-// Every lane of the workgroup must call it.
-template <int NT>
-__device__ inline void fold_dt_min(double dt, euler_scalars* __restrict__ sc)
+// compute_time_step_cpu's minimum (amr_solver.hpp:380-400) over one
+// workgroup's cells, which share a level and its cell sizes cs: the smallest
+// cs[d] / S_d over the dims whose largest speed S_d exceeds 1e-12.
+// Correctly rounded division is monotonic in the divisor, so cs[d] / S_d is
+// the smallest of the cells' cs[d] / speed -- the same double as the
+// per-cell minimum, for R divisions per workgroup instead of R per cell.
+// Folded into the launch-wide minimum; every lane of the workgroup must call
+// it.
+template <int R, int NT>
+__device__ inline void fold_speed_max(double* smax, const double* cs, euler_scalars* __restrict__ sc)
 {
-    __shared__ double part[NT / 64];
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        const double o = __shfl_down(dt, off, 64);
-        dt             = (o < dt) ? o : dt;
-    }
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = dt;
+    __shared__ double part[NT / 64][R];
+    for (int d = 0; d < R; ++d)
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            const double o = __shfl_down(smax[d], off, 64);
+            smax[d]        = (o > smax[d]) ? o : smax[d];
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < R; ++d) part[threadIdx.x >> 6][d] = smax[d];
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        for (int w = 1; w < NT / 64; ++w) dt = (part[w] < dt) ? part[w] : dt;
+        double dt = 1.7976931348623157e308;
+        for (int d = 0; d < R; ++d)
+        {
+            double sd = part[0][d];
+            for (int w = 1; w < NT / 64; ++w) sd = (part[w][d] > sd) ? part[w][d] : sd;
+            if (sd > 1e-12)
+            {
+                const double q = cs[d] / sd;
+                dt             = (q < dt) ? q : dt;
+            }
+        }
         // non-negative doubles order like their bit patterns; most
         // workgroups do not lower the minimum and skip the atomic (a stale
         // read only costs an extra one), which keeps thousands of
@@ -289,14 +303,15 @@ __device__ inline void euler_dt_leaf(box<R> const& B, const double* __restrict__
     constexpr int NV = R + 2;
     const int     ni = B.ni();
     const int     lv = key_level(keys[leaf]);
-    double        dt = 1.7976931348623157e308;
+    double        smax[R];
+    for (int d = 0; d < R; ++d) smax[d] = 0.0;
     for (int i = threadIdx.x; i < ni; i += NT)
     {
         double U[NV];
         for (int v = 0; v < NV; ++v) U[v] = u[v * fstride + static_cast<size_t>(leaf) * ni + i];
-        dt = cell_dt<R>(U, prm.cs[lv], prm.gamma, dt);
+        cell_speeds<R>(U, prm.gamma, smax);
     }
-    fold_dt_min<NT>(dt, sc);
+    fold_speed_max<R, NT>(smax, prm.cs[lv], sc);
 }
 
 template <int R, int N, int NT>
@@ -384,7 +399,8 @@ __device__ inline void euler_step_leaf(box<R> const& B, const double* __restrict
     const int lv = key_level(keys[leaf]);
     double    dtdx[R];
     for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
-    double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
+    double smax[R]; // next step's largest speeds over this workgroup's cells
+    for (int d = 0; d < R; ++d) smax[d] = 0.0;
     for (int i = threadIdx.x; i < ni; i += NT)
     {
         int c[R];
@@ -415,9 +431,9 @@ __device__ inline void euler_step_leaf(box<R> const& B, const double* __restrict
             o[v]                       = Uc[v] + total[v];
             un[v * fstride + base + i] = o[v];
         }
-        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
+        if (prm.next_dt) cell_speeds<R>(o, prm.gamma, smax);
     }
-    if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
+    if (prm.next_dt) fold_speed_max<R, NT>(smax, prm.cs[key_level(keys[leaf])], sc);
 }
 
 template <int R, int N, int NT>
@@ -476,7 +492,8 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
     const int lv = key_level(keys[leaf]);
     double    dtdx[R];
     for (int d = 0; d < R; ++d) dtdx[d] = dt / prm.cs[lv][d];
-    double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
+    double smax[R]; // next step's largest speeds over this workgroup's cells
+    for (int d = 0; d < R; ++d) smax[d] = 0.0;
     for (int i = threadIdx.x; i < S::NI; i += NT)
     {
         int c[R];
@@ -508,9 +525,9 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
             o[v]                              = Uc[v] + total[v];
             un[v * fstride + base + i] = o[v];
         }
-        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
+        if (prm.next_dt) cell_speeds<R>(o, prm.gamma, smax);
     }
-    if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
+    if (prm.next_dt) fold_speed_max<R, NT>(smax, prm.cs[key_level(keys[leaf])], sc);
 }
 
 // Strip of B layers (along layout dim 0, the slowest) of an n^R patch with
@@ -569,11 +586,11 @@ __device__ inline int strip_cell(int q, int r0, int* c)
 
 // The update of one staged strip (tile and LDS states formed): every cell's
 // Rusanov update as in k_euler_step_tile, stored to the next state, and the
-// next step's dt minimum over the new values folded into dtn.
+// new values' wave speeds folded into smax (the next step's dt).
 template <int R, int N, int B, int NT>
 __device__ inline void strip_update(const double (*tile)[strip_shape<R, N, B>::TS], const double* st_inv,
                                     const double* st_p, const double* st_a, int r0, size_t first, int lv, double dt,
-                                    euler_params const& prm, double* __restrict__ un, size_t fstride, double& dtn)
+                                    euler_params const& prm, double* __restrict__ un, size_t fstride, double* smax)
 {
     using T          = strip_shape<R, N, B>;
     using F          = euler_flux<R>;
@@ -618,7 +635,7 @@ __device__ inline void strip_update(const double (*tile)[strip_shape<R, N, B>::T
             o[v]                     = Uc[v] + total[v];
             un[v * fstride + first + i] = o[v];
         }
-        if (prm.next_dt) dtn = cell_dt<R>(o, prm.cs[lv], prm.gamma, dtn);
+        if (prm.next_dt) cell_speeds<R>(o, prm.gamma, smax);
     }
 }
 
@@ -694,9 +711,10 @@ __global__ __launch_bounds__(NT) void k_euler_step_strip(const double* __restric
     __syncthreads();
     strip_states<R, N, B, NT>(tile, st_inv, st_p, st_a, prm.gamma);
     __syncthreads();
-    double dtn = 1.7976931348623157e308; // next step's dt minimum over this workgroup's cells
-    strip_update<R, N, B, NT>(tile, st_inv, st_p, st_a, r0, first, key_level(keys[leaf]), dt, prm, un, fstride, dtn);
-    if (prm.next_dt) fold_dt_min<NT>(dtn, sc);
+    double smax[R]; // next step's largest speeds over this workgroup's cells
+    for (int d = 0; d < R; ++d) smax[d] = 0.0;
+    strip_update<R, N, B, NT>(tile, st_inv, st_p, st_a, r0, first, key_level(keys[leaf]), dt, prm, un, fstride, smax);
+    if (prm.next_dt) fold_speed_max<R, NT>(smax, prm.cs[key_level(keys[leaf])], sc);
 }
 
 // Largest strip depth B (a divisor of n) whose strip tiles of the R + 2
