@@ -264,11 +264,15 @@ struct instance
                            hipStream_t s)
     {
         if (n == 0) return;
-        // LDS tiles of every field where they fit, else LDS strips of the
-        // patch, else neighbours from memory
+        // 3D 8^3: one wave per leaf with the column in registers; other
+        // shapes: LDS tiles of every field where they fit, else LDS strips of
+        // the patch, else neighbours from memory
         constexpr size_t tile_bytes = (R + 2) * tile_shape<R, N>::TS * sizeof(double);
         constexpr int    B          = euler_strip_depth<R, N>(40 * 1024);
-        if constexpr (tile_bytes <= 48 * 1024)
+        if constexpr (R == 3 && N == 8)
+            hipLaunchKernelGGL((k_euler_step_3d8<false>), dim3(xcd_grid(n)), dim3(64), 0, s, u, un, fs, keys, links,
+                               metas, prm, sc, n);
+        else if constexpr (tile_bytes <= 48 * 1024)
             hipLaunchKernelGGL((k_euler_step_tile<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links,
                                metas, prm, sc, n);
         else if constexpr (B > 0)

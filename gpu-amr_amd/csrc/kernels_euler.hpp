@@ -530,6 +530,211 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
     if (prm.next_dt) fold_speed_max<R, NT>(smax, prm.cs[key_level(keys[leaf])], sc);
 }
 
+// The five conservative values and the state of one cell moved across
+// lanes as 64-bit DPP rows (lane_plus1) or LDS permutes (__shfl_down by 8),
+// every lane active.
+struct euler_cell3
+{
+    double U[5];
+    double inv_rho, p, a;
+};
+
+__device__ inline euler_cell3 cell_lane_plus1(euler_cell3 const& c)
+{
+    euler_cell3 o;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) o.U[v] = lane_plus1(c.U[v]);
+    o.inv_rho = lane_plus1(c.inv_rho);
+    o.p       = lane_plus1(c.p);
+    o.a       = lane_plus1(c.a);
+    return o;
+}
+
+__device__ inline euler_cell3 cell_lane_down8(euler_cell3 const& c)
+{
+    euler_cell3 o;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) o.U[v] = __shfl_down(c.U[v], 8, 64);
+    o.inv_rho = __shfl_down(c.inv_rho, 8, 64);
+    o.p       = __shfl_down(c.p, 8, 64);
+    o.a       = __shfl_down(c.a, 8, 64);
+    return o;
+}
+
+// 3D 8^3 patches, one wave per leaf, lane l = 8 y + x owning the z-column
+// (y, x), walked z = 0..7 with the column's cells in registers: each cell's
+// state (1/rho, p, a) is formed once, and each face's Rusanov flux is
+// evaluated once -- by the cell on its low side, from that cell's and its +x
+// (next lane), +y (lane + 8) or +z (next plane) neighbour's values and
+// states -- and handed to the cell on its high side (the previous lane, lane
+// - 8, the next plane).  The fluxes through the patch's x and y faces are
+// evaluated in a face-layout pass first (lane = 8 z + q: the ghost cell by
+// the face rules, the boundary cell, their states, the flux) and read from
+// LDS by the boundary lanes; the z faces' ghosts are the lane's own.  Every
+// flux and state is formed from the same operands by the same expressions
+// as in k_euler_step_tile, so the results are the same bits, for about half
+// of its VALU instructions (the tile kernel forms each cell's state seven
+// times and each face's flux twice).
+template <bool NTS>
+__global__ __launch_bounds__(64) void k_euler_step_3d8(const double* __restrict__ u, double* __restrict__ un,
+                                                       size_t fstride, const uint64_t* __restrict__ keys,
+                                                       const face_link* __restrict__ links,
+                                                       const uint32_t* __restrict__ metas, euler_params prm,
+                                                       euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    using F            = euler_flux<3>;
+    constexpr int NV   = 5;
+    constexpr int NI   = 512;
+    __shared__ double fb[4][NV][64]; // boundary fluxes -y, +y (index z*8+x), -x, +x (index z*8+y)
+    const uint32_t    leaf = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    if (leaf >= nleaves) return;
+    const int    lane = threadIdx.x, y = lane >> 3, x = lane & 7;
+    const double dt   = sc->step_dt;
+    const size_t base = static_cast<size_t>(leaf) * NI;
+    if (!(dt > 0.0))
+    {
+        for (int i = lane; i < NI; i += 64)
+            for (int v = 0; v < NV; ++v) un[v * fstride + base + i] = u[v * fstride + base + i];
+        return;
+    }
+    const face_link* L     = links + static_cast<size_t>(leaf) * 6;
+    const uint32_t*  M     = metas + static_cast<size_t>(leaf) * 6;
+    const double     gamma = prm.gamma;
+
+    // ---- boundary fluxes of the x and y faces (face layout p = z, q) -------
+    {
+        const int p = lane >> 3, q = lane & 7;
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+        {
+            const int  dir = 2 + f;            // -y, +y, -x, +x
+            const bool pos = (dir & 1) != 0;
+            const int  d   = (f < 2) ? 1 : 0;  // physical axis of the face normal
+            int        c[3], cb[3];
+            if (f < 2)
+            {
+                c[0] = p, c[1] = pos ? 8 : -1, c[2] = q;
+                cb[0] = p, cb[1] = pos ? 7 : 0, cb[2] = q;
+            }
+            else
+            {
+                c[0] = p, c[1] = q, c[2] = pos ? 8 : -1;
+                cb[0] = p, cb[1] = q, cb[2] = pos ? 7 : 0;
+            }
+            double G[NV], C[NV], fl[NV];
+            ghost_values<3, 8, NV>(u, fstride, L[dir], M[dir], dir, c, G);
+            const size_t at = base + patch_shape<3, 8>::lin(cb);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) C[v] = u[v * fstride + at];
+            const auto sg = F::state_of(G, gamma), sb = F::state_of(C, gamma);
+            if (pos)
+                F::rusanov(C, sb, G, sg, fl, d);
+            else
+                F::rusanov(G, sg, C, sb, fl, d);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) fb[f][v][lane] = fl[v];
+        }
+    }
+
+    // ---- the column, z = 0..7 ----------------------------------------------
+    const int lv      = key_level(keys[leaf]);
+    double    dtdx[3] = { dt / prm.cs[lv][0], dt / prm.cs[lv][1], dt / prm.cs[lv][2] };
+    double    smax[3] = { 0.0, 0.0, 0.0 };
+    euler_cell3 cur, nxt;
+    double      pre[NV]; // the plane after the next one, loaded an iteration ahead
+    {
+        double G[NV];
+        const int c[3] = { -1, y, x };
+        ghost_values<3, 8, NV>(u, fstride, L[0], M[0], 0, c, G);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+        {
+            cur.U[v] = u[v * fstride + base + lane];
+            pre[v]   = u[v * fstride + base + 64 + lane];
+        }
+        const auto st = F::state_of(cur.U, gamma);
+        cur.inv_rho = st.inv_rho, cur.p = st.p, cur.a = st.a;
+        const auto sg = F::state_of(G, gamma);
+        double     fz[NV];
+        F::rusanov(G, sg, cur.U, st, fz, 2);
+        __syncthreads(); // the boundary fluxes
+        double* out = un + base;
+#pragma unroll 1
+        for (int z = 0; z < 8; ++z)
+        {
+            const typename F::state sc_{ cur.inv_rho, cur.p, cur.a };
+            // total[k] -= dt/dx_d (fR[k] - fL[k]) for d = x, y, z in that
+            // order, each axis's fluxes dead before the next axis's are formed
+            double total[NV];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) total[k] = 0.0;
+            {
+                // x: the +x face from this lane and the next, the -x face from the previous lane
+                const euler_cell3 xc = cell_lane_plus1(cur);
+                double            fRx[NV];
+                F::rusanov(cur.U, sc_, xc.U, typename F::state{ xc.inv_rho, xc.p, xc.a }, fRx, 0);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fRx[v] = (x == 7) ? fb[3][v][z * 8 + y] : fRx[v];
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                {
+                    const double m1  = lane_minus1(fRx[v]);
+                    const double fLx = (x == 0) ? fb[2][v][z * 8 + y] : m1;
+                    total[v] -= dtdx[0] * (fRx[v] - fLx);
+                }
+            }
+            {
+                // y: the +y face from this lane and lane + 8, the -y face from lane - 8
+                const euler_cell3 yc = cell_lane_down8(cur);
+                double            fRy[NV];
+                F::rusanov(cur.U, sc_, yc.U, typename F::state{ yc.inv_rho, yc.p, yc.a }, fRy, 1);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fRy[v] = (y == 7) ? fb[1][v][z * 8 + x] : fRy[v];
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                {
+                    const double m8  = __shfl_up(fRy[v], 8, 64);
+                    const double fLy = (y == 0) ? fb[0][v][z * 8 + x] : m8;
+                    total[v] -= dtdx[1] * (fRy[v] - fLy);
+                }
+            }
+            // z: the +z neighbour is the next plane (prefetched) or the +z ghost
+            if (z < 7)
+            {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) nxt.U[v] = pre[v];
+                if (z < 6)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) pre[v] = u[v * fstride + base + (z + 2) * 64 + lane];
+            }
+            else
+            {
+                const int c8[3] = { 8, y, x };
+                ghost_values<3, 8, NV>(u, fstride, L[1], M[1], 1, c8, nxt.U);
+            }
+            {
+                const auto st = F::state_of(nxt.U, gamma);
+                nxt.inv_rho = st.inv_rho, nxt.p = st.p, nxt.a = st.a;
+            }
+            double fzr[NV];
+            F::rusanov(cur.U, sc_, nxt.U, typename F::state{ nxt.inv_rho, nxt.p, nxt.a }, fzr, 2);
+            double o[NV];
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+            {
+                total[k] -= dtdx[2] * (fzr[k] - fz[k]);
+                o[k] = cur.U[k] + total[k];
+                store_cell<NTS>(out + k * fstride + z * 64 + lane, o[k]);
+            }
+            if (prm.next_dt) cell_speeds<3>(o, gamma, smax);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) fz[v] = fzr[v];
+            cur = nxt;
+        }
+    }
+    if (prm.next_dt) fold_speed_max<3, 64>(smax, prm.cs[lv], sc);
+}
+
 // Strip of B layers (along layout dim 0, the slowest) of an n^R patch with
 // one ghost layer all round: (B + 2) * (n + 2)^(R-1) LDS cells per field.
 template <int R, int N, int B>
