@@ -314,13 +314,13 @@ __global__ __launch_bounds__(64 * W) void k_step2_3d8_uniform(const double* __re
             double       fLy = __shfl_up(fRy, 8, 64);
             if (k >= 1 && k <= 8)
             {
-                const int    z  = k - 1;
-                const double gx = fbound[x < 4 ? 2 : 3][z * 8 + y];
-                const double gy = fbound[y < 4 ? 0 : 1][z * 8 + x];
-                fLx             = (x == 0) ? gx : fLx;
-                fRx             = (x == 7) ? gx : fRx;
-                fLy             = (y == 0) ? gy : fLy;
-                fRy             = (y == 7) ? gy : fRy;
+                const int     z  = k - 1;
+                const double* gx = &fbound[x < 4 ? 2 : 3][z * 8 + y];
+                const double* gy = &fbound[y < 4 ? 0 : 1][z * 8 + x];
+                take_lds_if(x == 0, gx, fLx);
+                take_lds_if(x == 7, gx, fRx);
+                take_lds_if(y == 0, gy, fLy);
+                take_lds_if(y == 7, gy, fRy);
             }
             else
             {
@@ -380,12 +380,12 @@ __global__ __launch_bounds__(64 * W) void k_step2_3d8_uniform(const double* __re
         double       fLx   = lane_minus1(fRx);
         double       fRy   = face_flux(uc, __shfl_down(uc, 8, 64), v[1], h[1]);
         double       fLy   = __shfl_up(fRy, 8, 64);
-        const double gx    = fbound[x < 4 ? 2 : 3][z * 8 + y];
-        const double gy    = fbound[y < 4 ? 0 : 1][z * 8 + x];
-        fLx                = (x == 0) ? gx : fLx;
-        fRx                = (x == 7) ? gx : fRx;
-        fLy                = (y == 0) ? gy : fLy;
-        fRy                = (y == 7) ? gy : fRy;
+        const double* gx   = &fbound[x < 4 ? 2 : 3][z * 8 + y];
+        const double* gy   = &fbound[y < 4 ? 0 : 1][z * 8 + x];
+        take_lds_if(x == 0, gx, fLx);
+        take_lds_if(x == 7, gx, fRx);
+        take_lds_if(y == 0, gy, fLy);
+        take_lds_if(y == 7, gy, fRy);
         const double fzr   = face_flux(uc, b[2 + z], v[2], h[2]);
         const double fL[3] = { fLx, fLy, fz };
         const double fR[3] = { fRx, fRy, fzr };

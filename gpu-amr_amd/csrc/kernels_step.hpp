@@ -372,6 +372,18 @@ __device__ inline double lane_minus1(double v)
     return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(h)) << 32) | static_cast<uint32_t>(l));
 }
 
+// v = *p in the lanes where `take` holds, p pointing into LDS: a volatile
+// LDS read inside the branch, so the compiler keeps it a ds_read under the
+// lane mask (the other lanes keep v) instead of a read by every lane and a
+// 64-bit select (two VALU instructions).  Used where a boundary lane
+// replaces a cross-lane value with one from LDS; the LDS writes it reads
+// must be ordered before it by a barrier or leaf_sync.
+__device__ inline void take_lds_if(bool take, const double* p, double& v)
+{
+    typedef __attribute__((address_space(3))) const volatile double lds_double;
+    if (__builtin_expect(take, 1)) v = *(lds_double*)(p);
+}
+
 __device__ inline double lane_plus1(double v)
 {
     const uint64_t b  = __builtin_bit_cast(uint64_t, v);
