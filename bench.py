@@ -451,21 +451,22 @@ def euler_section(amr, name="3d", cpu_cycles=6, threads=16):
     }
 
 
-def euler_uniform_section(amr, steps=20):
-    """The 3D Euler step on a uniform depth-5 octree (32768 leaves of 8^3,
-    domain 1000^3, the 3D benchmark's acoustic pulse): HIP-event time per
-    step of advance_batch_async(steps) on the context stream (the step
-    kernel, the per-step dt logic, and the dt reduction once per batch).
-    80 algorithmic bytes per cell-update (five FP64 fields read and
-    written)."""
+def euler_uniform_section(amr, steps=20, rank=3):
+    """The Euler step on a uniform tree: 3D, a depth-5 octree (32768 leaves
+    of 8^3, domain 1000^3); 2D, a depth-5 quadtree (1024 leaves of 64^2,
+    domain 1000^2); the benchmarks' acoustic pulse.  HIP-event time per step
+    of advance_batch_async(steps) on the context stream (the step kernel, the
+    per-step dt logic, and the dt reduction once per batch).  16 B x (R + 2)
+    algorithmic bytes per cell-update (every field read and written)."""
     from gpu_amr_amd import workloads
 
-    cfg = dict(workloads.EULER_CONFIGS["3d"])
-    s = amr.EulerSolver(3, 5, 8, 1, 32768 + 64, (1000.0,) * 3, 1.4, 0.3)
+    cfg = dict(workloads.EULER_CONFIGS["3d" if rank == 3 else "2d"])
+    patch, leaves = (8, 32768) if rank == 3 else (64, 1024)
+    s = amr.EulerSolver(rank, 5, patch, 1, leaves + 64, (1000.0,) * rank, 1.4, 0.3)
     for _ in range(5):
         s.tree.refine_all()
     s.initialize(workloads.euler_ic(cfg))
-    cells = s.tree.size() * 512
+    cells = s.tree.size() * patch**rank
     s.advance_batch_async(3)
     s.finish_advance_batch()
     ev = HipEvents()
@@ -476,9 +477,12 @@ def euler_uniform_section(amr, steps=20):
     ev.record(e1, st)
     _, ex = s.finish_advance_batch()
     us = ev.elapsed_ms(e0, e1) / ex * 1e3
-    gbs = 80 * cells / (us * 1e-6) / 1e9
-    log(f"euler uniform 3d: {s.tree.size()} leaves, {ex} steps, {us:.1f} us per step, {gbs:.0f} GB/s algorithmic")
-    return {"workload": "3D Euler, uniform depth-5 octree, 32768 leaves of 8^3 FP64, acoustic pulse, domain 1000^3",
+    gbs = 16 * (rank + 2) * cells / (us * 1e-6) / 1e9
+    log(f"euler uniform {rank}d: {s.tree.size()} leaves, {ex} steps, {us:.1f} us per step, {gbs:.0f} GB/s algorithmic")
+    shape = f"{patch}^{rank}"
+    tree = "octree" if rank == 3 else "quadtree"
+    return {"workload": f"{rank}D Euler, uniform depth-5 {tree}, {s.tree.size()} leaves of {shape} FP64, acoustic pulse, "
+                        f"domain 1000^{rank}",
             "steps": ex, "us_per_step": us, "cell_updates_per_s": cells / (us * 1e-6),
             "algorithmic_GBps": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
 
@@ -502,6 +506,8 @@ def main():
                     help="only the Euler benchmarks (profiling): all, or one of them")
     ap.add_argument("--euler-uniform", type=int, default=0, metavar="STEPS",
                     help="only the 3D Euler step on the uniform 32768-leaf tree (profiling)")
+    ap.add_argument("--euler-uniform-2d", type=int, default=0, metavar="STEPS",
+                    help="only the 2D Euler step on the uniform 1024-leaf 64^2 tree (profiling)")
     ap.add_argument("--no-native-oracle", action="store_true",
                     help="CPU baselines with the shipped oracle build instead of one compiled for this CPU")
     args = ap.parse_args()
@@ -530,8 +536,11 @@ def main():
     amr = package()
     from gpu_amr_amd import workloads
 
-    if args.euler_uniform:
-        print(json.dumps({"euler_uniform_3d": euler_uniform_section(amr, args.euler_uniform)}), flush=True)
+    if args.euler_uniform or args.euler_uniform_2d:
+        if args.euler_uniform:
+            print(json.dumps({"euler_uniform_3d": euler_uniform_section(amr, args.euler_uniform)}), flush=True)
+        if args.euler_uniform_2d:
+            print(json.dumps({"euler_uniform_2d": euler_uniform_section(amr, args.euler_uniform_2d, 2)}), flush=True)
         return
 
     if args.euler_only:

@@ -64,6 +64,9 @@ constexpr int block_threads(int cells)
 // Per-launch extras of the step kernels: the +x run lists of the table
 // (runs of at most kRunK[k] leaves).  A run variant without its list falls
 // back to the wave-per-leaf kernel.
+// rows per wave of the 2D 64^2 Euler step (k_euler_step_2d64): 16 measured
+// best on the reference's 2D benchmarks (8 and 32 within 3 %, 4 12 % slower)
+constexpr int      kEuler2d64Rows   = 16;
 constexpr int      kRunLists        = 2;
 constexpr uint32_t kRunK[kRunLists] = { 4, 8 };
 struct step_aux
@@ -272,6 +275,9 @@ struct instance
         if constexpr (R == 3 && N == 8)
             hipLaunchKernelGGL((k_euler_step_3d8<false>), dim3(xcd_grid(n)), dim3(64), 0, s, u, un, fs, keys, links,
                                metas, prm, sc, n);
+        else if constexpr (R == 2 && N == 64)
+            hipLaunchKernelGGL((k_euler_step_2d64<kEuler2d64Rows>), dim3(xcd_grid(static_cast<uint64_t>(n) * (64 / kEuler2d64Rows))),
+                               dim3(64), 0, s, u, un, fs, keys, links, metas, prm, sc, n);
         else if constexpr (tile_bytes <= 48 * 1024)
             hipLaunchKernelGGL((k_euler_step_tile<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, u, un, fs, keys, links,
                                metas, prm, sc, n);

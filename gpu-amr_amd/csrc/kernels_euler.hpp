@@ -735,6 +735,200 @@ __global__ __launch_bounds__(64) void k_euler_step_3d8(const double* __restrict_
     if (prm.next_dt) fold_speed_max<3, 64>(smax, prm.cs[lv], sc);
 }
 
+// The value of lane l + 1 (wave_shl) or l - 1 (wave_shr) across the whole
+// wave; the lane whose source lies outside the wave (63, resp. 0) takes
+// `fill`.  Every lane must be active.
+__device__ inline double wave_plus1_fill(double v, double fill)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v), f = __builtin_bit_cast(uint64_t, fill);
+    const int      l = __builtin_amdgcn_update_dpp(static_cast<int>(f), static_cast<int>(b), 0x130, 0xF, 0xF, false);
+    const int      h =
+        __builtin_amdgcn_update_dpp(static_cast<int>(f >> 32), static_cast<int>(b >> 32), 0x130, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(h)) << 32) | static_cast<uint32_t>(l));
+}
+
+__device__ inline double wave_minus1_fill(double v, double fill)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v), f = __builtin_bit_cast(uint64_t, fill);
+    const int      l = __builtin_amdgcn_update_dpp(static_cast<int>(f), static_cast<int>(b), 0x138, 0xF, 0xF, false);
+    const int      h =
+        __builtin_amdgcn_update_dpp(static_cast<int>(f >> 32), static_cast<int>(b >> 32), 0x138, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(h)) << 32) | static_cast<uint32_t>(l));
+}
+
+// The four values and the state of one 2D cell.
+struct euler_cell2
+{
+    double U[4];
+    double inv_rho, p, a;
+};
+
+// 2D 64^2 patches (the reference's 2D benchmark shape): one wave per strip
+// of B rows, lane = x, the strip's rows walked y = r0 .. r0 + B - 1 with the
+// row's cells in registers.  Each cell's state is formed once and each face
+// flux is evaluated once: the +x flux by the cell on the face's low side from
+// its own values and state and those of the next lane (a wave-wide DPP
+// shift; lane 63 takes the +x ghost cell as the shift's fill), handed to the
+// next lane as its -x flux (lane 0 takes the -x boundary flux as the fill);
+// the +y flux of a row is the -y flux of the next row.  The -x boundary
+// fluxes and the +x ghost cells of the strip's rows are formed first, one
+// row per lane.  The fluxes and states come from the same operands by the
+// same expressions as in k_euler_step_tile, so the results are the same
+// bits.
+template <int B>
+__global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict__ u, double* __restrict__ un,
+                                                        size_t fstride, const uint64_t* __restrict__ keys,
+                                                        const face_link* __restrict__ links,
+                                                        const uint32_t* __restrict__ metas, euler_params prm,
+                                                        euler_scalars* __restrict__ sc, uint32_t nleaves)
+{
+    static_assert(B >= 2 && 64 % B == 0 && 2 * B <= 64, "strips of at least two rows");
+    using F           = euler_flux<2>;
+    constexpr int NV  = 4;
+    constexpr int N   = 64;
+    constexpr int NI  = N * N;
+    constexpr int NS  = N / B;
+    __shared__ double xg[NV + 3][B]; // +x ghost cell of each strip row: values, 1/rho, p, a
+    __shared__ double xf[NV][B];     // -x boundary flux of each strip row
+    const uint32_t    item = sweep_item(blockIdx.x, gridDim.x, prm.reverse);
+    const uint32_t    leaf = item / NS;
+    if (leaf >= nleaves) return;
+    const int    r0   = static_cast<int>(item % NS) * B;
+    const int    x    = threadIdx.x;
+    const double dt   = sc->step_dt;
+    const size_t base = static_cast<size_t>(leaf) * NI;
+    if (!(dt > 0.0))
+    {
+        for (int k = 0; k < B; ++k)
+            for (int v = 0; v < NV; ++v)
+                un[v * fstride + base + (r0 + k) * N + x] = u[v * fstride + base + (r0 + k) * N + x];
+        return;
+    }
+    const face_link* L     = links + static_cast<size_t>(leaf) * 4;
+    const uint32_t*  M     = metas + static_cast<size_t>(leaf) * 4;
+    const double     gamma = prm.gamma;
+    euler_cell2      cur, nxt;
+    double           pre[NV];   // the row after the next one, loaded an iteration ahead
+    double           below[NV]; // the row under the strip
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+    {
+        cur.U[v] = u[v * fstride + base + r0 * N + x];
+        pre[v]   = u[v * fstride + base + (r0 + 1) * N + x];
+    }
+    if (r0 > 0)
+    {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) below[v] = u[v * fstride + base + (r0 - 1) * N + x];
+    }
+    else
+    {
+        const int c[2] = { -1, x };
+        ghost_values<2, N, NV>(u, fstride, L[0], M[0], 0, c, below);
+    }
+
+    // ---- x boundary, one strip row per lane: lanes 0 .. B-1 the -x flux,
+    // lanes B .. 2B-1 the +x ghost cell and its state
+    if (x < 2 * B)
+    {
+        const int side = x >= B ? 1 : 0, k = x - side * B, row = r0 + k, dir = 2 + side;
+        const int c[2] = { row, side ? N : -1 };
+        double    G[NV];
+        ghost_values<2, N, NV>(u, fstride, L[dir], M[dir], dir, c, G);
+        const auto sg = F::state_of(G, gamma);
+        if (side)
+        {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xg[v][k] = G[v];
+            xg[NV][k] = sg.inv_rho, xg[NV + 1][k] = sg.p, xg[NV + 2][k] = sg.a;
+        }
+        else
+        {
+            double       C[NV], fl[NV];
+            const size_t at = base + static_cast<size_t>(row) * N;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) C[v] = u[v * fstride + at];
+            F::rusanov(G, sg, C, F::state_of(C, gamma), fl, 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xf[v][k] = fl[v];
+        }
+    }
+
+    // ---- the strip's rows ---------------------------------------------------
+    const int    lv      = key_level(keys[leaf]);
+    const double dtdx[2] = { dt / prm.cs[lv][0], dt / prm.cs[lv][1] };
+    double       smax[2] = { 0.0, 0.0 };
+    double       fy[NV]; // the -y flux of the current row
+    {
+        const auto st = F::state_of(cur.U, gamma);
+        cur.inv_rho = st.inv_rho, cur.p = st.p, cur.a = st.a;
+        F::rusanov(below, F::state_of(below, gamma), cur.U, st, fy, 1);
+    }
+    __syncthreads(); // the x boundary values
+    double* out = un + base;
+#pragma unroll 1
+    for (int k = 0; k < B; ++k)
+    {
+        const int               y = r0 + k;
+        const typename F::state sc_{ cur.inv_rho, cur.p, cur.a };
+        // total[v] -= dt/dx_d (fR[v] - fL[v]) for d = x, then y
+        double total[NV];
+        {
+            euler_cell2 xc;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xc.U[v] = wave_plus1_fill(cur.U[v], xg[v][k]);
+            xc.inv_rho = wave_plus1_fill(cur.inv_rho, xg[NV][k]);
+            xc.p       = wave_plus1_fill(cur.p, xg[NV + 1][k]);
+            xc.a       = wave_plus1_fill(cur.a, xg[NV + 2][k]);
+            double fRx[NV];
+            F::rusanov(cur.U, sc_, xc.U, typename F::state{ xc.inv_rho, xc.p, xc.a }, fRx, 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+            {
+                const double fLx = wave_minus1_fill(fRx[v], xf[v][k]);
+                total[v]         = 0.0;
+                total[v] -= dtdx[0] * (fRx[v] - fLx);
+            }
+        }
+        // the next row: prefetched; the row after it is the patch's or, past
+        // the patch's last row, the +y ghost row
+#pragma unroll
+        for (int v = 0; v < NV; ++v) nxt.U[v] = pre[v];
+        if (k + 2 <= B)
+        {
+            if (y + 2 < N)
+            {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) pre[v] = u[v * fstride + base + (y + 2) * N + x];
+            }
+            else
+            {
+                const int c[2] = { N, x };
+                ghost_values<2, N, NV>(u, fstride, L[1], M[1], 1, c, pre);
+            }
+        }
+        {
+            const auto st = F::state_of(nxt.U, gamma);
+            nxt.inv_rho = st.inv_rho, nxt.p = st.p, nxt.a = st.a;
+        }
+        double fyr[NV];
+        F::rusanov(cur.U, sc_, nxt.U, typename F::state{ nxt.inv_rho, nxt.p, nxt.a }, fyr, 1);
+        double o[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+        {
+            total[v] -= dtdx[1] * (fyr[v] - fy[v]);
+            o[v] = cur.U[v] + total[v];
+            out[v * fstride + y * N + x] = o[v];
+        }
+        if (prm.next_dt) cell_speeds<2>(o, gamma, smax);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fy[v] = fyr[v];
+        cur = nxt;
+    }
+    if (prm.next_dt) fold_speed_max<2, 64>(smax, prm.cs[lv], sc);
+}
+
 // Strip of B layers (along layout dim 0, the slowest) of an n^R patch with
 // one ghost layer all round: (B + 2) * (n + 2)^(R-1) LDS cells per field.
 template <int R, int N, int B>
