@@ -573,6 +573,11 @@ def main():
         s.set_step_fusion(False)
         for v, name in enumerate(names):
             s.set_step_variant(v)
+            # untimed steps first: right after a switch of kernel the first
+            # ten or so launches run up to 30 % slower (a transient, not the
+            # variant's steady rate)
+            s.advance_batch_async(20)
+            s.finish_advance_batch()
             g.synchronize()
             ev.record(e0, stream)
             s.advance_batch_async(20)
