@@ -134,7 +134,8 @@ def test_euler_remaining_time(amr):
     assert np.array_equal(s.tree.get_interior().view(np.uint64), before.view(np.uint64))
 
 
-@pytest.mark.parametrize("rank,n,depth,start,cycles,every", [(2, 16, 7, 2, 4, 5), (2, 8, 7, 2, 5, 4), (3, 4, 5, 1, 4, 3)])
+@pytest.mark.parametrize("rank,n,depth,start,cycles,every", [(2, 16, 7, 2, 4, 5), (2, 8, 7, 2, 5, 4), (3, 4, 5, 1, 4, 3),
+                                                              (2, 64, 7, 2, 3, 3), (3, 8, 5, 1, 3, 3)])
 def test_euler_adaptive(amr, rank, n, depth, start, cycles, every):
     """Regrids by the density criterion (max value, refine > 0.53, coarsen <
     0.49) between batches, as the reference's Euler drivers do."""
