@@ -533,6 +533,7 @@ struct run_lists
     uint32_t*       block_sums = nullptr;
     regrid_summary* summary    = nullptr;
     regrid_summary* h_summary  = nullptr; // pinned, one per list
+    bool            valid      = false;   // built for the current table
 
     run_lists() = default;
     run_lists(run_lists const&) = delete;
@@ -558,9 +559,12 @@ struct run_lists
         AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_summary), kRunLists * sizeof(regrid_summary), 0));
     }
 
+    // The lists for a step launch; none while they are stale (a run variant
+    // then runs the wave-per-leaf kernel).
     step_aux view() const
     {
         step_aux v;
+        if (!valid) return v;
         for (int k = 0; k < kRunLists; ++k)
         {
             v.list[k]  = list[k];
@@ -594,6 +598,7 @@ struct run_lists
         AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, kRunLists * sizeof(regrid_summary), hipMemcpyDeviceToHost, s));
         AMR_HIP_CHECK(hipStreamSynchronize(s));
         for (int k = 0; k < kRunLists; ++k) count[k] = h_summary[k].total;
+        valid = true;
     }
 };
 
@@ -761,6 +766,18 @@ struct amr_ctx
             throw amr_error(AMR_ERR_TOPOLOGY, "leaves do not tile the domain or violate 2:1 face balance");
         }
         if (after_regrid) max_level = static_cast<int>(h_summary->max_level);
+        // the run lists are rebuilt when a step or the autotuner next needs
+        // them (most trees run the wave-per-leaf kernel, which does not)
+        runs.valid = false;
+    }
+
+    // Builds the run lists of the current table if `v` (or, v < 0, any
+    // variant) needs them and they are stale.  Synchronizes the stream: not
+    // for use inside a graph capture.
+    void ensure_runs(int v)
+    {
+        if (runs.valid || !runs.flag) return;
+        if (v >= 0 && !(op->variant_needs(v) & NEEDS_RUNS)) return;
         runs.build(rank, keys[kc], static_cast<uint32_t>(n), depth, links, metas, stream);
     }
 
@@ -816,6 +833,7 @@ struct amr_ctx
     void step(double dt)
     {
         ensure_usable();
+        ensure_runs(variant);
         step_params prm = make_params(dt);
         prm.reverse     = static_cast<uint32_t>(sweeps++ & 1);
         op->step(*op, variant, u[cur], u[1 - cur], keys[kc], links, metas, prm, nullptr, nullptr, static_cast<uint32_t>(n),
@@ -828,6 +846,7 @@ struct amr_ctx
     int autotune(uint32_t trials, float* best_us)
     {
         ensure_usable();
+        ensure_runs(-1);
         step_params prm = make_params(compute_dt());
         variant         = autotune_steps(
             op, u[cur], u[1 - cur], n, ni,
