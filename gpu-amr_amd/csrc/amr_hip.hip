@@ -96,8 +96,11 @@ struct ops
     void (*padded)(ops const&, const double*, double*, const face_link*, const uint32_t*, uint32_t, hipStream_t);
     void (*criterion)(ops const&, int kind, const double*, const uint64_t*, const face_link*, const uint32_t*, double,
                       double, int, int, int8_t*, uint32_t, hipStream_t);
+    // emit(..., guard_cap, stream): guard_cap 0 emits unconditionally; else
+    // the emit checks the planned regrid on the device (regrid_applies)
     void (*emit)(ops const&, const uint64_t*, const double*, uint32_t, int, const uint8_t*, const uint8_t*,
-                 const uint32_t*, const uint32_t*, uint64_t*, double*, regrid_summary*, int, size_t, hipStream_t);
+                 const uint32_t*, const uint32_t*, uint64_t*, double*, regrid_summary*, int, size_t, uint32_t,
+                 hipStream_t);
     void (*adj_step)(ops const&, const double*, const double*, const uint64_t*, const face_link*, const uint32_t*,
                      const adjoint_params*, double*, double*, uint32_t, hipStream_t);
     void (*adj_regrid)(ops const&, const double*, const uint8_t*, const uint8_t*, const uint32_t*, const uint32_t*,
@@ -215,11 +218,12 @@ struct instance
     }
     static void emit(ops const&, const uint64_t* keys, const double* u, uint32_t n, int depth, const uint8_t* mark,
                      const uint8_t* head, const uint32_t* counts, const uint32_t* pos, uint64_t* keys_out,
-                     double* u_out, regrid_summary* summary, int nfields, size_t fstride, hipStream_t s)
+                     double* u_out, regrid_summary* summary, int nfields, size_t fstride, uint32_t guard_cap,
+                     hipStream_t s)
     {
         if (n == 0) return;
         hipLaunchKernelGGL((k_emit<R, N, NT>), dim3(xcd_grid(n)), dim3(NT), 0, s, keys, u, n, depth, mark, head, counts,
-                           pos, keys_out, u_out, summary, nfields, fstride);
+                           pos, keys_out, u_out, summary, nfields, fstride, guard_cap);
         check_launch("k_emit");
     }
     static void adj_step(ops const&, const double* lam_next, const double* u, const uint64_t* keys, const face_link* links,
@@ -380,11 +384,12 @@ struct box_instance
     }
     static void emit(ops const& o, const uint64_t* keys, const double* u, uint32_t n, int depth, const uint8_t* mark,
                      const uint8_t* head, const uint32_t* counts, const uint32_t* pos, uint64_t* keys_out,
-                     double* u_out, regrid_summary* summary, int nfields, size_t fstride, hipStream_t s)
+                     double* u_out, regrid_summary* summary, int nfields, size_t fstride, uint32_t guard_cap,
+                     hipStream_t s)
     {
         if (n == 0) return;
         hipLaunchKernelGGL((k_emit_box<R, 256>), dim3(xcd_grid(n)), dim3(256), 0, s, shape(o), keys, u, n, depth, mark,
-                           head, counts, pos, keys_out, u_out, summary, nfields, fstride);
+                           head, counts, pos, keys_out, u_out, summary, nfields, fstride, guard_cap);
         check_launch("k_emit_box");
     }
     static void adj_step(ops const& o, const double* lam_next, const double* u, const uint64_t* keys,
@@ -743,19 +748,18 @@ struct amr_ctx
 
     void sync() { AMR_HIP_CHECK(hipStreamSynchronize(stream)); }
 
-    // Neighbour records of the current leaves.  After a regrid the summary
-    // still holds the emit's deepest level (and no error bits: the regrid
-    // stops before the emit otherwise), so one read-back serves both.
-    void build_table(bool after_regrid = false)
+    // Neighbour records of the current leaves (a regrid builds its new table
+    // itself, see regrid).
+    void build_table()
     {
-        if (!after_regrid) AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
+        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
         const uint64_t items = n * static_cast<uint64_t>(ndir);
         if (rank == 2)
             hipLaunchKernelGGL(k_build_table<2>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[kc],
-                               static_cast<uint32_t>(n), depth, links, metas, summary);
+                               static_cast<uint32_t>(n), depth, links, metas, summary, 0u);
         else
             hipLaunchKernelGGL(k_build_table<3>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[kc],
-                               static_cast<uint32_t>(n), depth, links, metas, summary);
+                               static_cast<uint32_t>(n), depth, links, metas, summary, 0u);
         check_launch("k_build_table");
         AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
         sync();
@@ -765,7 +769,6 @@ struct amr_ctx
             broken = true;
             throw amr_error(AMR_ERR_TOPOLOGY, "leaves do not tile the domain or violate 2:1 face balance");
         }
-        if (after_regrid) max_level = static_cast<int>(h_summary->max_level);
         // the run lists are rebuilt when a step or the autotuner next needs
         // them (most trees run the wave-per-leaf kernel, which does not)
         runs.valid = false;
@@ -952,15 +955,22 @@ struct amr_ctx
     }
 
     // ndtree::reconstruct_tree (ndtree.hpp:1249-1271) with the flags already
-    // in device memory.
+    // in device memory.  The plan, the emit and the new table are queued
+    // back to back: the emit and the table build read the plan's summary on
+    // the device and do nothing unless the regrid changes the tree, asks for
+    // no root coarsening and fits the capacity (regrid_applies), so the host
+    // reads the outcome once, after all of them.  The links and metas are
+    // overwritten only when the regrid applies; the state and keys are
+    // written to the other buffers, which become current then.
     bool regrid()
     {
         ensure_usable();
-        const uint32_t nn = static_cast<uint32_t>(n);
-        AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
+        const uint32_t nn  = static_cast<uint32_t>(n);
+        const uint32_t cap32 = static_cast<uint32_t>(std::min<uint64_t>(cap, 0xffffffffu));
         if (nn <= static_cast<uint32_t>(kPlanMaxLeaves))
         {
-            // small tree: every planning pass in one workgroup
+            // small tree: every planning pass in one workgroup, which also
+            // clears the summary
             const int top = std::min(max_level, depth);
             if (rank == 2)
                 hipLaunchKernelGGL(k_regrid_plan<2>, dim3(1), dim3(kPlanThreads), 0, stream, flags, keys[kc], nn,
@@ -971,19 +981,39 @@ struct amr_ctx
             check_launch("k_regrid_plan");
         }
         else
+        {
+            AMR_HIP_CHECK(hipMemsetAsync(summary, 0, sizeof(regrid_summary), stream));
             plan_regrid(nn);
+        }
+        op->emit(*op, keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, nvar,
+                 fstride(), cap32, stream);
+        // the new table: at most 2^R leaves per old leaf, and no more than the capacity
+        const uint64_t most  = std::min<uint64_t>(static_cast<uint64_t>(nn) << rank, cap);
+        const uint64_t items = most * static_cast<uint64_t>(ndir);
+        if (rank == 2)
+            hipLaunchKernelGGL(k_build_table<2>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[1 - kc],
+                               static_cast<uint32_t>(most), depth, links, metas, summary, cap32);
+        else
+            hipLaunchKernelGGL(k_build_table<3>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[1 - kc],
+                               static_cast<uint32_t>(most), depth, links, metas, summary, cap32);
+        check_launch("k_build_table");
         AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
         sync();
         const regrid_summary s = *h_summary;
         if (s.err & 2u) throw amr_error(AMR_ERR_UNSUPPORTED, "coarsening to the root level is not supported");
         if (s.n_refine == 0 && s.n_coarsen == 0) return false;
         if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
-        op->emit(*op, keys[kc], u[cur], nn, depth, mark, head, counts, pos, keys[1 - kc], u[1 - cur], summary, nvar,
-                 fstride(), stream);
         cur = 1 - cur;
         kc  = 1 - kc;
         n   = s.total;
-        build_table(true);
+        runs.valid = false;
+        if (s.err & 1u)
+        {
+            // the table may hold -1 entries: no kernel may run on it again
+            broken = true;
+            throw amr_error(AMR_ERR_TOPOLOGY, "leaves do not tile the domain or violate 2:1 face balance");
+        }
+        max_level = static_cast<int>(s.max_level);
         return true;
     }
 };
@@ -1382,7 +1412,7 @@ struct amr_tape
     void replay_transfer(regrid_record const& r, const double* src, double* dst)
     {
         op->emit(*op, topos[static_cast<size_t>(r.topo_before)].keys, src, static_cast<uint32_t>(r.n_old), depth, r.mark,
-                 r.head, r.counts, r.pos, scratch_keys, dst, scratch_summary, 1, 0, stream);
+                 r.head, r.counts, r.pos, scratch_keys, dst, scratch_summary, 1, 0, 0u, stream);
     }
 
     void forward_step(uint64_t k, const double* src, double* dst)

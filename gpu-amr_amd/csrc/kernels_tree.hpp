@@ -36,6 +36,16 @@ struct regrid_summary
     uint32_t pad[3];
 };
 
+// Whether a planned regrid changes the tree and may be carried out: it
+// refines or coarsens something, asks for no root coarsening and fits the
+// capacity.  Read by the emit and the table build on the device, so that
+// the host learns the outcome from one read-back after both.
+__device__ inline bool regrid_applies(const regrid_summary* summary, uint32_t cap)
+{
+    const regrid_summary s = *summary;
+    return !(s.err & 2u) && s.total <= cap && (s.n_refine | s.n_coarsen) != 0;
+}
+
 // Index of the leaf containing integer point p (resolution 2^depth) among the
 // sorted keys, or -1 when no leaf contains it.
 template <int R>
@@ -303,8 +313,16 @@ __host__ __device__ inline double restrict_value(const double* __restrict__ u, u
 
 template <int R>
 __global__ void k_build_table(const uint64_t* __restrict__ keys, uint32_t n, int depth, face_link* __restrict__ links,
-                              uint32_t* __restrict__ metas, regrid_summary* __restrict__ summary)
+                              uint32_t* __restrict__ metas, regrid_summary* __restrict__ summary, uint32_t guard_cap)
 {
+    // after a regrid planned on the device (guard_cap = the capacity): the
+    // leaf count is the plan's, and nothing is built when the regrid leaves
+    // the tree alone or stops with an error (see regrid_applies)
+    if (guard_cap)
+    {
+        if (!regrid_applies(summary, guard_cap)) return;
+        n = summary->total;
+    }
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * 2 * R) return;
     face_link L;
@@ -456,6 +474,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_regrid_plan(
     uint32_t* __restrict__ counts, uint32_t* __restrict__ pos, regrid_summary* __restrict__ summary)
 {
     const uint32_t t = threadIdx.x;
+    if (t == 0) *summary = regrid_summary{}; // the counts below and the emit's deepest level start from 0
     for (uint32_t i = t; i < n; i += kPlanThreads) mark[i] = refine_mark(flags[i], keys[i], depth);
     __syncthreads();
     for (int l = top_level; l >= 1; --l)
@@ -598,10 +617,11 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ keys, 
                                              const uint8_t* __restrict__ head, const uint32_t* __restrict__ counts,
                                              const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys_out,
                                              double* __restrict__ u_out, regrid_summary* __restrict__ summary,
-                                             int nfields, size_t fstride)
+                                             int nfields, size_t fstride, uint32_t guard_cap)
 {
     constexpr box<R> B = box<R>::cube(N);
-    const uint32_t   i = xcd_item(blockIdx.x, gridDim.x);
+    if (guard_cap && !regrid_applies(summary, guard_cap)) return;
+    const uint32_t i = xcd_item(blockIdx.x, gridDim.x);
     if (i >= n) return;
     emit_leaf<R, NT>(B, keys, u, i, depth, mark, head, counts, pos, keys_out, u_out, summary, nfields, fstride);
 }
@@ -614,8 +634,9 @@ __global__ __launch_bounds__(NT) void k_emit_box(box<R> B, const uint64_t* __res
                                                  const uint32_t* __restrict__ counts,
                                                  const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys_out,
                                                  double* __restrict__ u_out, regrid_summary* __restrict__ summary,
-                                                 int nfields, size_t fstride)
+                                                 int nfields, size_t fstride, uint32_t guard_cap)
 {
+    if (guard_cap && !regrid_applies(summary, guard_cap)) return;
     const uint32_t i = xcd_item(blockIdx.x, gridDim.x);
     if (i >= n) return;
     emit_leaf<R, NT>(B, keys, u, i, depth, mark, head, counts, pos, keys_out, u_out, summary, nfields, fstride);
