@@ -773,8 +773,9 @@ struct euler_cell2
 // the +y flux of a row is the -y flux of the next row.  The -x boundary
 // fluxes and the +x ghost cells of the strip's rows are formed first, one
 // row per lane.  The fluxes and states come from the same operands by the
-// same expressions as in k_euler_step_tile, so the results are the same
-// bits.
+// same expressions as in k_euler_step_tile (the update of time_step_cpu,
+// amr_solver.hpp:265-353, with rusanovFlux, EulerPhysics.hpp:72-125), so the
+// results are the same bits.
 template <int B>
 __global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict__ u, double* __restrict__ un,
                                                         size_t fstride, const uint64_t* __restrict__ keys,
