@@ -258,23 +258,32 @@ __device__ inline void cell_speeds(const double* U, double gamma, double* smax)
 template <int R, int NT>
 __device__ inline void fold_speed_max(double* smax, const double* cs, euler_scalars* __restrict__ sc)
 {
-    __shared__ double part[NT / 64][R];
     for (int d = 0; d < R; ++d)
         for (int off = 32; off > 0; off >>= 1)
         {
             const double o = __shfl_down(smax[d], off, 64);
             smax[d]        = (o > smax[d]) ? o : smax[d];
         }
-    if ((threadIdx.x & 63) == 0)
-        for (int d = 0; d < R; ++d) part[threadIdx.x >> 6][d] = smax[d];
-    __syncthreads();
+    // lane 0 of each wave holds the wave's maxima; a one-wave workgroup
+    // needs no partials in LDS
+    __shared__ double part[NT > 64 ? NT / 64 : 1][R]; // unused (and dropped) when NT == 64
+    if constexpr (NT > 64)
+    {
+        if ((threadIdx.x & 63) == 0)
+            for (int d = 0; d < R; ++d) part[threadIdx.x >> 6][d] = smax[d];
+        __syncthreads();
+    }
     if (threadIdx.x == 0)
     {
         double dt = 1.7976931348623157e308;
         for (int d = 0; d < R; ++d)
         {
-            double sd = part[0][d];
-            for (int w = 1; w < NT / 64; ++w) sd = (part[w][d] > sd) ? part[w][d] : sd;
+            double sd = smax[d];
+            if constexpr (NT > 64)
+            {
+                sd = part[0][d];
+                for (int w = 1; w < NT / 64; ++w) sd = (part[w][d] > sd) ? part[w][d] : sd;
+            }
             if (sd > 1e-12)
             {
                 const double q = cs[d] / sd;
@@ -530,6 +539,17 @@ __global__ __launch_bounds__(NT) void k_euler_step_tile(const double* __restrict
     if (prm.next_dt) fold_speed_max<R, NT>(smax, prm.cs[key_level(keys[leaf])], sc);
 }
 
+// A value every lane holds alike, read from the first active lane into
+// scalar registers (the VALU result of a uniform division otherwise holds
+// vector registers).
+__device__ inline double wave_uniform(double v)
+{
+    const uint64_t b  = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b >> 32));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
 // The five conservative values and the state of one cell moved across
 // lanes as 64-bit DPP rows (lane_plus1) or LDS permutes (__shfl_down by 8),
 // every lane active.
@@ -575,12 +595,17 @@ __device__ inline euler_cell3 cell_lane_down8(euler_cell3 const& c)
 // as in k_euler_step_tile, so the results are the same bits, for about half
 // of its VALU instructions (the tile kernel forms each cell's state seven
 // times and each face's flux twice).
+// The face records are read once; unless a face has finer neighbours every
+// ghost is a single cell and all ghost and boundary-cell loads are issued
+// back to back, the +z ghost in the prefetch slot of plane 8.  Four waves
+// per SIMD: at most 128 VGPRs (z loop unrolled, so the prefetch needs no
+// register copies; the step's dt / dx in scalar registers) and 10 KiB of
+// LDS (the boundary fluxes only).
 template <bool NTS>
-__global__ __launch_bounds__(64) void k_euler_step_3d8(const double* __restrict__ u, double* __restrict__ un,
-                                                       size_t fstride, const uint64_t* __restrict__ keys,
-                                                       const face_link* __restrict__ links,
-                                                       const uint32_t* __restrict__ metas, euler_params prm,
-                                                       euler_scalars* __restrict__ sc, uint32_t nleaves)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_euler_step_3d8(
+    const double* __restrict__ u, double* __restrict__ un, size_t fstride, const uint64_t* __restrict__ keys,
+    const face_link* __restrict__ links, const uint32_t* __restrict__ metas, euler_params prm,
+    euler_scalars* __restrict__ sc, uint32_t nleaves)
 {
     using F            = euler_flux<3>;
     constexpr int NV   = 5;
@@ -600,66 +625,106 @@ __global__ __launch_bounds__(64) void k_euler_step_3d8(const double* __restrict_
     const face_link* L     = links + static_cast<size_t>(leaf) * 6;
     const uint32_t*  M     = metas + static_cast<size_t>(leaf) * 6;
     const double     gamma = prm.gamma;
+    const int        p = lane >> 3, q = lane & 7;
 
-    // ---- boundary fluxes of the x and y faces (face layout p = z, q) -------
+    // The leaf's six face records, read once.
+    uint32_t m[6];
+    int32_t  nb[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
     {
-        const int p = lane >> 3, q = lane & 7;
+        m[k]  = M[k];
+        nb[k] = L[k].ids[0];
+    }
+
+    // ---- ghost cells and the boundary fluxes of the x and y faces ----------
+    // Ghost k (direction k) of the lane: k = 0, 1 the -z / +z ghost of the
+    // column (y, x) = (p, q); k = 2..5 the ghost of face cell (z, x) = (p, q)
+    // of the -y / +y face, (z, y) = (p, q) of the -x / +x face.  bc[f] is
+    // the patch cell on the inner side of side face f = k - 2.
+    const int gc[6][3] = { { -1, p, q }, { 8, p, q }, { p, -1, q }, { p, 8, q }, { p, q, -1 }, { p, q, 8 } };
+    const int bc[4]    = { p * 64 + q, p * 64 + 56 + q, p * 64 + q * 8, p * 64 + q * 8 + 7 };
+    double    gzm[NV]; // the -z ghost (the +z ghost is loaded with the last planes)
+    const bool simple = single_cell_ghosts<3>(m);
+    auto      side_flux = [&](int f, const double* G, const double* C) {
+        const auto sg = F::state_of(G, gamma), sb = F::state_of(C, gamma);
+        double     fl[NV];
+        if (f & 1)
+            F::rusanov(C, sb, G, sg, fl, f < 2 ? 1 : 0);
+        else
+            F::rusanov(G, sg, C, sb, fl, f < 2 ? 1 : 0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fb[f][v][lane] = fl[v];
+    };
+    if (simple)
+    {
+        // every ghost is one cell (same-level copy or coarser injection):
+        // all 45 loads issued back to back
+        double G[6][NV], C[4][NV];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+        {
+            if (k == 1) continue;
+            const size_t src = ghost_source<3, 8>(nb[k], m[k], k, gc[k]);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) G[k][v] = u[v * fstride + src];
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) C[f][v] = u[v * fstride + base + bc[f]];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) gzm[v] = G[0][v];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) side_flux(f, G[2 + f], C[f]);
+    }
+    else
+    {
+        double G[NV];
+        ghost_values<3, 8, NV>(u, fstride, L[0], m[0], 0, gc[0], gzm);
 #pragma unroll
         for (int f = 0; f < 4; ++f)
         {
-            const int  dir = 2 + f;            // -y, +y, -x, +x
-            const bool pos = (dir & 1) != 0;
-            const int  d   = (f < 2) ? 1 : 0;  // physical axis of the face normal
-            int        c[3], cb[3];
-            if (f < 2)
-            {
-                c[0] = p, c[1] = pos ? 8 : -1, c[2] = q;
-                cb[0] = p, cb[1] = pos ? 7 : 0, cb[2] = q;
-            }
-            else
-            {
-                c[0] = p, c[1] = q, c[2] = pos ? 8 : -1;
-                cb[0] = p, cb[1] = q, cb[2] = pos ? 7 : 0;
-            }
-            double G[NV], C[NV], fl[NV];
-            ghost_values<3, 8, NV>(u, fstride, L[dir], M[dir], dir, c, G);
-            const size_t at = base + patch_shape<3, 8>::lin(cb);
+            double C[NV];
+            ghost_values<3, 8, NV>(u, fstride, L[2 + f], m[2 + f], 2 + f, gc[2 + f], G);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) C[v] = u[v * fstride + at];
-            const auto sg = F::state_of(G, gamma), sb = F::state_of(C, gamma);
-            if (pos)
-                F::rusanov(C, sb, G, sg, fl, d);
-            else
-                F::rusanov(G, sg, C, sb, fl, d);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) fb[f][v][lane] = fl[v];
+            for (int v = 0; v < NV; ++v) C[v] = u[v * fstride + base + bc[f]];
+            side_flux(f, G, C);
         }
     }
 
     // ---- the column, z = 0..7 ----------------------------------------------
-    const int lv      = key_level(keys[leaf]);
-    double    dtdx[3] = { dt / prm.cs[lv][0], dt / prm.cs[lv][1], dt / prm.cs[lv][2] };
-    double    smax[3] = { 0.0, 0.0, 0.0 };
+    // each field's cells of the leaf (uniform pointers, the lane's cell
+    // of plane z at [z * 64 + lane])
+    const double* uf[NV];
+    double*       of[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+    {
+        uf[v] = u + v * fstride + base;
+        of[v] = un + v * fstride + base;
+    }
     euler_cell3 cur, nxt;
     double      pre[NV]; // the plane after the next one, loaded an iteration ahead
-    {
-        double G[NV];
-        const int c[3] = { -1, y, x };
-        ghost_values<3, 8, NV>(u, fstride, L[0], M[0], 0, c, G);
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-        {
-            cur.U[v] = u[v * fstride + base + lane];
-            pre[v]   = u[v * fstride + base + 64 + lane];
-        }
+    for (int v = 0; v < NV; ++v)
+    {
+        cur.U[v] = uf[v][lane];
+        pre[v]   = uf[v][64 + lane];
+    }
+    const int lv      = key_level(keys[leaf]);
+    double    dtdx[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dtdx[d] = wave_uniform(dt / prm.cs[lv][d]); // kept in scalar registers
+    double    smax[3] = { 0.0, 0.0, 0.0 };
+    {
         const auto st = F::state_of(cur.U, gamma);
         cur.inv_rho = st.inv_rho, cur.p = st.p, cur.a = st.a;
-        const auto sg = F::state_of(G, gamma);
+        const auto sg = F::state_of(gzm, gamma);
         double     fz[NV];
-        F::rusanov(G, sg, cur.U, st, fz, 2);
+        F::rusanov(gzm, sg, cur.U, st, fz, 2);
         __syncthreads(); // the boundary fluxes
-        double* out = un + base;
-#pragma unroll 1
+#pragma unroll
         for (int z = 0; z < 8; ++z)
         {
             const typename F::state sc_{ cur.inv_rho, cur.p, cur.a };
@@ -704,14 +769,25 @@ __global__ __launch_bounds__(64) void k_euler_step_3d8(const double* __restrict_
 #pragma unroll
                 for (int v = 0; v < NV; ++v) nxt.U[v] = pre[v];
                 if (z < 6)
+                {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) pre[v] = u[v * fstride + base + (z + 2) * 64 + lane];
+                    for (int v = 0; v < NV; ++v) pre[v] = uf[v][(z + 2) * 64 + lane];
+                }
+                else if (simple)
+                {
+                    // plane 8: the +z ghost, a single cell
+                    const size_t src = ghost_source<3, 8>(nb[1], m[1], 1, gc[1]);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) pre[v] = u[v * fstride + src];
+                }
+            }
+            else if (simple)
+            {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) nxt.U[v] = pre[v];
             }
             else
-            {
-                const int c8[3] = { 8, y, x };
-                ghost_values<3, 8, NV>(u, fstride, L[1], M[1], 1, c8, nxt.U);
-            }
+                ghost_values<3, 8, NV>(u, fstride, L[1], m[1], 1, gc[1], nxt.U);
             {
                 const auto st = F::state_of(nxt.U, gamma);
                 nxt.inv_rho = st.inv_rho, nxt.p = st.p, nxt.a = st.a;
@@ -724,7 +800,7 @@ __global__ __launch_bounds__(64) void k_euler_step_3d8(const double* __restrict_
             {
                 total[k] -= dtdx[2] * (fzr[k] - fz[k]);
                 o[k] = cur.U[k] + total[k];
-                store_cell<NTS>(out + k * fstride + z * 64 + lane, o[k]);
+                store_cell<NTS>(of[k] + z * 64 + lane, o[k]);
             }
             if (prm.next_dt) cell_speeds<3>(o, gamma, smax);
 #pragma unroll
