@@ -851,13 +851,13 @@ struct euler_cell2
 // row per lane.  The fluxes and states come from the same operands by the
 // same expressions as in k_euler_step_tile (the update of time_step_cpu,
 // amr_solver.hpp:265-353, with rusanovFlux, EulerPhysics.hpp:72-125), so the
-// results are the same bits.
+// results are the same bits.  Five waves per SIMD (at most 96 VGPRs: rows
+// unrolled, uniform field pointers, dt / dx in scalar registers).
 template <int B>
-__global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict__ u, double* __restrict__ un,
-                                                        size_t fstride, const uint64_t* __restrict__ keys,
-                                                        const face_link* __restrict__ links,
-                                                        const uint32_t* __restrict__ metas, euler_params prm,
-                                                        euler_scalars* __restrict__ sc, uint32_t nleaves)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_euler_step_2d64(
+    const double* __restrict__ u, double* __restrict__ un, size_t fstride, const uint64_t* __restrict__ keys,
+    const face_link* __restrict__ links, const uint32_t* __restrict__ metas, euler_params prm,
+    euler_scalars* __restrict__ sc, uint32_t nleaves)
 {
     static_assert(B >= 2 && 64 % B == 0 && 2 * B <= 64, "strips of at least two rows");
     using F           = euler_flux<2>;
@@ -884,19 +884,29 @@ __global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict
     const face_link* L     = links + static_cast<size_t>(leaf) * 4;
     const uint32_t*  M     = metas + static_cast<size_t>(leaf) * 4;
     const double     gamma = prm.gamma;
-    euler_cell2      cur, nxt;
-    double           pre[NV];   // the row after the next one, loaded an iteration ahead
-    double           below[NV]; // the row under the strip
+    // each field's cells of the leaf (uniform pointers; the lane's cell of
+    // row y at [y * N + x])
+    const double* uf[NV];
+    double*       of[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v)
     {
-        cur.U[v] = u[v * fstride + base + r0 * N + x];
-        pre[v]   = u[v * fstride + base + (r0 + 1) * N + x];
+        uf[v] = u + v * fstride + base;
+        of[v] = un + v * fstride + base;
+    }
+    euler_cell2 cur, nxt;
+    double      pre[NV];   // the row after the next one, loaded an iteration ahead
+    double      below[NV]; // the row under the strip
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+    {
+        cur.U[v] = uf[v][r0 * N + x];
+        pre[v]   = uf[v][(r0 + 1) * N + x];
     }
     if (r0 > 0)
     {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) below[v] = u[v * fstride + base + (r0 - 1) * N + x];
+        for (int v = 0; v < NV; ++v) below[v] = uf[v][(r0 - 1) * N + x];
     }
     else
     {
@@ -933,7 +943,7 @@ __global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict
 
     // ---- the strip's rows ---------------------------------------------------
     const int    lv      = key_level(keys[leaf]);
-    const double dtdx[2] = { dt / prm.cs[lv][0], dt / prm.cs[lv][1] };
+    const double dtdx[2] = { wave_uniform(dt / prm.cs[lv][0]), wave_uniform(dt / prm.cs[lv][1]) };
     double       smax[2] = { 0.0, 0.0 };
     double       fy[NV]; // the -y flux of the current row
     {
@@ -942,8 +952,7 @@ __global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict
         F::rusanov(below, F::state_of(below, gamma), cur.U, st, fy, 1);
     }
     __syncthreads(); // the x boundary values
-    double* out = un + base;
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < B; ++k)
     {
         const int               y = r0 + k;
@@ -976,7 +985,7 @@ __global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict
             if (y + 2 < N)
             {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) pre[v] = u[v * fstride + base + (y + 2) * N + x];
+                for (int v = 0; v < NV; ++v) pre[v] = uf[v][(y + 2) * N + x];
             }
             else
             {
@@ -996,7 +1005,7 @@ __global__ __launch_bounds__(64) void k_euler_step_2d64(const double* __restrict
         {
             total[v] -= dtdx[1] * (fyr[v] - fy[v]);
             o[v] = cur.U[v] + total[v];
-            out[v * fstride + y * N + x] = o[v];
+            of[v][y * N + x] = o[v];
         }
         if (prm.next_dt) cell_speeds<2>(o, gamma, smax);
 #pragma unroll
