@@ -14,6 +14,7 @@
 #include "kernels_tree.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -661,6 +662,34 @@ __global__ void k_fill_instance(uint16_t* __restrict__ inst, uint64_t count, uin
 
 } // namespace
 
+// A context's read-back block in pinned host memory.  k_publish, queued
+// behind the work whose results the host needs, copies the regrid summary
+// and / or the Euler batch scalars into it and then stores the sequence
+// number with a system-scope release, so the host learns the outcome by
+// reading host memory instead of synchronizing the stream.
+struct host_mailbox
+{
+    regrid_summary summary;
+    euler_scalars  esc;
+    uint32_t       seq;
+    uint32_t       pad[3];
+};
+
+__global__ void k_publish(const regrid_summary* __restrict__ s, const euler_scalars* __restrict__ e,
+                          host_mailbox* box, uint32_t seq)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (s) box->summary = *s;
+    if (e) box->esc = *e;
+    __threadfence_system();
+    __hip_atomic_store(&box->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// How long the host spins on the mailbox before it falls back to a stream
+// synchronization: waits for a regrid or a short batch of steps end within
+// it (a spin sees the outcome about 8 us sooner than a synchronization).
+constexpr int64_t kMailboxSpinUs = 200;
+
 struct amr_ctx
 {
     amr_tree_config cfg{};
@@ -690,7 +719,8 @@ struct amr_ctx
     uint32_t*       pos = nullptr;
     uint32_t*       block_sums = nullptr;
     regrid_summary* summary = nullptr;
-    regrid_summary* h_summary = nullptr; // pinned
+    host_mailbox*   box     = nullptr; // pinned read-back block
+    uint32_t        box_seq = 0;
 
     // solver
     double          vel[3] = { 1.0, 0.5, 0.0 };
@@ -708,7 +738,6 @@ struct amr_ctx
     int             nvar  = 1;
     double          gamma = 1.4;
     euler_scalars*  esc   = nullptr; // device scalars of an Euler batch
-    euler_scalars*  h_esc = nullptr; // pinned copy
     size_t          fstride() const { return static_cast<size_t>(cap) * static_cast<size_t>(ni); }
     bool            euler() const { return nvar > 1; }
 
@@ -719,8 +748,7 @@ struct amr_ctx
                          (void*)padded, (void*)flags, (void*)mark, (void*)head, (void*)counts, (void*)pos,
                          (void*)block_sums, (void*)summary, (void*)esc })
             if (p) (void)hipFree(p);
-        if (h_summary) (void)hipHostFree(h_summary);
-        if (h_esc) (void)hipHostFree(h_esc);
+        if (box) (void)hipHostFree(box);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -748,6 +776,29 @@ struct amr_ctx
 
     void sync() { AMR_HIP_CHECK(hipStreamSynchronize(stream)); }
 
+    // Waits until every operation queued on the stream so far has finished
+    // and the mailbox holds the given summary / Euler scalars (either may be
+    // null): spins on the mailbox's sequence number for up to
+    // kMailboxSpinUs, then synchronizes the stream (which also reports a
+    // failure of the queued work).
+    void publish_wait(const regrid_summary* s, const euler_scalars* e)
+    {
+        const uint32_t want = ++box_seq;
+        hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, s, e, box, want);
+        check_launch("k_publish");
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 1;; ++i)
+        {
+            if (__atomic_load_n(&box->seq, __ATOMIC_ACQUIRE) == want) return;
+            if ((i & 255u) == 0u &&
+                std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kMailboxSpinUs))
+                break;
+        }
+        sync();
+        if (__atomic_load_n(&box->seq, __ATOMIC_ACQUIRE) != want)
+            throw amr_error(AMR_ERR_HIP, "the read-back of a finished stream did not arrive");
+    }
+
     // Neighbour records of the current leaves (a regrid builds its new table
     // itself, see regrid).
     void build_table()
@@ -761,9 +812,8 @@ struct amr_ctx
             hipLaunchKernelGGL(k_build_table<3>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[kc],
                                static_cast<uint32_t>(n), depth, links, metas, summary, 0u);
         check_launch("k_build_table");
-        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
-        sync();
-        if (h_summary->err & 1u)
+        publish_wait(summary, nullptr);
+        if (box->summary.err & 1u)
         {
             // the table may hold -1 entries: no kernel may run on it again
             broken = true;
@@ -896,11 +946,11 @@ struct amr_ctx
         }
     }
 
+    // The Euler batch scalars once every queued operation has finished.
     euler_scalars euler_read()
     {
-        AMR_HIP_CHECK(hipMemcpyAsync(h_esc, esc, sizeof(euler_scalars), hipMemcpyDeviceToHost, stream));
-        sync();
-        return *h_esc;
+        publish_wait(nullptr, esc);
+        return box->esc;
     }
 
     // compute_time_step_cpu: cfl * min over cells and dims of dx / speed.
@@ -997,9 +1047,8 @@ struct amr_ctx
             hipLaunchKernelGGL(k_build_table<3>, dim3(grid1d(items, 256)), dim3(256), 0, stream, keys[1 - kc],
                                static_cast<uint32_t>(most), depth, links, metas, summary, cap32);
         check_launch("k_build_table");
-        AMR_HIP_CHECK(hipMemcpyAsync(h_summary, summary, sizeof(regrid_summary), hipMemcpyDeviceToHost, stream));
-        sync();
-        const regrid_summary s = *h_summary;
+        publish_wait(summary, nullptr);
+        const regrid_summary s = box->summary;
         if (s.err & 2u) throw amr_error(AMR_ERR_UNSUPPORTED, "coarsening to the root level is not supported");
         if (s.n_refine == 0 && s.n_coarsen == 0) return false;
         if (s.total > cap) throw amr_error(AMR_ERR_CAPACITY, "regrid would exceed the tree capacity");
@@ -1617,7 +1666,8 @@ static std::unique_ptr<amr_ctx> create_ctx(const amr_tree_config* cfg, int nvar)
     c->pos        = dev_alloc<uint32_t>(cap);
     c->block_sums = dev_alloc<uint32_t>(cap / kScanBlock + 2);
     c->summary    = dev_alloc<regrid_summary>(1);
-    AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_summary), sizeof(regrid_summary), 0));
+    AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->box), sizeof(host_mailbox), 0));
+    std::memset(c->box, 0, sizeof(host_mailbox));
     // the periodic root (ndtree.hpp:411-421)
     const uint64_t root = 0;
     AMR_HIP_CHECK(hipMemcpyAsync(c->keys[0], &root, sizeof(root), hipMemcpyHostToDevice, c->stream));
@@ -1652,7 +1702,6 @@ int amr_euler_create(const amr_tree_config* cfg, double gamma, double cfl, amr_c
             c->gamma = gamma;
             c->cfl   = cfl;
             c->esc   = dev_alloc<euler_scalars>(1);
-            AMR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_esc), sizeof(euler_scalars), 0));
             *out = c.release();
         }
     );
@@ -2120,13 +2169,14 @@ int amr_solver_finish_advance_batch(amr_ctx* ctx, double* dt_sum, uint64_t* exec
         [&]
         {
             if (!ctx) throw amr_error(AMR_ERR_ARGUMENT, "null argument");
-            ctx->sync();
             if (ctx->euler())
             {
                 const euler_scalars e = ctx->euler_read();
                 ctx->pending_dt       = e.acc;
                 ctx->pending_steps    = e.executed;
             }
+            else
+                ctx->publish_wait(nullptr, nullptr);
             if (dt_sum) *dt_sum = ctx->pending_dt;
             if (executed_steps) *executed_steps = ctx->pending_steps;
         }
