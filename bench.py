@@ -504,6 +504,7 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--euler-only", nargs="?", const="all", choices=["all", "3d", "2d", "2d_active"],
                     help="only the Euler benchmarks (profiling): all, or one of them")
+    ap.add_argument("--c2-only", action="store_true", help="only the C2 active-AMR loop (profiling)")
     ap.add_argument("--euler-uniform", type=int, default=0, metavar="STEPS",
                     help="only the 3D Euler step on the uniform 32768-leaf tree (profiling)")
     ap.add_argument("--euler-uniform-2d", type=int, default=0, metavar="STEPS",
@@ -524,7 +525,7 @@ def main():
     cpu_info = host_cpu()
     native = None
     want_cpu = not (args.no_cpu and args.no_c2 and args.no_euler)
-    if want_cpu and not args.euler_only and not args.no_native_oracle and shutil.which("g++"):
+    if want_cpu and not args.euler_only and not args.c2_only and not args.no_native_oracle and shutil.which("g++"):
         if str(ROOT) not in sys.path:
             sys.path.insert(0, str(ROOT))
         import oracle
@@ -541,6 +542,10 @@ def main():
             print(json.dumps({"euler_uniform_3d": euler_uniform_section(amr, args.euler_uniform)}), flush=True)
         if args.euler_uniform_2d:
             print(json.dumps({"euler_uniform_2d": euler_uniform_section(amr, args.euler_uniform_2d, 2)}), flush=True)
+        return
+
+    if args.c2_only:
+        print(json.dumps({"c2": c2_section(amr, cpu_cycles=1)}), flush=True)
         return
 
     if args.euler_only:
