@@ -140,7 +140,8 @@ bool is(int rank, int const* ext, int h)
     X(2, 2, 12, 8)       \
     X(3, 1, 4, 4, 4)     \
     X(3, 1, 8, 8, 8)     \
-    X(3, 1, 4, 6, 8)
+    X(3, 1, 4, 6, 8)     \
+    X(3, 2, 6, 4, 4)
 
 iface* make(int rank, int const* ext, int h, int depth, std::size_t cap, std::array<double, 3> L)
 {

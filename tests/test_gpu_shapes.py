@@ -19,7 +19,8 @@ from test_oracle_tree import random_flags
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(2, (16, 8), 1, 6), (2, (8, 16), 1, 6), (2, (12, 8), 2, 5), (3, (4, 6, 8), 1, 4), (2, (6, 6), 1, 5)]
+SHAPES = [(2, (16, 8), 1, 6), (2, (8, 16), 1, 6), (2, (12, 8), 2, 5), (3, (4, 6, 8), 1, 4), (2, (6, 6), 1, 5),
+          (3, (6, 4, 4), 2, 4)]
 
 
 def say(*a):
