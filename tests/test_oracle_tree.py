@@ -95,10 +95,12 @@ def test_regrid_sequences_3d(seed):
 
 
 # Non-square patches (static_shape<N, M[, O]>): extents per layout dim, the
-# last is physical x; both orientations, a halo-2 shape and a 3D box.
+# last is physical x; both orientations, a halo-2 shape and two 3D boxes
+# (halo 1 and 2).
 @pytest.mark.parametrize(
     "rank,ext,h,depth,start,seed",
-    [(2, (16, 8), 1, 5, 2, 21), (2, (8, 16), 1, 5, 2, 22), (2, (12, 8), 2, 5, 2, 23), (3, (4, 6, 8), 1, 4, 1, 24)],
+    [(2, (16, 8), 1, 5, 2, 21), (2, (8, 16), 1, 5, 2, 22), (2, (12, 8), 2, 5, 2, 23), (3, (4, 6, 8), 1, 4, 1, 24),
+     (3, (6, 4, 4), 2, 4, 2, 25)],
 )
 def test_regrid_sequences_non_square(rank, ext, h, depth, start, seed):
     ch, _ = run_sequence(rank, ext, h, depth, start, 6 if rank == 2 else 4, seed)
