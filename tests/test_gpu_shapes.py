@@ -157,12 +157,13 @@ def euler_ic(rank):
     return ic
 
 
-@pytest.mark.parametrize("rank,ext,depth,start,cycles", [(2, (16, 8), 7, 2, 4), (3, (4, 6, 8), 5, 1, 3)])
-def test_euler_non_square_adaptive(amr, rank, ext, depth, start, cycles):
-    say(f"case: Euler adaptive rank={rank} ext={ext}")
+@pytest.mark.parametrize("rank,ext,h,depth,start,cycles",
+                         [(2, (16, 8), 1, 7, 2, 4), (3, (4, 6, 8), 1, 5, 1, 3), (3, (6, 4, 4), 2, 5, 1, 3)])
+def test_euler_non_square_adaptive(amr, rank, ext, h, depth, start, cycles):
+    say(f"case: Euler adaptive rank={rank} ext={ext} h={h}")
     cap = (1 << (rank * min(depth, start + cycles))) + 64
-    s = amr.EulerSolver(rank, depth, ext, 1, cap)
-    o = OracleEuler(rank, ext, 1, depth, 2 * cap + 64)
+    s = amr.EulerSolver(rank, depth, ext, h, cap)
+    o = OracleEuler(rank, ext, h, depth, 2 * cap + 64)
     o.set_params(1.4, 0.3, threads=16)
     for _ in range(start):
         s.tree.refine_all()
