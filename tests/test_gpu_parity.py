@@ -570,3 +570,50 @@ def test_swap_buffers_semantics(amr):
     g.swap_buffers()
     assert np.array_equal(g.get_interior().view(np.uint64), u2.view(np.uint64))
     g.close()
+
+
+@pytest.mark.parametrize("rank,n,depth", [(2, 16, 16), (3, 8, 16), (3, 4, 16)])
+def test_deep_graded_tree(amr, rank, n, depth):
+    """A tree refined towards one point down to the deepest level the key
+    format holds (16): keys and tables after every regrid, then halos
+    (against the oracle and the geometric brute force) and steps with
+    16 levels of cell sizes in one tree."""
+    say(f"case: graded tree to level {depth} rank={rank} n={n}")
+    cap = 16384
+    vel = (1.0, -0.5, 0.25)[:rank]
+    g, s, o = pair(amr, rank, n, 1, depth, cap, vel, 0.4)
+    g.refine_all()
+    o.refine_all()
+    point = np.array([0.3141592653, 0.2718281828, 0.1618033988][:rank])
+    at = np.floor(point * (1 << depth)).astype(np.int64)
+    for lv in range(2, depth + 1):
+        keys = o.keys()
+        flags = np.zeros(len(keys), dtype=np.int8)
+        for i, k in enumerate(keys):
+            c, level = brute.decode(k, rank)
+            size = 1 << (depth - level)
+            if all(c[d] <= at[d] < c[d] + size for d in range(rank)):
+                flags[i] = 1
+        assert flags.sum() == 1
+        assert g.reconstruct_tree(flags) == o.reconstruct(flags), f"level {lv}: changed flag"
+        assert_topology(g, o, f"level {lv}")
+    levels = sorted({int(k) & 63 for k in o.keys()})
+    say(f"leaves {g.size()}, levels {levels[0]}..{levels[-1]}")
+    assert levels[-1] == depth
+    rng = np.random.default_rng(depth + rank)
+    v = rng.standard_normal(o.size() * o.ni)
+    g.set_interior(v)
+    o.set_interior(v)
+    o.halo_exchange()
+    gp, op = g.get_patches(), o.get_padded()
+    eh = brute.expected_halo(o.keys(), o.get_interior(), rank, n, 1, depth)
+    m = ~np.isnan(eh)
+    assert_fields(gp[m], op[m], "halos")
+    assert_fields(gp[m], eh[m], "halos vs geometry")
+    assert s.compute_dt() == o.compute_dt()
+    s.advance_batch_async(4)
+    tot, ex = s.finish_advance_batch()
+    otot, oex = o.advance_batch(4)
+    assert ex == oex == 4 and tot == otot
+    assert_fields(g.get_interior(), o.get_interior(), "4 steps")
+    g.close()
