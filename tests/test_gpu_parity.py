@@ -349,6 +349,29 @@ def test_capacity_and_root_errors(amr):
     with pytest.raises(amr.AmrError) as e:
         g.reconstruct_tree(np.full(4, 2, dtype=np.int8))
     assert e.value.code == amr.AMR_ERR_UNSUPPORTED
+    # both refusals left the tree as it was: it still steps, like the oracle
+    o = OracleTree(2, 16, 1, 7, 64)
+    o.refine_all()
+    np.testing.assert_array_equal(g.keys(), o.keys())
+    s = amr.AmrSolver(g, (1.0, 0.5), 0.4)
+    o.set_params((1.0, 0.5), 0.4, threads=1)
+    v = np.random.default_rng(3).standard_normal(o.size() * o.ni)
+    g.set_interior(v)
+    o.set_interior(v)
+    o.halo_exchange()  # the oracle's step reads the halos of the last exchange
+    s.advance_batch_async(3)
+    assert s.finish_advance_batch() == o.advance_batch(3)
+    assert_fields(g.get_interior(), o.get_interior(), "steps after the refusals")
+    g.close()
+    # a regrid that fills the capacity exactly is accepted
+    g2 = amr.NdTree(2, 7, 16, 1, 16)
+    g2.refine_all()
+    g2.refine_all()
+    assert g2.size() == 16
+    s2 = amr.AmrSolver(g2, (1.0, 0.5), 0.4)
+    s2.advance_batch_async(2)
+    assert s2.finish_advance_batch()[1] == 2
+    g2.close()
     # any even patch extent runs (12^2 has no specialised kernels); an odd
     # one cannot be split between children
     amr.NdTree(2, 7, 12, 1, 10).close()
