@@ -131,6 +131,8 @@ def test_euler_remaining_time(amr):
     before = s.tree.get_interior().copy()
     s.advance_batch_async(4, 0.0)
     assert s.finish_advance_batch() == (0.0, 0)
+    s.advance_batch_async(0)
+    assert s.finish_advance_batch() == (0.0, 0)
     assert np.array_equal(s.tree.get_interior().view(np.uint64), before.view(np.uint64))
 
 

@@ -371,6 +371,13 @@ def test_capacity_and_root_errors(amr):
     s2 = amr.AmrSolver(g2, (1.0, 0.5), 0.4)
     s2.advance_batch_async(2)
     assert s2.finish_advance_batch()[1] == 2
+    # empty batches: no steps, no time
+    before = g2.get_interior().copy()
+    s2.advance_batch_async(0)
+    assert s2.finish_advance_batch() == (0.0, 0)
+    s2.advance_batch_async(5, 0.0)
+    assert s2.finish_advance_batch() == (0.0, 0)
+    assert np.array_equal(g2.get_interior().view(np.uint64), before.view(np.uint64))
     g2.close()
     # any even patch extent runs (12^2 has no specialised kernels); an odd
     # one cannot be split between children
